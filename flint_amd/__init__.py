@@ -1,0 +1,187 @@
+"""flint_amd -- MI355X implementation of FLINT's fft_small multiplication path.
+
+Python view of the C ABI declared in include/flint_amd.h (libflint_amd.so,
+built in-tree by flint_amd/build.py).  The functions mirror FLINT's
+operator interface for the path (names and argument meaning follow
+src/fft_small.h:850-865); data is passed as numpy uint64 arrays for the
+host-pointer entry points and as raw device addresses for the _device
+variants.  There is no CPU fallback: every call runs on the GPU and an
+error is raised when the library or the device is unavailable.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libflint_amd.so")
+
+_u64 = ctypes.c_uint64
+_s64 = ctypes.c_int64
+_P64 = ctypes.POINTER(ctypes.c_uint64)
+_PD = ctypes.POINTER(ctypes.c_double)
+_VP = ctypes.c_void_p
+
+FA_OK = 0
+
+
+class FlintAmdError(RuntimeError):
+    pass
+
+
+class NmodT(ctypes.Structure):
+    """FLINT's nmod_t (flint.h.in:441-447)."""
+    _fields_ = [("n", _u64), ("ninv", _u64), ("norm", _u64)]
+
+
+class SdFftCtx(ctypes.Structure):
+    """fa_sd_fft_ctx_struct (include/flint_amd.h)."""
+    _fields_ = [("p", ctypes.c_double), ("pinv", ctypes.c_double), ("mod", NmodT),
+                ("primitive_2power_root", _u64), ("depth", ctypes.c_int32),
+                ("slot", ctypes.c_int32)]
+
+
+# exported symbols and their signatures: (restype, argtypes)
+SIGNATURES = {
+    "mpn_mul_default_mpn_ctx": (None, [_P64, _P64, _s64, _P64, _s64]),
+    "_nmod_poly_mul_mid_default_mpn_ctx": (None, [_P64, _s64, _s64, _P64, _s64, _P64, _s64, NmodT]),
+    "fa_mpn_mul": (ctypes.c_int, [_P64, _P64, _s64, _P64, _s64]),
+    "fa_mpn_mul_device": (ctypes.c_int, [_VP, _VP, _s64, _VP, _s64, _VP]),
+    "fa_mpn_plan": (ctypes.c_int, [_s64, _s64, _P64]),
+    "fa_nmod_poly_mul_mid": (ctypes.c_int, [_P64, _s64, _s64, _P64, _s64, _P64, _s64, NmodT]),
+    "fa_fmpz_poly_mul_mid": (ctypes.c_int, [_P64, _u64, _s64, _s64, _P64, _u64, _s64, _P64, _u64,
+                                            _s64]),
+    "fa_sd_fft_ctx_init_prime": (ctypes.c_int, [ctypes.POINTER(SdFftCtx), _u64]),
+    "fa_sd_fft_ctx_clear": (None, [ctypes.POINTER(SdFftCtx)]),
+    "fa_sd_fft_trunc": (ctypes.c_int, [ctypes.POINTER(SdFftCtx), _PD, _u64, _u64, _u64]),
+    "fa_sd_ifft_trunc": (ctypes.c_int, [ctypes.POINTER(SdFftCtx), _PD, _u64, _u64]),
+    "fa_sd_fft_ctx_w": (ctypes.c_double, [ctypes.POINTER(SdFftCtx), _u64]),
+    "fa_last_error": (ctypes.c_char_p, []),
+    "fa_init": (ctypes.c_int, []),
+    "fa_default_primes": (ctypes.c_int, [_P64]),
+    "fa_sync": (ctypes.c_int, []),
+    "fa_stream": (_VP, []),
+    "fa_synth_limbs": (None, [_u64, _P64, _u64, ctypes.c_int]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libflint_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FlintAmdError("libflint_amd.so is not built (run flint_amd/build.py)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise FlintAmdError("flint_amd error %d: %s" % (rc, lib().fa_last_error().decode()))
+    return rc
+
+
+def _p(a):
+    return a.ctypes.data_as(_P64)
+
+
+def _arr(a):
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def default_primes():
+    out = np.zeros(8, dtype=np.uint64)
+    _check(lib().fa_default_primes(_p(out)))
+    return [int(x) for x in out]
+
+
+def synth_limbs(seed, n, top_nonzero=True):
+    a = np.empty(n, dtype=np.uint64)
+    lib().fa_synth_limbs(seed, _p(a), n, 1 if top_nonzero else 0)
+    return a
+
+
+def mpn_plan(an, bn):
+    out = np.zeros(6, dtype=np.int64)
+    _check(lib().fa_mpn_plan(an, bn, out.ctypes.data_as(_P64)))
+    return dict(zip(("np", "bits", "depth", "d1", "d2", "zlen"), (int(x) for x in out)))
+
+
+def mpn_mul(a, b):
+    """mpn_mul_default_mpn_ctx: full product of two limb arrays."""
+    a = _arr(a)
+    b = _arr(b)
+    z = np.empty(len(a) + len(b), dtype=np.uint64)
+    _check(lib().fa_mpn_mul(_p(z), _p(a), len(a), _p(b), len(b)))
+    return z
+
+
+def mpn_mul_device(z_ptr, a_ptr, an, b_ptr, bn, stream=None):
+    """Device-resident product: raw device addresses, enqueued on `stream`."""
+    return _check(lib().fa_mpn_mul_device(z_ptr, a_ptr, an, b_ptr, bn, stream))
+
+
+def nmod_init(n):
+    """nmod_init (nmod.h:284): n, ninv = floor((2^128-1)/(n<<norm)) - 2^64, norm."""
+    n = int(n)
+    norm = 64 - n.bit_length()
+    ninv = ((1 << 128) - 1) // (n << norm) - (1 << 64)
+    return NmodT(n, ninv & ((1 << 64) - 1), norm)
+
+
+def nmod_poly_mul_mid(a, b, zl, zh, n):
+    """_nmod_poly_mul_mid_default_mpn_ctx: coefficients [zl, zh) of a*b mod n."""
+    a = _arr(a)
+    b = _arr(b)
+    z = np.zeros(max(zh - zl, 1), dtype=np.uint64)
+    _check(lib().fa_nmod_poly_mul_mid(_p(z), zl, zh, _p(a), len(a), _p(b), len(b), nmod_init(n)))
+    return z[: max(zh - zl, 0)]
+
+
+def fmpz_poly_mul_mid(a, aw, b, bw, zl, zh, zw):
+    """_fmpz_poly_mul_mid_default_mpn_ctx on w-limb two's complement
+    coefficients.  Returns the (zh-zl)*zw output limbs, or None where the
+    reference declines (more than 8 primes, fmpz_poly_mul.c:377-379)."""
+    a = _arr(a)
+    b = _arr(b)
+    z = np.zeros(max((zh - zl) * zw, 1), dtype=np.uint64)
+    rc = _check(lib().fa_fmpz_poly_mul_mid(_p(z), zw, zl, zh, _p(a), aw, len(a) // aw,
+                                           _p(b), bw, len(b) // bw))
+    return z[: (zh - zl) * zw] if rc == 1 else None
+
+
+class SdFft:
+    """sd_fft_ctx_t with sd_fft_trunc / sd_ifft_trunc (fft_small.h:206-281)."""
+
+    def __init__(self, p):
+        self.ctx = SdFftCtx()
+        _check(lib().fa_sd_fft_ctx_init_prime(ctypes.byref(self.ctx), p))
+
+    def __del__(self):
+        try:
+            lib().fa_sd_fft_ctx_clear(ctypes.byref(self.ctx))
+        except Exception:
+            pass
+
+    def w(self, j):
+        return lib().fa_sd_fft_ctx_w(ctypes.byref(self.ctx), j)
+
+    def fft_trunc(self, d, L, itrunc, otrunc):
+        d = np.ascontiguousarray(d, dtype=np.float64).copy()
+        _check(lib().fa_sd_fft_trunc(ctypes.byref(self.ctx), d.ctypes.data_as(_PD), L, itrunc,
+                                     otrunc))
+        return d
+
+    def ifft_trunc(self, d, L, trunc):
+        d = np.ascontiguousarray(d, dtype=np.float64).copy()
+        _check(lib().fa_sd_ifft_trunc(ctypes.byref(self.ctx), d.ctypes.data_as(_PD), L, trunc))
+        return d

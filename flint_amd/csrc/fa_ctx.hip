@@ -1,0 +1,571 @@
+// fa_ctx.hip -- primes, device twiddle tables, per-thread state, plans.
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+
+#include "fa_arith.h"
+#include "fa_internal.h"
+
+typedef unsigned __int128 u128;
+
+#define DEFAULT_PRIME UINT64_C(0x0003f00000000001)   // default_ctx.c:17
+
+// ------------------------------------------------------------------------
+// errors
+// ------------------------------------------------------------------------
+
+static thread_local char fa_errbuf[512] = "";
+
+int fa_set_error(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(fa_errbuf, sizeof(fa_errbuf), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+extern "C" const char* fa_last_error(void) { return fa_errbuf; }
+
+// ------------------------------------------------------------------------
+// word arithmetic
+// ------------------------------------------------------------------------
+
+uint64_t fa_mulmod_u64(uint64_t a, uint64_t b, uint64_t n)
+{
+    return (uint64_t)(((u128)a * b) % n);
+}
+
+uint64_t fa_powmod_u64(uint64_t a, uint64_t e, uint64_t n)
+{
+    uint64_t r = 1 % n;
+    a %= n;
+    while (e) {
+        if (e & 1) r = fa_mulmod_u64(r, a, n);
+        a = fa_mulmod_u64(a, a, n);
+        e >>= 1;
+    }
+    return r;
+}
+
+double fa_signed_rep(uint64_t x, uint64_t n)
+{
+    x %= n;
+    return (x > n / 2) ? -(double)(n - x) : (double)x;
+}
+
+static int nbits64(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
+
+static int is_prime_u64(uint64_t n)
+{
+    static const uint64_t bases[7] = {2, 325, 9375, 28178, 450775, 9780504, 1795265022};
+    if (n < 2) return 0;
+    for (uint64_t s = 2; s < 64 && s * s <= n; s++)
+        if (n % s == 0) return n == s;
+    uint64_t d = n - 1;
+    int r = 0;
+    while (!(d & 1)) { d >>= 1; r++; }
+    for (int i = 0; i < 7; i++) {
+        uint64_t a = bases[i] % n;
+        if (!a) continue;
+        uint64_t x = fa_powmod_u64(a, d, n);
+        if (x == 1 || x == n - 1) continue;
+        int comp = 1;
+        for (int j = 1; j < r && comp; j++) {
+            x = fa_mulmod_u64(x, x, n);
+            if (x == n - 1) comp = 0;
+        }
+        if (comp) return 0;
+    }
+    return 1;
+}
+
+// mpn_mul.c:1000-1016
+static uint64_t next_fft_number(uint64_t p)
+{
+    int bits = nbits64(p);
+    int l = __builtin_ctzll(p - 1);
+    uint64_t q = p - (UINT64_C(2) << l);
+    if (nbits64(q) == bits) return q;
+    if (l < 5) return (UINT64_C(1) << (bits - 2)) + 1;
+    return (UINT64_C(1) << bits) - (UINT64_C(1) << (l - 1)) + 1;
+}
+
+static int jacobi_u64(uint64_t a, uint64_t n)
+{
+    int t = 1;
+    a %= n;
+    while (a) {
+        while (!(a & 1)) {
+            a >>= 1;
+            uint64_t r = n & 7;
+            if (r == 3 || r == 5) t = -t;
+        }
+        uint64_t tmp = a; a = n; n = tmp;
+        if ((a & 3) == 3 && (n & 3) == 3) t = -t;
+        a %= n;
+    }
+    return n == 1 ? t : 0;
+}
+
+// ulong_extras/primitive_root_prime.c:15-26 then sd_fft_ctx.c:32-37
+static uint64_t primitive_2power_root(uint64_t p)
+{
+    uint64_t a = 2;
+    if (!((p & 7) == 3 || (p & 7) == 5))
+        for (a = 3; jacobi_u64(a, p) != -1; a += 2) {}
+    int v = __builtin_ctzll(p - 1);
+    return fa_powmod_u64(a, (p - 1) >> v, p);
+}
+
+static void default_primes(uint64_t* out)
+{
+    uint64_t p = DEFAULT_PRIME;
+    for (int i = 0; i < FA_NPRIMES; i++) {
+        if (i > 0) p = next_fft_number(p);
+        while (!is_prime_u64(p)) p = next_fft_number(p);
+        out[i] = p;
+    }
+}
+
+extern "C" int fa_default_primes(uint64_t* out8)
+{
+    default_primes(out8);
+    return FA_OK;
+}
+
+// ------------------------------------------------------------------------
+// device tables
+// ------------------------------------------------------------------------
+
+// w2[q] = hi[q >> 12] * lo[q & 4095] reduced to [-n/2, n/2]; q = (h<<12)+l
+// has disjoint bits, so w[2q] = w[2(h<<12)] w[2l] (theta is additive).
+__global__ void fa_expand_twiddles(double* __restrict__ out, const double* __restrict__ lo,
+                                   const double* __restrict__ hi, uint64_t count, double n,
+                                   double ninv)
+{
+    uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    double r = fa_mulmod(hi[q >> 12], lo[q & 4095], n, ninv);
+    double h = 0.5 * (n - 1.0);
+    if (r > h) r -= n;
+    else if (r < -h) r += n;
+    out[q] = r;
+}
+
+static std::mutex fa_table_mutex;
+static fa_tables* fa_table_cache[16];
+
+#define HIPCHECK(x)                                                                     \
+    do {                                                                                \
+        hipError_t e_ = (x);                                                            \
+        if (e_ != hipSuccess)                                                           \
+            return fa_set_error(FA_E_HIP, "%s failed: %s", #x, hipGetErrorString(e_));  \
+    } while (0)
+
+static int check_device(int dev)
+{
+    hipDeviceProp_t prop;
+    HIPCHECK(hipGetDeviceProperties(&prop, dev));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fa_set_error(FA_E_NODEV, "device %d is %s, this build targets gfx950", dev,
+                            prop.gcnArchName);
+    return FA_OK;
+}
+
+static int build_tables(fa_tables* T, int depth)
+{
+    uint64_t P[FA_NPRIMES];
+    default_primes(P);
+    uint64_t count = (depth >= 1) ? (UINT64_C(1) << (depth - 1)) : 1;
+    uint64_t nlo = 4096, nhi = count > 4096 ? count >> 12 : 1;
+    double* hlo = (double*)malloc(nlo * sizeof(double) * 2);
+    double* hhi = (double*)malloc(nhi * sizeof(double) * 2);
+    double *dlo = NULL, *dhi = NULL;
+    int rc = FA_OK;
+    if (hipMalloc(&dlo, nlo * sizeof(double) * 2) != hipSuccess ||
+        hipMalloc(&dhi, nhi * sizeof(double) * 2) != hipSuccess) {
+        rc = fa_set_error(FA_E_NOMEM, "twiddle staging allocation failed");
+        goto done;
+    }
+    for (int i = 0; i < FA_NPRIMES && rc == FA_OK; i++) {
+        fa_prime_host* pr = &T->pr[i];
+        uint64_t n = P[i];
+        pr->n = n;
+        pr->p = (double)n;
+        pr->pinv = 1.0 / pr->p;
+        pr->v = __builtin_ctzll(n - 1);
+        pr->g = primitive_2power_root(n);
+        for (int k = 0; k < 8; k++)
+            pr->two32pow[k] = fa_signed_rep(fa_powmod_u64(2, 32 * (uint64_t)k, n), n);
+        // w2[q] for q < 2^(v-1): w2[2^(j-1)] = w[2^j] = g^(2^(v-j-1))
+        uint64_t gi = fa_powmod_u64(pr->g, n - 2, n);
+        uint64_t* lo = (uint64_t*)malloc(nlo * sizeof(uint64_t));
+        uint64_t* li = (uint64_t*)malloc(nlo * sizeof(uint64_t));
+        lo[0] = li[0] = 1;
+        for (uint64_t j = 1; (UINT64_C(1) << (j - 1)) < nlo; j++) {
+            uint64_t base = UINT64_C(1) << (j - 1);
+            uint64_t root = fa_powmod_u64(pr->g, UINT64_C(1) << (pr->v - j - 1), n);
+            uint64_t rinv = fa_powmod_u64(gi, UINT64_C(1) << (pr->v - j - 1), n);
+            for (uint64_t r = 0; r < base; r++) {
+                lo[base + r] = fa_mulmod_u64(root, lo[r], n);
+                li[base + r] = fa_mulmod_u64(rinv, li[r], n);
+            }
+        }
+        for (uint64_t r = 0; r < nlo; r++) {
+            hlo[r] = fa_signed_rep(lo[r], n);
+            hlo[nlo + r] = fa_signed_rep(li[r], n);
+        }
+        // hi[h] = w2[h << 12] = prod over set bits b of h of w2[2^(b+12)]
+        for (uint64_t h = 0; h < nhi; h++) {
+            uint64_t x = 1, xi = 1;
+            for (int b = 0; b < 40; b++) {
+                if (!((h >> b) & 1)) continue;
+                int j = b + 12 + 1;   // w2[2^(j-1)]
+                x = fa_mulmod_u64(x, fa_powmod_u64(pr->g, UINT64_C(1) << (pr->v - j - 1), n), n);
+                xi = fa_mulmod_u64(xi, fa_powmod_u64(gi, UINT64_C(1) << (pr->v - j - 1), n), n);
+            }
+            hhi[h] = fa_signed_rep(x, n);
+            hhi[nhi + h] = fa_signed_rep(xi, n);
+        }
+        free(lo);
+        free(li);
+        double *w2 = NULL, *w2i = NULL;
+        uint64_t alloc = count > nlo ? count : nlo;
+        if (hipMalloc(&w2, alloc * sizeof(double)) != hipSuccess ||
+            hipMalloc(&w2i, alloc * sizeof(double)) != hipSuccess) {
+            rc = fa_set_error(FA_E_NOMEM, "twiddle table allocation failed");
+            break;
+        }
+        if (hipMemcpy(dlo, hlo, nlo * 2 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dhi, hhi, nhi * 2 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            rc = fa_set_error(FA_E_HIP, "twiddle upload failed");
+            break;
+        }
+        uint64_t cnt = alloc;
+        unsigned grid = (unsigned)((cnt + 255) / 256);
+        hipLaunchKernelGGL(fa_expand_twiddles, dim3(grid), dim3(256), 0, 0, w2, dlo, dhi, cnt,
+                           pr->p, pr->pinv);
+        hipLaunchKernelGGL(fa_expand_twiddles, dim3(grid), dim3(256), 0, 0, w2i, dlo + nlo,
+                           dhi + nhi, cnt, pr->p, pr->pinv);
+        if (hipDeviceSynchronize() != hipSuccess) {
+            rc = fa_set_error(FA_E_HIP, "twiddle expansion failed");
+            break;
+        }
+        if (T->w2[i]) { hipFree(T->w2[i]); hipFree(T->w2inv[i]); }
+        T->w2[i] = w2;
+        T->w2inv[i] = w2i;
+    }
+    if (rc == FA_OK) T->depth = depth;
+done:
+    if (dlo) hipFree(dlo);
+    if (dhi) hipFree(dhi);
+    free(hlo);
+    free(hhi);
+    return rc;
+}
+
+fa_tables* fa_get_tables(int depth)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        fa_set_error(FA_E_NODEV, "hipGetDevice failed");
+        return NULL;
+    }
+    if (dev < 0 || dev >= 16) return NULL;
+    if (depth < 13) depth = 13;
+    if (depth > FA_MAX_DEPTH + 1) {
+        fa_set_error(FA_E_RANGE, "transform depth %d above %d", depth, FA_MAX_DEPTH);
+        return NULL;
+    }
+    std::lock_guard<std::mutex> g(fa_table_mutex);
+    fa_tables* T = fa_table_cache[dev];
+    if (T && T->depth >= depth) return T;
+    if (!T) {
+        if (check_device(dev) != FA_OK) return NULL;
+        T = (fa_tables*)calloc(1, sizeof(fa_tables));
+        T->device = dev;
+    }
+    // grow in steps so that repeated growth stays cheap
+    int want = depth < 20 ? 20 : depth;
+    if (build_tables(T, want) != FA_OK) {
+        if (!fa_table_cache[dev]) free(T);
+        return NULL;
+    }
+    fa_table_cache[dev] = T;
+    return T;
+}
+
+extern "C" int fa_init(void)
+{
+    return fa_get_tables(20) ? FA_OK : FA_E_NODEV;
+}
+
+// ------------------------------------------------------------------------
+// per-thread state
+// ------------------------------------------------------------------------
+
+static thread_local fa_thread_state* fa_tls_state = NULL;
+
+fa_thread_state* fa_state()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        fa_set_error(FA_E_NODEV, "no HIP device");
+        return NULL;
+    }
+    fa_thread_state* st = fa_tls_state;
+    if (st && st->device == dev) return st;
+    st = (fa_thread_state*)calloc(1, sizeof(fa_thread_state));
+    st->device = dev;
+    if (hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking) != hipSuccess) {
+        fa_set_error(FA_E_HIP, "stream creation failed");
+        free(st);
+        return NULL;
+    }
+    fa_tls_state = st;
+    return st;
+}
+
+static int grow(void** buf, size_t* size, size_t bytes, hipStream_t s)
+{
+    if (*size >= bytes) return FA_OK;
+    if (*buf) {
+        hipStreamSynchronize(s);
+        hipFree(*buf);
+        *buf = NULL;
+        *size = 0;
+    }
+    size_t want = bytes + bytes / 8;
+    want = (want + 0xFFFFF) & ~(size_t)0xFFFFF;
+    if (hipMalloc(buf, want) != hipSuccess) {
+        *buf = NULL;
+        return fa_set_error(FA_E_NOMEM, "device allocation of %zu bytes failed", want);
+    }
+    *size = want;
+    return FA_OK;
+}
+
+int fa_scratch_reserve(fa_thread_state* st, size_t bytes, void** out)
+{
+    int rc = grow(&st->scratch, &st->scratch_size, bytes, st->stream);
+    *out = st->scratch;
+    return rc;
+}
+
+int fa_hbuf_reserve(fa_thread_state* st, size_t bytes, void** out)
+{
+    int rc = grow(&st->hbuf, &st->hbuf_size, bytes, st->stream);
+    *out = st->hbuf;
+    return rc;
+}
+
+extern "C" int fa_sync(void)
+{
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    HIPCHECK(hipStreamSynchronize(st->stream));
+    return FA_OK;
+}
+
+extern "C" void* fa_stream(void)
+{
+    fa_thread_state* st = fa_state();
+    return st ? (void*)st->stream : NULL;
+}
+
+// ------------------------------------------------------------------------
+// CRT constants (mpn_mul.c:1093-1137)
+// ------------------------------------------------------------------------
+
+static void ml_mul_1(uint64_t* r, const uint64_t* a, int n, uint64_t b)
+{
+    uint64_t c = 0;
+    for (int i = 0; i < n; i++) {
+        u128 t = (u128)a[i] * b + c;
+        r[i] = (uint64_t)t;
+        c = (uint64_t)(t >> 64);
+    }
+}
+
+static uint64_t ml_divexact_1(uint64_t* r, const uint64_t* a, int n, uint64_t d)
+{
+    u128 rem = 0;
+    for (int i = n - 1; i >= 0; i--) {
+        u128 cur = (rem << 64) | a[i];
+        r[i] = (uint64_t)(cur / d);
+        rem = cur % d;
+    }
+    return (uint64_t)rem;
+}
+
+static uint64_t ml_mod_1(const uint64_t* a, int n, uint64_t d)
+{
+    u128 rem = 0;
+    for (int i = n - 1; i >= 0; i--) rem = ((rem << 64) | a[i]) % d;
+    return (uint64_t)rem;
+}
+
+void fa_crt_consts_init(fa_crt_consts* C, int np, int offset)
+{
+    uint64_t P[FA_NPRIMES];
+    default_primes(P);
+    memset(C, 0, sizeof(*C));
+    C->np = np;
+    C->M[0] = 1;
+    for (int i = 0; i < np; i++) {
+        uint64_t p = P[offset + i];
+        C->p[i] = (double)p;
+        C->pinv[i] = 1.0 / (double)p;
+        C->rp[i] = 1.0 / (double)p;
+        ml_mul_1(C->M, C->M, FA_CRT_MAXL, p);
+    }
+    C->nl = FA_CRT_MAXL;
+    while (C->nl > 1 && C->M[C->nl - 1] == 0) C->nl--;
+    for (int i = 0; i < np; i++) ml_divexact_1(C->cof[i], C->M, FA_CRT_MAXL, P[offset + i]);
+}
+
+uint64_t fa_crt_cop(int np, int i)
+{
+    fa_crt_consts C;
+    uint64_t P[FA_NPRIMES];
+    default_primes(P);
+    fa_crt_consts_init(&C, np, 0);
+    return ml_mod_1(C.cof[i], FA_CRT_MAXL, P[i]);
+}
+
+void fa_prime_set_init(fa_prime_set* S, const fa_tables* T, int np, int depth, int offset,
+                       int with_norm)
+{
+    memset(S, 0, sizeof(*S));
+    fa_crt_consts C;
+    fa_crt_consts_init(&C, np, offset);
+    for (int i = 0; i < np; i++) {
+        const fa_prime_host* pr = &T->pr[offset + i];
+        S->p[i] = pr->p;
+        S->pinv[i] = pr->pinv;
+        for (int k = 0; k < 8; k++) S->two32pow[i][k] = pr->two32pow[k];
+        S->w2[i] = T->w2[offset + i];
+        S->w2inv[i] = T->w2inv[offset + i];
+        if (with_norm) {
+            // m = (cop * 2^depth)^-1 mod p (mpn_mul.c:1517-1518, plan.c:127-149)
+            uint64_t n = pr->n;
+            uint64_t cop = (np == 1) ? 1 : ml_mod_1(C.cof[i], FA_CRT_MAXL, n);
+            uint64_t m = fa_mulmod_u64(cop, fa_powmod_u64(2, (uint64_t)depth, n), n);
+            m = fa_powmod_u64(m, n - 2, n);
+            S->norm[i] = fa_signed_rep(m, n);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// GPU plan for integer products
+// ------------------------------------------------------------------------
+
+static uint64_t cdiv_u64(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+static int clog2_u64(uint64_t x) { return x <= 1 ? 0 : 64 - __builtin_clzll(x - 1); }
+
+// M >= c * 2^e
+static int ml_ge_ui_2exp(const uint64_t* M, uint64_t c, uint64_t e)
+{
+    uint64_t t[FA_CRT_MAXL + 4];
+    memset(t, 0, sizeof(t));
+    uint64_t q = e / 64, r = e % 64;
+    if (q + 2 > FA_CRT_MAXL + 3) return 0;
+    t[q] = c << r;
+    if (r) t[q + 1] = c >> (64 - r);
+    for (int i = FA_CRT_MAXL + 3; i >= 0; i--) {
+        uint64_t mi = (i < FA_CRT_MAXL) ? M[i] : 0;
+        if (mi != t[i]) return mi > t[i];
+    }
+    return 1;
+}
+
+// Chooses (np, bits) with bits a multiple of 8 such that every coefficient
+// of the chunked product, at most min(alen, blen) * 2^(2 bits), stays below
+// M = p_0 ... p_(np-1) (the exactness bound of mpn_mul.c:79-95), minimizing
+// the transform volume np * 2^depth.
+int fa_mpn_choose_plan(uint64_t an, uint64_t bn, fa_mpn_plan_t* P)
+{
+    if (an == 0 || bn == 0) return fa_set_error(FA_E_ARG, "empty operand");
+    double best = -1;
+    for (int np = 2; np <= FA_NPRIMES; np++) {
+        fa_crt_consts C;
+        fa_crt_consts_init(&C, np, 0);
+        int best_bits = 0;
+        for (int bits = 8; bits <= 200; bits += 8) {
+            uint64_t alen = cdiv_u64(64 * an, bits), blen = cdiv_u64(64 * bn, bits);
+            uint64_t mlen = alen < blen ? alen : blen;
+            // the CRT kernel keeps a group's overhang (< M / 2^bits) inside
+            // the next group's bits/8 limbs
+            if (9 * bits < 64 * C.nl + 64) continue;
+            if (ml_ge_ui_2exp(C.M, mlen, 2 * (uint64_t)bits)) best_bits = bits;
+        }
+        if (!best_bits) continue;
+        uint64_t alen = cdiv_u64(64 * an, best_bits), blen = cdiv_u64(64 * bn, best_bits);
+        uint64_t zlen = alen + blen - 1;
+        int depth = clog2_u64(zlen);
+        if (depth < 4) depth = 4;
+        if (depth > FA_MAX_DEPTH) continue;
+        double cost = (double)np * (double)(UINT64_C(1) << depth);
+        if (best < 0 || cost < best) {
+            best = cost;
+            P->np = np;
+            P->bits = best_bits;
+            P->depth = depth;
+            P->alen = alen;
+            P->blen = blen;
+            P->zlen = zlen;
+        }
+    }
+    if (best < 0)
+        return fa_set_error(FA_E_RANGE, "product of %llu x %llu limbs exceeds 2^%d-point plans",
+                            (unsigned long long)an, (unsigned long long)bn, FA_MAX_DEPTH);
+    P->d2 = P->depth <= FA_ROW_LG ? P->depth : FA_ROW_LG;
+    P->d1 = P->depth - P->d2;
+    return FA_OK;
+}
+
+extern "C" int fa_mpn_plan(int64_t an, int64_t bn, int64_t* out)
+{
+    fa_mpn_plan_t P;
+    if (an <= 0 || bn <= 0) return fa_set_error(FA_E_ARG, "an, bn must be >= 1");
+    int rc = fa_mpn_choose_plan((uint64_t)an, (uint64_t)bn, &P);
+    if (rc) return rc;
+    out[0] = P.np; out[1] = P.bits; out[2] = P.depth; out[3] = P.d1; out[4] = P.d2;
+    out[5] = (int64_t)P.zlen;
+    return FA_OK;
+}
+
+// ------------------------------------------------------------------------
+// synthetic inputs
+// ------------------------------------------------------------------------
+
+static uint64_t splitmix64(uint64_t* x)
+{
+    uint64_t z = (*x += UINT64_C(0x9E3779B97F4A7C15));
+    z = (z ^ (z >> 30)) * UINT64_C(0xBF58476D1CE4E5B9);
+    z = (z ^ (z >> 27)) * UINT64_C(0x94D049BB133111EB);
+    return z ^ (z >> 31);
+}
+
+static uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+extern "C" void fa_synth_limbs(uint64_t seed, uint64_t* a, uint64_t n, int top_nonzero)
+{
+    uint64_t s[4], x = seed;
+    for (int i = 0; i < 4; i++) s[i] = splitmix64(&x);
+    auto next = [&]() {
+        uint64_t result = rotl64(s[1] * 5, 7) * 9;
+        uint64_t t = s[1] << 17;
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
+        s[2] ^= t; s[3] = rotl64(s[3], 45);
+        return result;
+    };
+    for (uint64_t i = 0; i < n; i++) a[i] = next();
+    if (top_nonzero && n > 0)
+        while (a[n - 1] == 0) a[n - 1] = next();
+}

@@ -1,0 +1,105 @@
+// fa_internal.h -- host-side structures shared by the flint_amd sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/flint_amd.h"
+
+#define FA_NPRIMES 8
+#define FA_MAX_DEPTH 24          // longest transform: 2^24 points
+#define FA_ROW_LG 12             // row length of the two-level decomposition
+#define FA_LDS_COL_DOUBLES 8192  // LDS budget of one column tile (64 KiB)
+#define FA_CRT_MAXL 8            // limbs of M = prod p_i (np <= 8: < 400 bits)
+
+// one of the 8 default primes with its device tables
+struct fa_prime_host {
+    uint64_t n;
+    double p, pinv;
+    uint64_t g;              // primitive 2^v-th root (sd_fft_ctx.c:32-37)
+    int v;                   // 2-valuation of n-1
+    double two32pow[8];      // 2^(32k) mod n in [-n/2, n/2]
+};
+
+// device-wide tables (one set per device, shared by all host threads)
+struct fa_tables {
+    int device;
+    int depth;                     // w2 holds 2^(depth-1) entries
+    fa_prime_host pr[FA_NPRIMES];
+    double* w2[FA_NPRIMES];        // w2[q] = w[2q]
+    double* w2inv[FA_NPRIMES];     // w2[q]^-1
+};
+
+// CRT constants for np primes (mpn_mul.c:1093-1137)
+struct fa_crt_consts {
+    int np;
+    int nl;                        // limbs of M
+    double p[FA_NPRIMES];
+    double pinv[FA_NPRIMES];
+    double rp[FA_NPRIMES];         // 1/p_i for the quotient estimate
+    uint64_t M[FA_CRT_MAXL];
+    uint64_t cof[FA_NPRIMES][FA_CRT_MAXL];   // M / p_i
+};
+
+// the kernel-side view of the per-prime tables
+struct fa_prime_set {
+    double p[FA_NPRIMES];
+    double pinv[FA_NPRIMES];
+    double norm[FA_NPRIMES];       // pointwise normalizer (cop*2^D)^-1
+    double two32pow[FA_NPRIMES][8];
+    const double* w2[FA_NPRIMES];
+    const double* w2inv[FA_NPRIMES];
+};
+
+// per-host-thread execution state (the analogue of the thread-local
+// default mpn_ctx, default_ctx.c:14-15)
+struct fa_thread_state {
+    int device;
+    hipStream_t stream;
+    void* scratch;
+    size_t scratch_size;
+    void* hbuf;                    // device staging for host-pointer calls
+    size_t hbuf_size;
+};
+
+// mpn plan chosen for the GPU
+struct fa_mpn_plan_t {
+    int np;
+    int bits;            // multiple of 8
+    int depth;           // transform length 2^depth
+    int d1, d2;          // column / row split (d1 = 0: single LDS kernel)
+    uint64_t alen, blen, zlen;
+};
+
+extern "C++" {
+int fa_set_error(int code, const char* fmt, ...);
+fa_tables* fa_get_tables(int depth);
+fa_thread_state* fa_state();
+int fa_scratch_reserve(fa_thread_state* st, size_t bytes, void** out);
+int fa_hbuf_reserve(fa_thread_state* st, size_t bytes, void** out);
+void fa_crt_consts_init(fa_crt_consts* C, int np, int offset);
+uint64_t fa_crt_cop(int np, int i);         // (M/p_i) mod p_i
+int fa_mpn_choose_plan(uint64_t an, uint64_t bn, fa_mpn_plan_t* P);
+void fa_prime_set_init(fa_prime_set* S, const fa_tables* T, int np, int depth, int offset,
+                       int with_norm);
+uint64_t fa_mulmod_u64(uint64_t a, uint64_t b, uint64_t n);
+uint64_t fa_powmod_u64(uint64_t a, uint64_t e, uint64_t n);
+double fa_signed_rep(uint64_t x, uint64_t n);
+
+// convolution core: residues of np primes, transforms, pointwise, inverse.
+// Operands are given through a pack descriptor; the result (np rows of
+// 2^depth doubles, coefficient order) is left in A.
+struct fa_pack_desc {
+    int kind;                  // FA_PACK_MPN, FA_PACK_U64, FA_PACK_S64W
+    const uint64_t* src;
+    uint64_t n;                // limbs (mpn) or coefficients
+    int bits;                  // chunk bits (mpn)
+    int width;                 // limbs per coefficient (signed coefficients)
+};
+enum { FA_PACK_MPN = 0, FA_PACK_U64 = 1, FA_PACK_S64W = 2 };
+
+int fa_convolve(fa_thread_state* st, const fa_tables* T, int np, int offset, int depth,
+                const fa_pack_desc* a, const fa_pack_desc* b, double* A, double* B,
+                hipStream_t stream);
+}

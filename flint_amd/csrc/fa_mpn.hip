@@ -1,0 +1,656 @@
+// fa_mpn.hip -- integer multiplication: limb splitting, transforms,
+// pointwise product, inverse transforms, CRT recombination with a carry
+// scan.  Restates _mpn_ctx_mpn_mul_range (mpn_mul.c:1617-1978) for one GPU.
+//
+// Data layout in HBM for a product with np primes and depth D = d1 + d2:
+//   A[i][x], B[i][x], i < np, x < 2^D doubles (prime-major rows).
+// The transform is done in place in two levels: the first d1 levels act on
+// columns (elements with equal x mod 2^d2, "column passes"), the last d2
+// levels on contiguous rows of 2^d2 elements ("row pass").  After the row
+// pass A holds evaluations in bit-reversed order, i.e. A[i][x] = a(w[x])
+// mod p_i, the layout of sd_fft_trunc without its 4x4 block transposition.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "fa_internal.h"
+#include "fa_ntt.h"
+
+// ------------------------------------------------------------------------
+// limb splitting (mpn_mul.c:595-695, slow_mpn_to_fft :422-481)
+// ------------------------------------------------------------------------
+
+// 32-bit word of the little-endian limb array starting at byte offset
+// `byte` (zero beyond the array)
+FA_DEV uint32_t fa_load_u32_at(const uint32_t* __restrict__ w, uint64_t nwords, uint64_t byte)
+{
+    uint64_t i = byte >> 2;
+    int sh = (int)(byte & 3) * 8;
+    uint32_t lo = (i < nwords) ? w[i] : 0u;
+    if (sh == 0) return lo;
+    uint32_t hi = (i + 1 < nwords) ? w[i + 1] : 0u;
+    return (lo >> sh) | (hi << (32 - sh));
+}
+
+// residue mod p of chunk j = bits [j*bits, (j+1)*bits) of the limb array,
+// bits a multiple of 8; result in [-p/2, p/2]
+FA_DEV double fa_chunk_residue(const uint64_t* __restrict__ limbs, uint64_t nlimbs, int bits,
+                               uint64_t j, const double* __restrict__ two32pow, double n,
+                               double ninv)
+{
+    const uint32_t* w = (const uint32_t*)limbs;
+    uint64_t nwords = 2 * nlimbs;
+    uint64_t start = j * (uint64_t)(bits >> 3);
+    if (start >= 8 * nlimbs) return 0.0;
+    int nbytes = bits >> 3;
+    if (bits <= 48) {
+        uint32_t lo = fa_load_u32_at(w, nwords, start);
+        uint32_t hi = (nbytes > 4) ? fa_load_u32_at(w, nwords, start + 4) : 0u;
+        if (nbytes < 4) lo &= (1u << (8 * nbytes)) - 1u;
+        if (nbytes > 4 && nbytes < 8) hi &= (1u << (8 * (nbytes - 4))) - 1u;
+        return (double)lo + 4294967296.0 * (double)hi;
+    }
+    double acc = 0.0;
+    for (int k = 0; 4 * k < nbytes; k++) {
+        uint32_t v = fa_load_u32_at(w, nwords, start + 4 * (uint64_t)k);
+        int rem = nbytes - 4 * k;
+        if (rem < 4) v &= (1u << (8 * rem)) - 1u;
+        acc += (k == 0) ? (double)v : fa_mulmod((double)v, two32pow[k], n, ninv);
+    }
+    return fa_reduce_pm1n(acc, n, ninv);
+}
+
+// ------------------------------------------------------------------------
+// operand import: the value of element x (coefficient index) of an operand
+// ------------------------------------------------------------------------
+
+struct fa_operand {
+    int kind;                  // FA_PACK_*
+    const uint64_t* src;
+    uint64_t n;                // limbs (mpn) or coefficients
+    int bits;
+    int width;
+};
+
+// signed w-limb two's complement coefficient mod p: Horner over the 32-bit
+// words from the top, minus 2^(64 w) for a negative coefficient
+FA_DEV double fa_coeff_residue_s(const uint64_t* __restrict__ c, int w,
+                                 const double* __restrict__ two32pow, double n, double ninv)
+{
+    const uint32_t* u = (const uint32_t*)c;
+    const double c32 = two32pow[1];
+    double r = 0.0, t = 1.0;
+    for (int k = 2 * w - 1; k >= 0; k--) {
+        r = fa_reduce_pm1n(fa_mulmod(r, c32, n, ninv) + (double)u[k], n, ninv);
+        t = fa_reduce_pm1n(fa_mulmod(t, c32, n, ninv), n, ninv);
+    }
+    if ((int64_t)c[w - 1] < 0) r = fa_reduce_pm1n(r - t, n, ninv);
+    return r;
+}
+
+FA_DEV double fa_operand_value(const fa_operand& op, uint64_t x,
+                               const double* __restrict__ two32pow, double n, double ninv)
+{
+    if (op.kind == FA_PACK_MPN)
+        return fa_chunk_residue(op.src, op.n, op.bits, x, two32pow, n, ninv);
+    if (x >= op.n) return 0.0;
+    if (op.kind == FA_PACK_U64) {
+        uint64_t v = op.src[x];
+        double lo = (double)(uint32_t)v, hi = (double)(uint32_t)(v >> 32);
+        if (v < (UINT64_C(1) << 50)) return (double)v;
+        return fa_reduce_pm1n(lo + fa_mulmod(hi, two32pow[1], n, ninv), n, ninv);
+    }
+    return fa_coeff_residue_s(op.src + x * (uint64_t)op.width, op.width, two32pow, n, ninv);
+}
+
+// ------------------------------------------------------------------------
+// transform kernels
+// ------------------------------------------------------------------------
+
+// depth <= 12: one workgroup per prime does import, both forward
+// transforms, the pointwise product and the inverse transform in LDS.
+__global__ void __launch_bounds__(256) fa_k_small_conv(fa_operand a, fa_operand b, int depth,
+                                                       double* __restrict__ out, uint64_t stride,
+                                                       fa_prime_set S)
+{
+    extern __shared__ double lds[];
+    const int i = blockIdx.x;
+    const double n = S.p[i], ninv = S.pinv[i];
+    const int N = 1 << depth;
+    double* sa = lds;
+    double* sb = lds + N;
+    for (int x = threadIdx.x; x < N; x += blockDim.x) {
+        sa[x] = fa_operand_value(a, (uint64_t)x, S.two32pow[i], n, ninv);
+        sb[x] = fa_operand_value(b, (uint64_t)x, S.two32pow[i], n, ninv);
+    }
+    __syncthreads();
+    fa_lds_fwd(sa, depth, 0, 0, S.w2[i], n, ninv);
+    fa_lds_fwd(sb, depth, 0, 0, S.w2[i], n, ninv);
+    const double m = S.norm[i];
+    for (int x = threadIdx.x; x < N; x += blockDim.x)
+        sa[x] = fa_mulmod(fa_mulmod(sa[x], m, n, ninv), sb[x], n, ninv);
+    __syncthreads();
+    fa_lds_inv(sa, depth, 0, 0, S.w2inv[i], n, ninv);
+    double* o = out + (uint64_t)i * stride;
+    for (int x = threadIdx.x; x < N; x += blockDim.x) o[x] = sa[x];
+}
+
+// first d1 levels on a tile of 2^lgC columns: import + forward.
+// blockIdx.x = column group, blockIdx.y = prime, blockIdx.z = operand (a/b)
+__global__ void __launch_bounds__(256) fa_k_col_fwd(fa_operand a, fa_operand b, int d1, int d2,
+                                                    int lgC, double* __restrict__ A,
+                                                    double* __restrict__ B, uint64_t stride,
+                                                    fa_prime_set S)
+{
+    extern __shared__ double lds[];
+    const int i = blockIdx.y;
+    const double n = S.p[i], ninv = S.pinv[i];
+    const fa_operand& op = blockIdx.z ? b : a;
+    double* dst = (blockIdx.z ? B : A) + (uint64_t)i * stride;
+    const int C = 1 << lgC;
+    const uint64_t col0 = (uint64_t)blockIdx.x << lgC;
+    const int total = (1 << d1) << lgC;
+    for (int u = threadIdx.x; u < total; u += blockDim.x) {
+        uint64_t row = (uint64_t)(u >> lgC), c = (uint64_t)(u & (C - 1));
+        uint64_t x = (row << d2) + col0 + c;
+        lds[u] = fa_operand_value(op, x, S.two32pow[i], n, ninv);
+    }
+    __syncthreads();
+    fa_lds_fwd(lds, d1, lgC, 0, S.w2[i], n, ninv);
+    for (int u = threadIdx.x; u < total; u += blockDim.x) {
+        uint64_t row = (uint64_t)(u >> lgC), c = (uint64_t)(u & (C - 1));
+        dst[(row << d2) + col0 + c] = lds[u];
+    }
+}
+
+// last d2 levels of both operands on row r, pointwise product with the
+// normalizer, then the first d2 inverse levels, in place in A.
+// blockIdx.x = row, blockIdx.y = prime
+__global__ void __launch_bounds__(256) fa_k_row_conv(double* __restrict__ A,
+                                                     const double* __restrict__ B, int d2,
+                                                     uint64_t stride, fa_prime_set S)
+{
+    extern __shared__ double lds[];
+    const int i = blockIdx.y;
+    const double n = S.p[i], ninv = S.pinv[i];
+    const int N2 = 1 << d2;
+    const uint64_t r = blockIdx.x;
+    double* arow = A + (uint64_t)i * stride + (r << d2);
+    const double* brow = B + (uint64_t)i * stride + (r << d2);
+    double* sa = lds;
+    double* sb = lds + N2;
+    for (int x = threadIdx.x; x < N2; x += blockDim.x) {
+        sa[x] = arow[x];
+        sb[x] = brow[x];
+    }
+    __syncthreads();
+    fa_lds_fwd(sa, d2, 0, r, S.w2[i], n, ninv);
+    fa_lds_fwd(sb, d2, 0, r, S.w2[i], n, ninv);
+    const double m = S.norm[i];
+    for (int x = threadIdx.x; x < N2; x += blockDim.x)
+        sa[x] = fa_mulmod(fa_mulmod(sa[x], m, n, ninv), sb[x], n, ninv);
+    __syncthreads();
+    fa_lds_inv(sa, d2, 0, r, S.w2inv[i], n, ninv);
+    for (int x = threadIdx.x; x < N2; x += blockDim.x) arow[x] = sa[x];
+}
+
+// last inverse levels (the first d1 levels of the forward) on columns
+__global__ void __launch_bounds__(256) fa_k_col_inv(double* __restrict__ A, int d1, int d2,
+                                                    int lgC, uint64_t stride, fa_prime_set S)
+{
+    extern __shared__ double lds[];
+    const int i = blockIdx.y;
+    const double n = S.p[i], ninv = S.pinv[i];
+    double* dst = A + (uint64_t)i * stride;
+    const int C = 1 << lgC;
+    const uint64_t col0 = (uint64_t)blockIdx.x << lgC;
+    const int total = (1 << d1) << lgC;
+    for (int u = threadIdx.x; u < total; u += blockDim.x) {
+        uint64_t row = (uint64_t)(u >> lgC), c = (uint64_t)(u & (C - 1));
+        lds[u] = dst[(row << d2) + col0 + c];
+    }
+    __syncthreads();
+    fa_lds_inv(lds, d1, lgC, 0, S.w2inv[i], n, ninv);
+    for (int u = threadIdx.x; u < total; u += blockDim.x) {
+        uint64_t row = (uint64_t)(u >> lgC), c = (uint64_t)(u & (C - 1));
+        dst[(row << d2) + col0 + c] = lds[u];
+    }
+}
+
+// ------------------------------------------------------------------------
+// CRT recombination and carry propagation
+// ------------------------------------------------------------------------
+
+#define FA_WIN 9        // sliding window limbs: S (<= 7 limbs) shifted + pending
+#define FA_CRT_WG 256
+
+// x_i = c * cop_i^-1 mod p_i for the true coefficient c < M (the
+// normalizer folded in the pointwise product), so
+// c = sum_i x_i (M/p_i) - q M with q = floor(sum_i x_i / p_i)
+// (crt_helpers.h _big_mul/_big_addmul/_reduce_big_sum, mpn_mul.c:883-906).
+FA_DEV void fa_crt_coeff(const double* __restrict__ Q, uint64_t stride, uint64_t j,
+                         const fa_crt_consts& C, uint64_t s[FA_CRT_MAXL])
+{
+#pragma unroll
+    for (int l = 0; l < FA_CRT_MAXL; l++) s[l] = 0;
+    double qe = 0.0;
+    for (int i = 0; i < C.np; i++) {
+        uint64_t x = fa_reduce_0n_u64(Q[(uint64_t)i * stride + j], C.p[i], C.pinv[i]);
+        qe += (double)x * C.rp[i];
+        uint64_t carry = 0;
+#pragma unroll
+        for (int l = 0; l < FA_CRT_MAXL; l++) {
+            uint64_t lo = x * C.cof[i][l];
+            uint64_t hi = __umul64hi(x, C.cof[i][l]);
+            uint64_t t = s[l] + lo;
+            hi += (t < lo);
+            t += carry;
+            hi += (t < carry);
+            s[l] = t;
+            carry = hi;
+        }
+    }
+    // subtract q*M; q may be off by one near integers: fix up with M
+    uint64_t q = (uint64_t)qe;
+    uint64_t borrow = 0, carry = 0;
+#pragma unroll
+    for (int l = 0; l < FA_CRT_MAXL; l++) {
+        uint64_t lo = q * C.M[l];
+        uint64_t hi = __umul64hi(q, C.M[l]);
+        lo += carry;
+        hi += (lo < carry);
+        carry = hi;
+        uint64_t t = s[l] - lo;
+        uint64_t b1 = (s[l] < lo);
+        uint64_t t2 = t - borrow;
+        uint64_t b2 = (t < borrow);
+        s[l] = t2;
+        borrow = b1 | b2;
+    }
+    if (borrow) {
+        uint64_t c2 = 0;
+#pragma unroll
+        for (int l = 0; l < FA_CRT_MAXL; l++) {
+            uint64_t t = s[l] + C.M[l];
+            uint64_t cc = (t < s[l]);
+            t += c2;
+            cc |= (t < c2);
+            s[l] = t;
+            c2 = cc;
+        }
+    } else {
+        // s >= M ?
+        int ge = 1;
+#pragma unroll
+        for (int l = FA_CRT_MAXL - 1; l >= 0; l--) {
+            if (ge == 1 && s[l] != C.M[l]) ge = (s[l] > C.M[l]) ? 2 : 0;
+        }
+        if (ge) {
+            uint64_t br = 0;
+#pragma unroll
+            for (int l = 0; l < FA_CRT_MAXL; l++) {
+                uint64_t t = s[l] - C.M[l];
+                uint64_t b1 = (s[l] < C.M[l]);
+                uint64_t t2 = t - br;
+                uint64_t b2 = (t < br);
+                s[l] = t2;
+                br = b1 | b2;
+            }
+        }
+    }
+}
+
+// One thread per group of 8 consecutive coefficients (8*bits is a multiple
+// of 64, so a group owns bits/8 whole limbs).  Thread 0 of each workgroup
+// recomputes the group just before the workgroup's range so that its
+// overhang is available without a second pass.
+__global__ void __launch_bounds__(FA_CRT_WG) fa_k_crt_mpn(const double* __restrict__ Q,
+                                                         uint64_t stride, fa_crt_consts C,
+                                                         uint64_t zlen, int bits,
+                                                         uint64_t* __restrict__ z, uint64_t zn,
+                                                         uint64_t ngroups,
+                                                         uint8_t* __restrict__ flags,
+                                                         uint8_t* __restrict__ agg)
+{
+    __shared__ uint64_t sh_ovh[FA_CRT_WG][FA_WIN];
+    __shared__ uint8_t sh_f[FA_CRT_WG];
+    const int tid = threadIdx.x;
+    const int64_t g = (int64_t)blockIdx.x * (FA_CRT_WG - 1) + tid - 1;
+    const int own = bits >> 3;
+    const bool active = g >= 0 && (uint64_t)g < ngroups;
+
+    uint64_t win[FA_WIN];
+#pragma unroll
+    for (int l = 0; l < FA_WIN; l++) win[l] = 0;
+
+    if (active) {
+        uint64_t zbase = (uint64_t)g * own;
+        uint64_t emitted = 0;
+        uint64_t prev_limb = 0;
+        for (int k = 0; k < 8; k++) {
+            uint64_t j = (uint64_t)g * 8 + k;
+            uint64_t bitoff = (uint64_t)k * bits;
+            int sh = (int)(bitoff & 63);
+            // limbs below floor(bitoff/64) are final: emit them
+            uint64_t cur_limb = bitoff >> 6;
+            while (prev_limb < cur_limb) {
+                if (tid > 0 && zbase + emitted < zn) z[zbase + emitted] = win[0];
+                emitted++;
+#pragma unroll
+                for (int l = 0; l < FA_WIN - 1; l++) win[l] = win[l + 1];
+                win[FA_WIN - 1] = 0;
+                prev_limb++;
+            }
+            if (j < zlen) {
+                uint64_t s[FA_CRT_MAXL];
+                fa_crt_coeff(Q, stride, j, C, s);
+                uint64_t carry = 0;
+#pragma unroll
+                for (int l = 0; l < FA_WIN; l++) {
+                    uint64_t cur = (l < FA_CRT_MAXL) ? s[l] : 0;
+                    uint64_t prv = (l >= 1 && l - 1 < FA_CRT_MAXL) ? s[l - 1] : 0;
+                    uint64_t v = sh ? ((cur << sh) | (prv >> (64 - sh))) : cur;
+                    uint64_t t = win[l] + v;
+                    uint64_t c1 = (t < v);
+                    t += carry;
+                    c1 |= (t < carry);
+                    win[l] = t;
+                    carry = c1;
+                }
+            }
+        }
+        uint64_t end_limb = (uint64_t)own;   // 8*bits/64
+        while (prev_limb < end_limb) {
+            if (tid > 0 && zbase + emitted < zn) z[zbase + emitted] = win[0];
+            emitted++;
+#pragma unroll
+            for (int l = 0; l < FA_WIN - 1; l++) win[l] = win[l + 1];
+            win[FA_WIN - 1] = 0;
+            prev_limb++;
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < FA_WIN; l++) sh_ovh[tid][l] = win[l];
+    __syncthreads();
+
+    uint8_t f = 0;   // bit0: carry out (generate), bit1: all limbs ones (propagate)
+    if (active && tid > 0) {
+        uint64_t zbase = (uint64_t)g * own;
+        uint64_t carry = 0;
+        int prop = 1;
+        for (int l = 0; l < own; l++) {
+            if (zbase + l >= zn) { prop = 0; break; }
+            uint64_t add = (l < FA_WIN) ? sh_ovh[tid - 1][l] : 0;
+            uint64_t v = z[zbase + l];
+            uint64_t cin = carry;
+            uint64_t t = v + add;
+            uint64_t c1 = (t < add);
+            t += cin;
+            c1 |= (t < cin);
+            carry = c1;
+            if (add | cin) z[zbase + l] = t;
+            if (t != ~UINT64_C(0)) prop = 0;
+            if (l >= FA_WIN - 1 && carry == 0 && !prop) break;
+        }
+        f = (uint8_t)(carry | (prop << 1));
+        flags[g] = f;
+    }
+    sh_f[tid] = (tid > 0 && active) ? f : 2;   // identity: propagate, no generate
+    __syncthreads();
+    if (tid == 0) {
+        uint8_t gg = 0, pp = 1;
+        for (int t = 1; t < FA_CRT_WG; t++) {
+            uint8_t ft = sh_f[t];
+            gg = (uint8_t)((ft & 1) | (((ft >> 1) & 1) & gg));
+            pp = (uint8_t)(pp & ((ft >> 1) & 1));
+        }
+        agg[blockIdx.x] = (uint8_t)(gg | (pp << 1));
+    }
+}
+
+// exclusive scan of workgroup (generate, propagate) pairs -> carry-in
+__global__ void __launch_bounds__(256) fa_k_carry_scan(const uint8_t* __restrict__ agg,
+                                                       uint8_t* __restrict__ cin, uint64_t nblk)
+{
+    __shared__ uint8_t sg[256], sp[256];
+    __shared__ uint8_t running;
+    if (threadIdx.x == 0) running = 0;
+    __syncthreads();
+    for (uint64_t base = 0; base < nblk; base += 256) {
+        uint64_t b = base + threadIdx.x;
+        uint8_t a = (b < nblk) ? agg[b] : 2;
+        sg[threadIdx.x] = a & 1;
+        sp[threadIdx.x] = (a >> 1) & 1;
+        __syncthreads();
+        // inclusive Hillis-Steele scan of the operator
+        // (g1,p1) then (g2,p2) -> (g2 | p2 & g1, p1 & p2)
+        for (int off = 1; off < 256; off <<= 1) {
+            uint8_t g1 = 0, p1 = 1;
+            if ((int)threadIdx.x >= off) { g1 = sg[threadIdx.x - off]; p1 = sp[threadIdx.x - off]; }
+            __syncthreads();
+            uint8_t g2 = sg[threadIdx.x], p2 = sp[threadIdx.x];
+            sg[threadIdx.x] = g2 | (p2 & g1);
+            sp[threadIdx.x] = p1 & p2;
+            __syncthreads();
+        }
+        uint8_t run = running;
+        // exclusive value for b: inclusive of b-1 applied to running
+        uint8_t ex;
+        if (threadIdx.x == 0) ex = run;
+        else ex = sg[threadIdx.x - 1] | (sp[threadIdx.x - 1] & run);
+        if (b < nblk) cin[b] = ex;
+        __syncthreads();
+        if (threadIdx.x == 255) running = sg[255] | (sp[255] & run);
+        __syncthreads();
+    }
+}
+
+// add the incoming carries: group g gets a carry iff the exclusive scan of
+// the flags of the groups before it (within the workgroup) and the
+// workgroup carry-in produce one
+__global__ void __launch_bounds__(FA_CRT_WG) fa_k_carry_apply(const uint8_t* __restrict__ flags,
+                                                             const uint8_t* __restrict__ cin,
+                                                             uint64_t* __restrict__ z, uint64_t zn,
+                                                             int own, uint64_t ngroups)
+{
+    __shared__ uint8_t sg[FA_CRT_WG], sp[FA_CRT_WG];
+    const int tid = threadIdx.x;
+    const int64_t g = (int64_t)blockIdx.x * (FA_CRT_WG - 1) + tid - 1;
+    const bool active = tid > 0 && g >= 0 && (uint64_t)g < ngroups;
+    uint8_t f = active ? flags[g] : 2;
+    if (tid == 0) f = (uint8_t)(cin[blockIdx.x] & 1);   // seed with the block carry-in
+    sg[tid] = f & 1;
+    sp[tid] = (f >> 1) & 1;
+    __syncthreads();
+    for (int off = 1; off < FA_CRT_WG; off <<= 1) {
+        uint8_t g1 = 0, p1 = 1;
+        if (tid >= off) { g1 = sg[tid - off]; p1 = sp[tid - off]; }
+        __syncthreads();
+        uint8_t g2 = sg[tid], p2 = sp[tid];
+        sg[tid] = g2 | (p2 & g1);
+        sp[tid] = p1 & p2;
+        __syncthreads();
+    }
+    if (!active) return;
+    uint8_t carry_in = sg[tid - 1];
+    if (!carry_in) return;
+    uint64_t zbase = (uint64_t)g * own;
+    for (int l = 0; l < own && zbase + l < zn; l++) {
+        uint64_t v = z[zbase + l] + 1;
+        z[zbase + l] = v;
+        if (v != 0) break;
+    }
+}
+
+// ------------------------------------------------------------------------
+// host drivers
+// ------------------------------------------------------------------------
+
+#define LAUNCH_CHECK()                                                               \
+    do {                                                                             \
+        hipError_t e_ = hipGetLastError();                                           \
+        if (e_ != hipSuccess)                                                        \
+            return fa_set_error(FA_E_HIP, "kernel launch failed: %s", hipGetErrorString(e_)); \
+    } while (0)
+
+static fa_operand to_operand(const fa_pack_desc* d)
+{
+    fa_operand o;
+    o.kind = d->kind;
+    o.src = d->src;
+    o.n = d->n;
+    o.bits = d->bits;
+    o.width = d->width;
+    return o;
+}
+
+static int col_lg(int d1, int d2)
+{
+    int lgC = 0;
+    while (((1 << d1) << (lgC + 1)) <= FA_LDS_COL_DOUBLES && lgC + 1 <= d2) lgC++;
+    return lgC;
+}
+
+int fa_convolve(fa_thread_state* st, const fa_tables* T, int np, int offset, int depth,
+                const fa_pack_desc* a, const fa_pack_desc* b, double* A, double* B,
+                hipStream_t stream)
+{
+    (void)st;
+    fa_prime_set S;
+    fa_prime_set_init(&S, T, np, depth, offset, 1);
+    fa_operand oa = to_operand(a), ob = to_operand(b);
+    const uint64_t stride = UINT64_C(1) << depth;
+    if (depth <= FA_ROW_LG) {
+        size_t lds = 2 * stride * sizeof(double);
+        hipLaunchKernelGGL(fa_k_small_conv, dim3(np), dim3(256), lds, stream, oa, ob, depth, A,
+                           stride, S);
+        LAUNCH_CHECK();
+        return FA_OK;
+    }
+    const int d2 = FA_ROW_LG, d1 = depth - d2;
+    const int lgC = col_lg(d1, d2);
+    const size_t col_lds = ((size_t)1 << d1 << lgC) * sizeof(double);
+    dim3 gcol((unsigned)(1u << (d2 - lgC)), (unsigned)np, 2);
+    hipLaunchKernelGGL(fa_k_col_fwd, gcol, dim3(256), col_lds, stream, oa, ob, d1, d2, lgC, A, B,
+                       stride, S);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(fa_k_row_conv, dim3(1u << d1, np), dim3(256),
+                       2 * ((size_t)1 << d2) * sizeof(double), stream, A, B, d2, stride, S);
+    LAUNCH_CHECK();
+    dim3 gci((unsigned)(1u << (d2 - lgC)), (unsigned)np, 1);
+    hipLaunchKernelGGL(fa_k_col_inv, gci, dim3(256), col_lds, stream, A, d1, d2, lgC, stride, S);
+    LAUNCH_CHECK();
+    return FA_OK;
+}
+
+// scratch layout for one mpn product
+struct mpn_scratch {
+    double* A;
+    double* B;
+    uint8_t* flags;
+    uint8_t* agg;
+    uint8_t* cin;
+};
+
+static size_t mpn_scratch_bytes(const fa_mpn_plan_t& P, uint64_t zn, uint64_t* ngroups,
+                                uint64_t* nblk)
+{
+    uint64_t own = (uint64_t)P.bits / 8;
+    *ngroups = (zn + own - 1) / own;
+    *nblk = (*ngroups + (FA_CRT_WG - 2)) / (FA_CRT_WG - 1);
+    size_t tr = ((size_t)P.np << P.depth) * sizeof(double);
+    size_t bytes = 2 * tr + *ngroups + 2 * (*nblk) + 4096;
+    return bytes;
+}
+
+static int mpn_mul_device(uint64_t* z, const uint64_t* a, uint64_t an, const uint64_t* b,
+                          uint64_t bn, hipStream_t stream)
+{
+    fa_mpn_plan_t P;
+    int rc = fa_mpn_choose_plan(an, bn, &P);
+    if (rc) return rc;
+    fa_tables* T = fa_get_tables(P.depth);
+    if (!T) return FA_E_NODEV;
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    if (!stream) stream = st->stream;
+    uint64_t zn = an + bn, ngroups, nblk;
+    size_t bytes = mpn_scratch_bytes(P, zn, &ngroups, &nblk);
+    void* buf;
+    rc = fa_scratch_reserve(st, bytes, &buf);
+    if (rc) return rc;
+    size_t tr = ((size_t)P.np << P.depth) * sizeof(double);
+    mpn_scratch W;
+    W.A = (double*)buf;
+    W.B = (double*)((char*)buf + tr);
+    W.flags = (uint8_t*)((char*)buf + 2 * tr);
+    W.agg = W.flags + ngroups;
+    W.cin = W.agg + nblk;
+
+    fa_pack_desc da = {FA_PACK_MPN, a, an, P.bits, 1};
+    fa_pack_desc db = {FA_PACK_MPN, b, bn, P.bits, 1};
+    rc = fa_convolve(st, T, P.np, 0, P.depth, &da, &db, W.A, W.B, stream);
+    if (rc) return rc;
+
+    fa_crt_consts C;
+    fa_crt_consts_init(&C, P.np, 0);
+    const uint64_t stride = UINT64_C(1) << P.depth;
+    hipLaunchKernelGGL(fa_k_crt_mpn, dim3((unsigned)nblk), dim3(FA_CRT_WG), 0, stream, W.A,
+                       stride, C, P.zlen, P.bits, z, zn, ngroups, W.flags, W.agg);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(fa_k_carry_scan, dim3(1), dim3(256), 0, stream, W.agg, W.cin, nblk);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(fa_k_carry_apply, dim3((unsigned)nblk), dim3(FA_CRT_WG), 0, stream,
+                       W.flags, W.cin, z, zn, P.bits / 8, ngroups);
+    LAUNCH_CHECK();
+    return FA_OK;
+}
+
+extern "C" int fa_mpn_mul_device(uint64_t* z, const uint64_t* a, int64_t an, const uint64_t* b,
+                                 int64_t bn, void* stream)
+{
+    if (an <= 0 || bn <= 0 || !z || !a || !b)
+        return fa_set_error(FA_E_ARG, "fa_mpn_mul_device: bad arguments");
+    return mpn_mul_device(z, a, (uint64_t)an, b, (uint64_t)bn, (hipStream_t)stream);
+}
+
+extern "C" int fa_mpn_mul(uint64_t* z, const uint64_t* a, int64_t an, const uint64_t* b,
+                          int64_t bn)
+{
+    if (an <= 0 || bn <= 0 || !z || !a || !b)
+        return fa_set_error(FA_E_ARG, "fa_mpn_mul: bad arguments");
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    uint64_t zn = (uint64_t)(an + bn);
+    size_t bytes = ((size_t)an + (size_t)bn + zn) * sizeof(uint64_t);
+    void* hb;
+    int rc = fa_hbuf_reserve(st, bytes, &hb);
+    if (rc) return rc;
+    uint64_t* da = (uint64_t*)hb;
+    uint64_t* db = da + an;
+    uint64_t* dz = db + bn;
+    if (hipMemcpyAsync(da, a, an * sizeof(uint64_t), hipMemcpyHostToDevice, st->stream) != hipSuccess ||
+        hipMemcpyAsync(db, b, bn * sizeof(uint64_t), hipMemcpyHostToDevice, st->stream) != hipSuccess)
+        return fa_set_error(FA_E_HIP, "operand upload failed");
+    rc = mpn_mul_device(dz, da, (uint64_t)an, db, (uint64_t)bn, st->stream);
+    if (rc) {
+        hipStreamSynchronize(st->stream);
+        return rc;
+    }
+    if (hipMemcpyAsync(z, dz, zn * sizeof(uint64_t), hipMemcpyDeviceToHost, st->stream) != hipSuccess)
+        return fa_set_error(FA_E_HIP, "product download failed");
+    if (hipStreamSynchronize(st->stream) != hipSuccess)
+        return fa_set_error(FA_E_HIP, "stream synchronize failed: %s",
+                            hipGetErrorString(hipGetLastError()));
+    return FA_OK;
+}
+
+extern "C" void mpn_mul_default_mpn_ctx(uint64_t* r1, const uint64_t* i1, int64_t n1,
+                                        const uint64_t* i2, int64_t n2)
+{
+    int rc = fa_mpn_mul(r1, i1, n1, i2, n2);
+    if (rc != FA_OK) {
+        fprintf(stderr, "mpn_mul_default_mpn_ctx: %s\n", fa_last_error());
+        abort();
+    }
+}

@@ -1,0 +1,132 @@
+/*
+ * flint_amd.h -- C ABI of the MI355X (gfx950) implementation of FLINT's
+ * fft_small multiplication path.
+ *
+ * Reference interfaces are cited as file:line of flintlib/flint 3.7.0-dev.
+ * Plain pointers and sizes only.  Functions returning int report
+ *   FA_OK (0) on success, a negative FA_E* code on failure
+ * (fa_last_error() has the message); the fmpz entry points additionally
+ * return FA_DECLINED (1... see below) exactly where the reference declines.
+ *
+ * Host-pointer entry points are synchronous: they return after the output
+ * is in host memory.  Device-pointer entry points (suffix _device) enqueue
+ * on the given HIP stream (NULL = the calling thread's stream) and return
+ * immediately.  Each host thread gets its own stream and scratch, mirroring
+ * the thread-local default mpn_ctx of default_ctx.c:14-15.
+ */
+#ifndef FLINT_AMD_H
+#define FLINT_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FA_OK 0
+#define FA_E_ARG (-1)       /* invalid argument */
+#define FA_E_HIP (-2)       /* HIP runtime error */
+#define FA_E_NOMEM (-3)     /* device allocation failed */
+#define FA_E_RANGE (-4)     /* size outside the supported range */
+#define FA_E_NODEV (-5)     /* no usable gfx950 device */
+
+/* layout of FLINT's nmod_t (flint.h.in:441-447) */
+typedef struct {
+    uint64_t n;
+    uint64_t ninv;
+    uint64_t norm;
+} fa_nmod_t;
+
+/* ---------------------------------------------------------------------- */
+/* the drop-in entry points, with the reference's exact signatures        */
+/* ---------------------------------------------------------------------- */
+
+/* replaces mpn_mul_default_mpn_ctx, fft_small.h:852, default_ctx.c:41-45:
+   r1[0, n1+n2) = i1 * i2.  Aborts the process on device failure, as the
+   reference's flint_throw does (sd_fft_ctx.c:63-74). */
+void mpn_mul_default_mpn_ctx(uint64_t* r1, const uint64_t* i1, int64_t n1,
+                             const uint64_t* i2, int64_t n2);
+
+/* replaces _nmod_poly_mul_mid_default_mpn_ctx, fft_small.h:853,
+   default_ctx.c:47-51: res[0, zh-zl) = coefficients [zl, zh) of a*b mod n */
+void _nmod_poly_mul_mid_default_mpn_ctx(uint64_t* res, int64_t zl, int64_t zh,
+                                        const uint64_t* a, int64_t an,
+                                        const uint64_t* b, int64_t bn,
+                                        fa_nmod_t mod);
+
+/* ---------------------------------------------------------------------- */
+/* status-returning C ABI                                                 */
+/* ---------------------------------------------------------------------- */
+
+/* mpn_ctx_mpn_mul, mpn_mul.c:1980-1983 (full product, an, bn >= 1) */
+int fa_mpn_mul(uint64_t* z, const uint64_t* a, int64_t an,
+               const uint64_t* b, int64_t bn);
+
+/* same with operands and output in device memory (no PCIe copies) */
+int fa_mpn_mul_device(uint64_t* z, const uint64_t* a, int64_t an,
+                      const uint64_t* b, int64_t bn, void* stream);
+
+/* plan the device call would use: out = {np, bits, depth, d1, d2, zlen} */
+int fa_mpn_plan(int64_t an, int64_t bn, int64_t* out);
+
+/* _nmod_poly_mul_mid_mpn_ctx, nmod_poly_mul.c:904-972 */
+int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh,
+                         const uint64_t* a, int64_t an,
+                         const uint64_t* b, int64_t bn, fa_nmod_t mod);
+
+/* _fmpz_poly_mul_mid_mpn_ctx, fmpz_poly_mul.c:319-401.  Coefficients are
+   passed as fixed-width two's-complement limb vectors (aw, bw, zw limbs per
+   coefficient), the form fmpz_get_signed_ui_array / fmpz_set_signed_ui_array
+   (fmpz.h:238-239) convert to and from.  Returns 1 on success and 0 when
+   the reference declines (more than 8 primes needed, fmpz_poly_mul.c:
+   377-379), leaving z untouched; negative on error. */
+int fa_fmpz_poly_mul_mid(uint64_t* z, uint64_t zw, int64_t zl, int64_t zh,
+                         const uint64_t* a, uint64_t aw, int64_t an,
+                         const uint64_t* b, uint64_t bw, int64_t bn);
+
+/* ---------------------------------------------------------------------- */
+/* sd_fft_ctx plugin surface, fft_small.h:206-281                         */
+/* ---------------------------------------------------------------------- */
+
+typedef struct {
+    double p;                          /* the FFT prime */
+    double pinv;
+    fa_nmod_t mod;
+    uint64_t primitive_2power_root;    /* sd_fft_ctx.c:32-37 */
+    int32_t depth;                     /* twiddle depth available on device */
+    int32_t slot;                      /* library-internal table handle */
+} fa_sd_fft_ctx_struct;
+
+/* sd_fft_ctx_init_prime, sd_fft_ctx.c:56-154 */
+int fa_sd_fft_ctx_init_prime(fa_sd_fft_ctx_struct* Q, uint64_t pp);
+/* sd_fft_ctx_clear, sd_fft_ctx.c:16-26 */
+void fa_sd_fft_ctx_clear(fa_sd_fft_ctx_struct* Q);
+/* sd_fft_trunc, sd_fft.c:1230-1274 (host data, in place) */
+int fa_sd_fft_trunc(fa_sd_fft_ctx_struct* Q, double* d, uint64_t L,
+                    uint64_t itrunc, uint64_t otrunc);
+/* sd_ifft_trunc, sd_ifft.c:1503-1533 (host data, in place) */
+int fa_sd_ifft_trunc(fa_sd_fft_ctx_struct* Q, double* d, uint64_t L, uint64_t trunc);
+/* sd_fft_ctx_w, fft_small.h:308-312 */
+double fa_sd_fft_ctx_w(const fa_sd_fft_ctx_struct* Q, uint64_t j);
+
+/* ---------------------------------------------------------------------- */
+/* library state                                                          */
+/* ---------------------------------------------------------------------- */
+
+const char* fa_last_error(void);
+int fa_init(void);                   /* optional: create device tables now */
+int fa_default_primes(uint64_t* out8);
+/* synchronize the calling thread's stream */
+int fa_sync(void);
+/* the calling thread's stream (hipStream_t) */
+void* fa_stream(void);
+
+/* seeded synthetic inputs (SplitMix64 -> xoshiro256**), identical to the
+   generator the tests use, written on the host */
+void fa_synth_limbs(uint64_t seed, uint64_t* a, uint64_t n, int top_nonzero);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
