@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""bench.py -- device-resident throughput of the MI355X fft_small path.
+
+Workload (N=1): plain mpn_mul of two naturals of 2^24 64-bit limbs each,
+uniform random limbs from the seeded generator (seed 0xF1A70002, top limb
+nonzero), one product per call, operands and product resident in HBM.
+A "step" is one full product.  The metric is limb-products/s = an*bn per
+second.  After the timed region the product is checked against
+(a mod q)(b mod q) for three 61-bit primes q.
+
+The roofline entry is for the dominant kernel: algorithmic bytes per launch
+(8 bytes per transform point per array pass, at the reference plan's
+geometry np*T) divided by that kernel's average launch time, measured with
+HIP events recorded around each launch on the launching stream.
+
+The CPU baseline is the oracle (oracle/fsmall_oracle.c, a scalar C
+restatement of the reference) timed on a bounded sample, scaled to the full
+workload by the n log n cost of the transform.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12            # MI355X HBM3E, bytes/s
+FP64_LANE_OPS_PEAK = 3.93e13  # 78.6 TFLOP/s FP64 vector = 3.93e13 lane-ops/s
+
+# reference-plan geometry (mpn_ctx_best_profile at 1 thread, mpn_mul.c:1248-1340)
+WORKLOADS = {
+    "mpn_2p24": dict(an=1 << 24, bn=1 << 24, seed=0xF1A70002, ref_np=6, ref_bits=136,
+                     ref_T=15790336, ref_depth=24,
+                     desc="plain mpn_mul, 2^24 x 2^24 limbs, one product per call"),
+    "mpn_65536": dict(an=65536, bn=65536, seed=0xF1A70001, ref_np=6, ref_bits=140,
+                      ref_T=60160, ref_depth=16,
+                      desc="plain mpn_mul, 65536 x 65536 limbs, one product per call"),
+}
+
+# array passes (8 bytes per point each) of every kernel of the mpn pipeline
+KERNEL_PASSES = {
+    "fa_k_col_fwd": 2,     # writes the transformed a and b columns; reads the limbs
+    "fa_k_row_conv": 3,    # reads a and b rows, writes the product row
+    "fa_k_col_inv": 2,     # reads and writes the product columns
+    "fa_k_crt_mpn": 1,     # reads the residues; writes the output limbs
+    "fa_k_small_conv": 1,
+}
+
+
+def kernel_bytes(name, W):
+    pts = W["ref_np"] * W["ref_T"]
+    b = 8 * KERNEL_PASSES.get(name, 0) * pts
+    if name == "fa_k_col_fwd":
+        b += 8 * (W["an"] + W["bn"])
+    if name == "fa_k_crt_mpn":
+        b += 8 * (W["an"] + W["bn"])
+    return b
+
+
+def product_model(W):
+    depth = W["ref_depth"]
+    k = 0 if depth <= 12 else 6 * ((depth + 11) // 12) - 2
+    B = 8 * (W["an"] + W["bn"]) * 2 + 8 * W["ref_np"] * W["ref_T"] * k
+    F = W["ref_np"] * W["ref_T"] * (13.5 * depth + 12)
+    return B, F
+
+
+def mod_q(limbs, q):
+    return int.from_bytes(np.ascontiguousarray(limbs).tobytes(), "little") % q
+
+
+def cpu_baseline(W, sample_log2):
+    """oracle mpn_mul on a 2^s x 2^s sample, scaled to the workload"""
+    from oracle import oracle as O
+    n = 1 << sample_log2
+    rng = np.random.default_rng(12345)
+    a = rng.integers(0, 2**64, n, dtype=np.uint64)
+    b = rng.integers(0, 2**64, n, dtype=np.uint64)
+    a[-1] |= np.uint64(1)
+    b[-1] |= np.uint64(1)
+    t0 = time.perf_counter()
+    O.mpn_mul(a, b)
+    dt = time.perf_counter() - t0
+    N = W["an"]
+    scale = (N * np.log2(N)) / (n * np.log2(n))
+    t_full = dt * scale
+    return {
+        "value": W["an"] * W["bn"] / t_full,
+        "unit": "limb-products/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "oracle mpn_mul of 2^%d x 2^%d limbs took %.2f s on 1 thread; scaled to "
+                  "2^%d x 2^%d by n log n (x%.2f)" % (sample_log2, sample_log2, dt,
+                                                      int(np.log2(N)), int(np.log2(N)), scale),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="mpn_2p24", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-log2", type=int, default=22)
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_mpn_2p24.json"))
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(multi-GPU sharding is design-only, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import flint_amd as fa
+
+    W = WORKLOADS[args.workload]
+    an, bn = W["an"], W["bn"]
+    t0 = time.time()
+    a_h, b_h = fa.synth_mpn_pair(W["seed"], an, bn)
+    print("# inputs generated in %.2fs" % (time.time() - t0), file=sys.stderr, flush=True)
+
+    dev = torch.device("cuda:0")
+    a = torch.from_numpy(a_h.view(np.int64)).to(dev)
+    b = torch.from_numpy(b_h.view(np.int64)).to(dev)
+    z = torch.empty(an + bn, dtype=torch.int64, device=dev)
+    # a dedicated (default-priority) stream: the library enqueues on it and the
+    # timing events are recorded on it
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    sptr = stream.cuda_stream
+    plan = fa.mpn_plan(an, bn)
+
+    def step():
+        fa.mpn_mul_device(z.data_ptr(), a.data_ptr(), an, b.data_ptr(), bn, sptr)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print("# warmup done", file=sys.stderr, flush=True)
+
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    print("# timed: %.3f ms/product" % ms, file=sys.stderr, flush=True)
+
+    # per-kernel launch times (HIP events around each launch, same stream)
+    fa.prof_reset()
+    fa.prof_enable(True)
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    fa.prof_enable(False)
+    kt = fa.prof_read()
+    per_kernel = {k: dict(avg_ms=v[0] / v[1], launches=v[1]) for k, v in kt.items()}
+    dominant = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"] * per_kernel[k]["launches"])
+    dk = per_kernel[dominant]
+    dbytes = kernel_bytes(dominant, W)
+    achieved = dbytes / (dk["avg_ms"] * 1e-3)
+    traffic = None
+    if os.path.exists(args.traffic_json):
+        with open(args.traffic_json) as f:
+            tj = json.load(f)
+        traffic = tj.get("kernels", {}).get(dominant, {}).get("hbm_bytes_per_launch")
+
+    checked = None
+    if not args.no_check:
+        zh = z.cpu().numpy().view(np.uint64)
+        ok = True
+        for q in (2305843009213693951, 2305843009213693921, 2305843009213693907):
+            ok &= (mod_q(a_h, q) * mod_q(b_h, q)) % q == mod_q(zh, q)
+        checked = bool(ok)
+        if not ok:
+            print("bench.py: PRODUCT MISMATCH", file=sys.stderr)
+
+    B, F = product_model(W)
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(W, args.cpu_sample_log2)
+        except Exception as e:  # oracle missing: report null
+            print("# cpu baseline unavailable: %s" % e, file=sys.stderr)
+
+    value = an * bn / (ms * 1e-3)
+    out = {
+        "metric": "limb-products/s batched vs plain GPU vs CPU, bit-exact; % HBM roofline",
+        "value": value,
+        "unit": "limb-products/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": W["desc"],
+            "an": an, "bn": bn, "seed": W["seed"],
+            "plan": plan,
+            "bit_exact_mod_q_check": checked,
+            "product_model_bytes": B,
+            "product_model_hbm_frac": (B / (ms * 1e-3)) / HBM_PEAK,
+            "product_model_fp64_lane_ops": F,
+            "product_model_fp64_frac": (F / (ms * 1e-3)) / FP64_LANE_OPS_PEAK,
+            "kernels_avg_ms": {k: round(v["avg_ms"], 5) for k, v in per_kernel.items()},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -62,6 +62,11 @@ SIGNATURES = {
     "fa_sync": (ctypes.c_int, []),
     "fa_stream": (_VP, []),
     "fa_synth_limbs": (None, [_u64, _P64, _u64, ctypes.c_int]),
+    "fa_synth_mpn_pair": (None, [_u64, _P64, _u64, _P64, _u64]),
+    "fa_prof_enable": (None, [ctypes.c_int]),
+    "fa_prof_reset": (None, []),
+    "fa_prof_read": (ctypes.c_int, [ctypes.c_char_p, _PD, _P64, ctypes.c_int]),
+    "fa_copy_device": (ctypes.c_int, [_VP, _VP, _u64, _VP]),
 }
 
 _lib = None
@@ -108,6 +113,39 @@ def synth_limbs(seed, n, top_nonzero=True):
     a = np.empty(n, dtype=np.uint64)
     lib().fa_synth_limbs(seed, _p(a), n, 1 if top_nonzero else 0)
     return a
+
+
+def synth_mpn_pair(seed, an, bn):
+    a = np.empty(an, dtype=np.uint64)
+    b = np.empty(bn, dtype=np.uint64)
+    lib().fa_synth_mpn_pair(seed, _p(a), an, _p(b), bn)
+    return a, b
+
+
+def prof_enable(on=True):
+    lib().fa_prof_enable(1 if on else 0)
+
+
+def prof_reset():
+    lib().fa_prof_reset()
+
+
+def prof_read(max_entries=64):
+    """{kernel name: (total ms, launches)} recorded since prof_reset()."""
+    names = ctypes.create_string_buffer(64 * max_entries)
+    ms = np.zeros(max_entries, dtype=np.float64)
+    cnt = np.zeros(max_entries, dtype=np.int64)
+    n = lib().fa_prof_read(names, ms.ctypes.data_as(_PD), cnt.ctypes.data_as(_P64), max_entries)
+    raw = names.raw
+    out = {}
+    for i in range(n):
+        nm = raw[64 * i: 64 * i + 64].split(b"\0", 1)[0].decode()
+        out[nm] = (float(ms[i]), int(cnt[i]))
+    return out
+
+
+def copy_device(dst_ptr, src_ptr, nbytes, stream=None):
+    return _check(lib().fa_copy_device(dst_ptr, src_ptr, nbytes, stream))
 
 
 def mpn_plan(an, bn):

@@ -141,15 +141,16 @@ extern "C" int fa_default_primes(uint64_t* out8)
 // device tables
 // ------------------------------------------------------------------------
 
-// w2[q] = hi[q >> 12] * lo[q & 4095] reduced to [-n/2, n/2]; q = (h<<12)+l
-// has disjoint bits, so w[2q] = w[2(h<<12)] w[2l] (theta is additive).
+// w2[q] = hi[q >> lg] * lo[q & (2^lg - 1)] reduced to [-n/2, n/2]: q splits
+// into disjoint bit ranges, so w[2q] = w[2(h << lg)] w[2l] (theta is additive
+// over disjoint bits, fft_small.h:124-199).
 __global__ void fa_expand_twiddles(double* __restrict__ out, const double* __restrict__ lo,
-                                   const double* __restrict__ hi, uint64_t count, double n,
-                                   double ninv)
+                                   const double* __restrict__ hi, uint64_t count, int lg,
+                                   double n, double ninv)
 {
     uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= count) return;
-    double r = fa_mulmod(hi[q >> 12], lo[q & 4095], n, ninv);
+    double r = fa_mulmod(hi[q >> lg], lo[q & ((UINT64_C(1) << lg) - 1)], n, ninv);
     double h = 0.5 * (n - 1.0);
     if (r > h) r -= n;
     else if (r < -h) r += n;
@@ -176,96 +177,117 @@ static int check_device(int dev)
     return FA_OK;
 }
 
+int fa_prime_info(uint64_t n, fa_prime_host* pr)
+{
+    if (n < 3 || !(n & 1)) return fa_set_error(FA_E_ARG, "%llu is not an odd prime",
+                                               (unsigned long long)n);
+    pr->n = n;
+    pr->p = (double)n;
+    pr->pinv = 1.0 / pr->p;
+    pr->v = __builtin_ctzll(n - 1);
+    pr->g = primitive_2power_root(n);
+    for (int k = 0; k < 8; k++)
+        pr->two32pow[k] = fa_signed_rep(fa_powmod_u64(2, 32 * (uint64_t)k, n), n);
+    return FA_OK;
+}
+
+// device tables w2[q] = w[2q] and w2inv[q] = w[2q]^-1 for q < 2^(depth-1)
+// (at least one entry); sd_fft_ctx.c:56-229 grows the same table lazily
+int fa_build_prime_tables(const fa_prime_host* pr, int depth, double** w2_out,
+                          double** w2inv_out)
+{
+    uint64_t n = pr->n;
+    int v = pr->v;
+    if (depth > v) return fa_set_error(FA_E_RANGE, "prime %llu supports depth <= %d",
+                                       (unsigned long long)n, v);
+    uint64_t count = depth >= 1 ? (UINT64_C(1) << (depth - 1)) : 1;
+    int lg = 0;
+    while ((UINT64_C(1) << lg) < count && lg < 12) lg++;
+    uint64_t nlo = UINT64_C(1) << lg, nhi = (count + nlo - 1) / nlo;
+    uint64_t gi = fa_powmod_u64(pr->g, n - 2, n);
+    // w2[2^(j-1)] = w[2^j] = g^(2^(v-j-1))
+    auto root = [&](int j, int inv) {
+        return fa_powmod_u64(inv ? gi : pr->g, UINT64_C(1) << (v - j - 1), n);
+    };
+    double* hlo = (double*)malloc(2 * nlo * sizeof(double));
+    double* hhi = (double*)malloc(2 * nhi * sizeof(double));
+    uint64_t* lo = (uint64_t*)malloc(nlo * sizeof(uint64_t));
+    uint64_t* li = (uint64_t*)malloc(nlo * sizeof(uint64_t));
+    lo[0] = li[0] = 1;
+    for (int j = 1; (UINT64_C(1) << (j - 1)) < nlo; j++) {
+        uint64_t base = UINT64_C(1) << (j - 1);
+        uint64_t r = root(j, 0), ri = root(j, 1);
+        for (uint64_t t = 0; t < base; t++) {
+            lo[base + t] = fa_mulmod_u64(r, lo[t], n);
+            li[base + t] = fa_mulmod_u64(ri, li[t], n);
+        }
+    }
+    for (uint64_t t = 0; t < nlo; t++) {
+        hlo[t] = fa_signed_rep(lo[t], n);
+        hlo[nlo + t] = fa_signed_rep(li[t], n);
+    }
+    for (uint64_t h = 0; h < nhi; h++) {
+        uint64_t x = 1, xi = 1;
+        for (int b = 0; b < 40; b++) {
+            if (!((h >> b) & 1)) continue;
+            int j = b + lg + 1;
+            x = fa_mulmod_u64(x, root(j, 0), n);
+            xi = fa_mulmod_u64(xi, root(j, 1), n);
+        }
+        hhi[h] = fa_signed_rep(x, n);
+        hhi[nhi + h] = fa_signed_rep(xi, n);
+    }
+    free(lo);
+    free(li);
+    int rc = FA_OK;
+    double *dlo = NULL, *dhi = NULL, *w2 = NULL, *w2i = NULL;
+    if (hipMalloc(&dlo, 2 * nlo * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dhi, 2 * nhi * sizeof(double)) != hipSuccess ||
+        hipMalloc(&w2, count * sizeof(double)) != hipSuccess ||
+        hipMalloc(&w2i, count * sizeof(double)) != hipSuccess) {
+        rc = fa_set_error(FA_E_NOMEM, "twiddle table allocation failed");
+    } else if (hipMemcpy(dlo, hlo, 2 * nlo * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(dhi, hhi, 2 * nhi * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fa_set_error(FA_E_HIP, "twiddle upload failed");
+    } else {
+        unsigned grid = (unsigned)((count + 255) / 256);
+        hipLaunchKernelGGL(fa_expand_twiddles, dim3(grid), dim3(256), 0, 0, w2, dlo, dhi, count, lg,
+                           pr->p, pr->pinv);
+        hipLaunchKernelGGL(fa_expand_twiddles, dim3(grid), dim3(256), 0, 0, w2i, dlo + nlo,
+                           dhi + nhi, count, lg, pr->p, pr->pinv);
+        if (hipDeviceSynchronize() != hipSuccess)
+            rc = fa_set_error(FA_E_HIP, "twiddle expansion failed");
+    }
+    if (dlo) (void)hipFree(dlo);
+    if (dhi) (void)hipFree(dhi);
+    free(hlo);
+    free(hhi);
+    if (rc != FA_OK) {
+        if (w2) (void)hipFree(w2);
+        if (w2i) (void)hipFree(w2i);
+        return rc;
+    }
+    *w2_out = w2;
+    *w2inv_out = w2i;
+    return FA_OK;
+}
+
 static int build_tables(fa_tables* T, int depth)
 {
     uint64_t P[FA_NPRIMES];
     default_primes(P);
-    uint64_t count = (depth >= 1) ? (UINT64_C(1) << (depth - 1)) : 1;
-    uint64_t nlo = 4096, nhi = count > 4096 ? count >> 12 : 1;
-    double* hlo = (double*)malloc(nlo * sizeof(double) * 2);
-    double* hhi = (double*)malloc(nhi * sizeof(double) * 2);
-    double *dlo = NULL, *dhi = NULL;
-    int rc = FA_OK;
-    if (hipMalloc(&dlo, nlo * sizeof(double) * 2) != hipSuccess ||
-        hipMalloc(&dhi, nhi * sizeof(double) * 2) != hipSuccess) {
-        rc = fa_set_error(FA_E_NOMEM, "twiddle staging allocation failed");
-        goto done;
-    }
-    for (int i = 0; i < FA_NPRIMES && rc == FA_OK; i++) {
+    for (int i = 0; i < FA_NPRIMES; i++) {
         fa_prime_host* pr = &T->pr[i];
-        uint64_t n = P[i];
-        pr->n = n;
-        pr->p = (double)n;
-        pr->pinv = 1.0 / pr->p;
-        pr->v = __builtin_ctzll(n - 1);
-        pr->g = primitive_2power_root(n);
-        for (int k = 0; k < 8; k++)
-            pr->two32pow[k] = fa_signed_rep(fa_powmod_u64(2, 32 * (uint64_t)k, n), n);
-        // w2[q] for q < 2^(v-1): w2[2^(j-1)] = w[2^j] = g^(2^(v-j-1))
-        uint64_t gi = fa_powmod_u64(pr->g, n - 2, n);
-        uint64_t* lo = (uint64_t*)malloc(nlo * sizeof(uint64_t));
-        uint64_t* li = (uint64_t*)malloc(nlo * sizeof(uint64_t));
-        lo[0] = li[0] = 1;
-        for (uint64_t j = 1; (UINT64_C(1) << (j - 1)) < nlo; j++) {
-            uint64_t base = UINT64_C(1) << (j - 1);
-            uint64_t root = fa_powmod_u64(pr->g, UINT64_C(1) << (pr->v - j - 1), n);
-            uint64_t rinv = fa_powmod_u64(gi, UINT64_C(1) << (pr->v - j - 1), n);
-            for (uint64_t r = 0; r < base; r++) {
-                lo[base + r] = fa_mulmod_u64(root, lo[r], n);
-                li[base + r] = fa_mulmod_u64(rinv, li[r], n);
-            }
-        }
-        for (uint64_t r = 0; r < nlo; r++) {
-            hlo[r] = fa_signed_rep(lo[r], n);
-            hlo[nlo + r] = fa_signed_rep(li[r], n);
-        }
-        // hi[h] = w2[h << 12] = prod over set bits b of h of w2[2^(b+12)]
-        for (uint64_t h = 0; h < nhi; h++) {
-            uint64_t x = 1, xi = 1;
-            for (int b = 0; b < 40; b++) {
-                if (!((h >> b) & 1)) continue;
-                int j = b + 12 + 1;   // w2[2^(j-1)]
-                x = fa_mulmod_u64(x, fa_powmod_u64(pr->g, UINT64_C(1) << (pr->v - j - 1), n), n);
-                xi = fa_mulmod_u64(xi, fa_powmod_u64(gi, UINT64_C(1) << (pr->v - j - 1), n), n);
-            }
-            hhi[h] = fa_signed_rep(x, n);
-            hhi[nhi + h] = fa_signed_rep(xi, n);
-        }
-        free(lo);
-        free(li);
-        double *w2 = NULL, *w2i = NULL;
-        uint64_t alloc = count > nlo ? count : nlo;
-        if (hipMalloc(&w2, alloc * sizeof(double)) != hipSuccess ||
-            hipMalloc(&w2i, alloc * sizeof(double)) != hipSuccess) {
-            rc = fa_set_error(FA_E_NOMEM, "twiddle table allocation failed");
-            break;
-        }
-        if (hipMemcpy(dlo, hlo, nlo * 2 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(dhi, hhi, nhi * 2 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
-            rc = fa_set_error(FA_E_HIP, "twiddle upload failed");
-            break;
-        }
-        uint64_t cnt = alloc;
-        unsigned grid = (unsigned)((cnt + 255) / 256);
-        hipLaunchKernelGGL(fa_expand_twiddles, dim3(grid), dim3(256), 0, 0, w2, dlo, dhi, cnt,
-                           pr->p, pr->pinv);
-        hipLaunchKernelGGL(fa_expand_twiddles, dim3(grid), dim3(256), 0, 0, w2i, dlo + nlo,
-                           dhi + nhi, cnt, pr->p, pr->pinv);
-        if (hipDeviceSynchronize() != hipSuccess) {
-            rc = fa_set_error(FA_E_HIP, "twiddle expansion failed");
-            break;
-        }
-        if (T->w2[i]) { hipFree(T->w2[i]); hipFree(T->w2inv[i]); }
+        fa_prime_info(P[i], pr);
+        double *w2, *w2i;
+        int rc = fa_build_prime_tables(pr, depth, &w2, &w2i);
+        if (rc) return rc;
+        if (T->w2[i]) { (void)hipFree(T->w2[i]); (void)hipFree(T->w2inv[i]); }
         T->w2[i] = w2;
         T->w2inv[i] = w2i;
     }
-    if (rc == FA_OK) T->depth = depth;
-done:
-    if (dlo) hipFree(dlo);
-    if (dhi) hipFree(dhi);
-    free(hlo);
-    free(hhi);
-    return rc;
+    T->depth = depth;
+    return FA_OK;
 }
 
 fa_tables* fa_get_tables(int depth)
@@ -334,8 +356,8 @@ static int grow(void** buf, size_t* size, size_t bytes, hipStream_t s)
 {
     if (*size >= bytes) return FA_OK;
     if (*buf) {
-        hipStreamSynchronize(s);
-        hipFree(*buf);
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(*buf);
         *buf = NULL;
         *size = 0;
     }
@@ -510,7 +532,9 @@ int fa_mpn_choose_plan(uint64_t an, uint64_t bn, fa_mpn_plan_t* P)
         int depth = clog2_u64(zlen);
         if (depth < 4) depth = 4;
         if (depth > FA_MAX_DEPTH) continue;
-        double cost = (double)np * (double)(UINT64_C(1) << depth);
+        // transform volume weighted by the levels, plus the quadratic CRT
+        double cost = (double)np * (double)(UINT64_C(1) << depth) * (depth + 6) +
+                      0.5 * (double)zlen * np * np;
         if (best < 0 || cost < best) {
             best = cost;
             P->np = np;
@@ -554,18 +578,42 @@ static uint64_t splitmix64(uint64_t* x)
 
 static uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
 
-extern "C" void fa_synth_limbs(uint64_t seed, uint64_t* a, uint64_t n, int top_nonzero)
-{
-    uint64_t s[4], x = seed;
-    for (int i = 0; i < 4; i++) s[i] = splitmix64(&x);
-    auto next = [&]() {
+namespace {
+struct synth_rng {
+    uint64_t s[4];
+    explicit synth_rng(uint64_t seed)
+    {
+        uint64_t x = seed;
+        for (int i = 0; i < 4; i++) s[i] = splitmix64(&x);
+    }
+    uint64_t next()
+    {
         uint64_t result = rotl64(s[1] * 5, 7) * 9;
         uint64_t t = s[1] << 17;
         s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
         s[2] ^= t; s[3] = rotl64(s[3], 45);
         return result;
-    };
-    for (uint64_t i = 0; i < n; i++) a[i] = next();
-    if (top_nonzero && n > 0)
-        while (a[n - 1] == 0) a[n - 1] = next();
+    }
+    void limbs(uint64_t* a, uint64_t n, int top_nonzero)
+    {
+        for (uint64_t i = 0; i < n; i++) a[i] = next();
+        if (top_nonzero && n > 0)
+            while (a[n - 1] == 0) a[n - 1] = next();
+    }
+};
+}  // namespace
+
+extern "C" void fa_synth_limbs(uint64_t seed, uint64_t* a, uint64_t n, int top_nonzero)
+{
+    synth_rng r(seed);
+    r.limbs(a, n, top_nonzero);
+}
+
+// both operands of one product from one stream: a then b, top limbs nonzero
+extern "C" void fa_synth_mpn_pair(uint64_t seed, uint64_t* a, uint64_t an, uint64_t* b,
+                                  uint64_t bn)
+{
+    synth_rng r(seed);
+    r.limbs(a, an, 1);
+    r.limbs(b, bn, 1);
 }

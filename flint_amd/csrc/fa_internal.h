@@ -75,6 +75,9 @@ struct fa_mpn_plan_t {
 extern "C++" {
 int fa_set_error(int code, const char* fmt, ...);
 fa_tables* fa_get_tables(int depth);
+int fa_prime_info(uint64_t n, fa_prime_host* pr);
+int fa_build_prime_tables(const fa_prime_host* pr, int depth, double** w2, double** w2inv);
+int fa_mulmod_satisfies_bounds(uint64_t n);
 fa_thread_state* fa_state();
 int fa_scratch_reserve(fa_thread_state* st, size_t bytes, void** out);
 int fa_hbuf_reserve(fa_thread_state* st, size_t bytes, void** out);
@@ -103,3 +106,19 @@ int fa_convolve(fa_thread_state* st, const fa_tables* T, int np, int offset, int
                 const fa_pack_desc* a, const fa_pack_desc* b, double* A, double* B,
                 hipStream_t stream);
 }
+
+// ---------------------------------------------------------------------------
+// optional per-kernel timing with HIP events on the launching stream
+// ---------------------------------------------------------------------------
+extern "C++" {
+int fa_prof_on();
+void fa_prof_begin(const char* name, hipStream_t s);
+void fa_prof_end(hipStream_t s);
+}
+
+#define FA_LAUNCH(name, kern, grid, block, lds, stream, ...)                          \
+    do {                                                                              \
+        if (fa_prof_on()) fa_prof_begin(name, stream);                                \
+        hipLaunchKernelGGL(kern, grid, block, lds, stream, __VA_ARGS__);              \
+        if (fa_prof_on()) fa_prof_end(stream);                                        \
+    } while (0)

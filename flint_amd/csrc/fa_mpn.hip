@@ -522,7 +522,7 @@ int fa_convolve(fa_thread_state* st, const fa_tables* T, int np, int offset, int
     const uint64_t stride = UINT64_C(1) << depth;
     if (depth <= FA_ROW_LG) {
         size_t lds = 2 * stride * sizeof(double);
-        hipLaunchKernelGGL(fa_k_small_conv, dim3(np), dim3(256), lds, stream, oa, ob, depth, A,
+        FA_LAUNCH("fa_k_small_conv", fa_k_small_conv, dim3(np), dim3(256), lds, stream, oa, ob, depth, A,
                            stride, S);
         LAUNCH_CHECK();
         return FA_OK;
@@ -531,14 +531,14 @@ int fa_convolve(fa_thread_state* st, const fa_tables* T, int np, int offset, int
     const int lgC = col_lg(d1, d2);
     const size_t col_lds = ((size_t)1 << d1 << lgC) * sizeof(double);
     dim3 gcol((unsigned)(1u << (d2 - lgC)), (unsigned)np, 2);
-    hipLaunchKernelGGL(fa_k_col_fwd, gcol, dim3(256), col_lds, stream, oa, ob, d1, d2, lgC, A, B,
+    FA_LAUNCH("fa_k_col_fwd", fa_k_col_fwd, gcol, dim3(256), col_lds, stream, oa, ob, d1, d2, lgC, A, B,
                        stride, S);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(fa_k_row_conv, dim3(1u << d1, np), dim3(256),
+    FA_LAUNCH("fa_k_row_conv", fa_k_row_conv, dim3(1u << d1, np), dim3(256),
                        2 * ((size_t)1 << d2) * sizeof(double), stream, A, B, d2, stride, S);
     LAUNCH_CHECK();
     dim3 gci((unsigned)(1u << (d2 - lgC)), (unsigned)np, 1);
-    hipLaunchKernelGGL(fa_k_col_inv, gci, dim3(256), col_lds, stream, A, d1, d2, lgC, stride, S);
+    FA_LAUNCH("fa_k_col_inv", fa_k_col_inv, gci, dim3(256), col_lds, stream, A, d1, d2, lgC, stride, S);
     LAUNCH_CHECK();
     return FA_OK;
 }
@@ -595,12 +595,12 @@ static int mpn_mul_device(uint64_t* z, const uint64_t* a, uint64_t an, const uin
     fa_crt_consts C;
     fa_crt_consts_init(&C, P.np, 0);
     const uint64_t stride = UINT64_C(1) << P.depth;
-    hipLaunchKernelGGL(fa_k_crt_mpn, dim3((unsigned)nblk), dim3(FA_CRT_WG), 0, stream, W.A,
+    FA_LAUNCH("fa_k_crt_mpn", fa_k_crt_mpn, dim3((unsigned)nblk), dim3(FA_CRT_WG), 0, stream, W.A,
                        stride, C, P.zlen, P.bits, z, zn, ngroups, W.flags, W.agg);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(fa_k_carry_scan, dim3(1), dim3(256), 0, stream, W.agg, W.cin, nblk);
+    FA_LAUNCH("fa_k_carry_scan", fa_k_carry_scan, dim3(1), dim3(256), 0, stream, W.agg, W.cin, nblk);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(fa_k_carry_apply, dim3((unsigned)nblk), dim3(FA_CRT_WG), 0, stream,
+    FA_LAUNCH("fa_k_carry_apply", fa_k_carry_apply, dim3((unsigned)nblk), dim3(FA_CRT_WG), 0, stream,
                        W.flags, W.cin, z, zn, P.bits / 8, ngroups);
     LAUNCH_CHECK();
     return FA_OK;

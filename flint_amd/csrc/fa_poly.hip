@@ -1,0 +1,389 @@
+// fa_poly.hip -- polynomial products over Z/nZ and Z on the convolution core.
+//
+//   fa_nmod_poly_mul_mid   restates _nmod_poly_mul_mid_mpn_ctx
+//                          (nmod_poly_mul.c:904-972, plan.c:356-398)
+//   fa_fmpz_poly_mul_mid   restates _fmpz_poly_mul_mid_mpn_ctx
+//                          (fmpz_poly_mul.c:319-401)
+//
+// Both import coefficients into np residue rows, run the transform /
+// pointwise / inverse core (fa_convolve) and recombine each output
+// coefficient from its residues with the CRT of fa_crt_coeff; nmod then
+// reduces the exact integer modulo n, fmpz emits it as a centred signed
+// value.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "fa_internal.h"
+#include "fa_ntt.h"
+
+typedef unsigned __int128 u128;
+
+// exact integer coefficient from np residues (same recombination as the
+// mpn path: x_i = c * cop_i^-1, c = sum x_i (M/p_i) - floor(sum x_i/p_i) M)
+FA_DEV void fa_crt_coeff_poly(const double* __restrict__ Q, uint64_t stride, uint64_t j,
+                              const fa_crt_consts& C, uint64_t s[FA_CRT_MAXL])
+{
+#pragma unroll
+    for (int l = 0; l < FA_CRT_MAXL; l++) s[l] = 0;
+    double qe = 0.0;
+    for (int i = 0; i < C.np; i++) {
+        uint64_t x = fa_reduce_0n_u64(Q[(uint64_t)i * stride + j], C.p[i], C.pinv[i]);
+        qe += (double)x * C.rp[i];
+        uint64_t carry = 0;
+#pragma unroll
+        for (int l = 0; l < FA_CRT_MAXL; l++) {
+            uint64_t lo = x * C.cof[i][l];
+            uint64_t hi = __umul64hi(x, C.cof[i][l]);
+            uint64_t t = s[l] + lo;
+            hi += (t < lo);
+            t += carry;
+            hi += (t < carry);
+            s[l] = t;
+            carry = hi;
+        }
+    }
+    uint64_t q = (uint64_t)qe;
+    uint64_t borrow = 0, carry = 0;
+#pragma unroll
+    for (int l = 0; l < FA_CRT_MAXL; l++) {
+        uint64_t lo = q * C.M[l];
+        uint64_t hi = __umul64hi(q, C.M[l]);
+        lo += carry;
+        hi += (lo < carry);
+        carry = hi;
+        uint64_t t = s[l] - lo;
+        uint64_t b1 = (s[l] < lo);
+        uint64_t t2 = t - borrow;
+        uint64_t b2 = (t < borrow);
+        s[l] = t2;
+        borrow = b1 | b2;
+    }
+    if (borrow) {
+        uint64_t c2 = 0;
+#pragma unroll
+        for (int l = 0; l < FA_CRT_MAXL; l++) {
+            uint64_t t = s[l] + C.M[l];
+            uint64_t cc = (t < s[l]);
+            t += c2;
+            cc |= (t < c2);
+            s[l] = t;
+            c2 = cc;
+        }
+    } else {
+        int ge = 1;
+#pragma unroll
+        for (int l = FA_CRT_MAXL - 1; l >= 0; l--)
+            if (ge == 1 && s[l] != C.M[l]) ge = (s[l] > C.M[l]) ? 2 : 0;
+        if (ge) {
+            uint64_t br = 0;
+#pragma unroll
+            for (int l = 0; l < FA_CRT_MAXL; l++) {
+                uint64_t t = s[l] - C.M[l];
+                uint64_t b1 = (s[l] < C.M[l]);
+                uint64_t t2 = t - br;
+                uint64_t b2 = (t < br);
+                s[l] = t2;
+                br = b1 | b2;
+            }
+        }
+    }
+}
+
+// (hi*2^64 + lo) mod n with hi < n, by the Moller-Granlund preinverse
+// (udiv_qrnnd_preinv; dinv = floor((2^128-1)/d) - 2^64 for d = n << norm,
+// the ninv of FLINT's nmod_t, nmod.h:284)
+FA_DEV uint64_t fa_ll_mod_preinv(uint64_t hi, uint64_t lo, uint64_t n, uint64_t dinv, int norm)
+{
+    uint64_t d = n << norm;
+    if (norm) {
+        hi = (hi << norm) | (lo >> (64 - norm));
+        lo <<= norm;
+    }
+    uint64_t p_lo = dinv * hi;
+    uint64_t p_hi = __umul64hi(dinv, hi);
+    uint64_t q0 = p_lo + lo;
+    uint64_t q1 = p_hi + hi + 1 + (q0 < p_lo);
+    uint64_t r = lo - q1 * d;
+    if (r > q0) r += d;
+    if (r >= d) r -= d;
+    return r >> norm;
+}
+
+__global__ void __launch_bounds__(256) fa_k_crt_nmod(const double* __restrict__ Q, uint64_t stride,
+                                                     fa_crt_consts C, uint64_t zl, uint64_t cnt,
+                                                     uint64_t n, uint64_t dinv, int norm,
+                                                     uint64_t* __restrict__ z)
+{
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    uint64_t s[FA_CRT_MAXL];
+    fa_crt_coeff_poly(Q, stride, zl + t, C, s);
+    uint64_t r = 0;
+#pragma unroll
+    for (int l = FA_CRT_MAXL - 1; l >= 0; l--) r = fa_ll_mod_preinv(r, s[l], n, dinv, norm);
+    z[t] = r;
+}
+
+// centred lift (fmpz_poly_mul.c:209-227): s > M/2 -> s - M, written as a
+// zw-limb two's complement value
+__global__ void __launch_bounds__(256) fa_k_crt_fmpz(const double* __restrict__ Q, uint64_t stride,
+                                                     fa_crt_consts C, uint64_t zl, uint64_t cnt,
+                                                     uint64_t zw, uint64_t* __restrict__ z)
+{
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    uint64_t s[FA_CRT_MAXL];
+    fa_crt_coeff_poly(Q, stride, zl + t, C, s);
+    // compare s with floor(M/2)
+    int gt = 0;
+    for (int l = FA_CRT_MAXL - 1; l >= 0; l--) {
+        uint64_t h = (C.M[l] >> 1) | ((l + 1 < FA_CRT_MAXL) ? (C.M[l + 1] << 63) : 0);
+        if (s[l] != h) { gt = s[l] > h; break; }
+    }
+    uint64_t ext = 0;
+    if (gt) {
+        uint64_t br = 0;
+#pragma unroll
+        for (int l = 0; l < FA_CRT_MAXL; l++) {
+            uint64_t t1 = s[l] - C.M[l];
+            uint64_t b1 = (s[l] < C.M[l]);
+            uint64_t t2 = t1 - br;
+            uint64_t b2 = (t1 < br);
+            s[l] = t2;
+            br = b1 | b2;
+        }
+        ext = ~UINT64_C(0);
+    }
+    uint64_t* o = z + t * zw;
+    for (uint64_t l = 0; l < zw; l++) {
+        uint64_t v = ext;
+#pragma unroll
+        for (int k = 0; k < FA_CRT_MAXL; k++)
+            if ((uint64_t)k == l) v = s[k];
+        o[l] = v;
+    }
+}
+
+// ------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------
+
+static int clog2_u64(uint64_t x) { return x <= 1 ? 0 : 64 - __builtin_clzll(x - 1); }
+static int nbits_u64(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
+
+// M_np >= c * 2^e (flint_mpn_cmp_ui_2exp as used by plan.c:28-32)
+static int prod_ge(const fa_crt_consts& C, uint64_t c, uint64_t e)
+{
+    uint64_t t[FA_CRT_MAXL + 4];
+    memset(t, 0, sizeof(t));
+    uint64_t q = e / 64, r = e % 64;
+    if (q + 2 > FA_CRT_MAXL + 3) return 0;
+    t[q] = c << r;
+    if (r) t[q + 1] = c >> (64 - r);
+    for (int i = FA_CRT_MAXL + 3; i >= 0; i--) {
+        uint64_t mi = (i < FA_CRT_MAXL) ? C.M[i] : 0;
+        if (mi != t[i]) return mi > t[i];
+    }
+    return 1;
+}
+
+struct poly_job {
+    int np, offset, depth;
+    fa_crt_consts C;
+};
+
+// device buffers for one polynomial product: operands, residue rows, output
+static int poly_buffers(fa_thread_state* st, uint64_t an_words, uint64_t bn_words, int np,
+                        int depth, uint64_t zwords, uint64_t** da, uint64_t** db, double** A,
+                        double** B, uint64_t** dz)
+{
+    size_t tr = ((size_t)np << depth) * sizeof(double);
+    size_t bytes = 2 * tr + (an_words + bn_words + zwords) * sizeof(uint64_t) + 256;
+    void* buf;
+    int rc = fa_scratch_reserve(st, bytes, &buf);
+    if (rc) return rc;
+    *A = (double*)buf;
+    *B = (double*)((char*)buf + tr);
+    *da = (uint64_t*)((char*)buf + 2 * tr);
+    *db = *da + an_words;
+    *dz = *db + bn_words;
+    return FA_OK;
+}
+
+#define HIPTRY(x, msg)                                                                  \
+    do {                                                                                \
+        if ((x) != hipSuccess) return fa_set_error(FA_E_HIP, "%s: %s", msg,             \
+                                                   hipGetErrorString(hipGetLastError())); \
+    } while (0)
+
+extern "C" int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh, const uint64_t* a,
+                                    int64_t an, const uint64_t* b, int64_t bn, fa_nmod_t mod)
+{
+    if (an <= 0 || bn <= 0 || !a || !b || mod.n == 0)
+        return fa_set_error(FA_E_ARG, "fa_nmod_poly_mul_mid: bad arguments");
+    if (zl < 0 || zl >= zh) return FA_OK;
+    if (!z) return fa_set_error(FA_E_ARG, "fa_nmod_poly_mul_mid: null output");
+    int64_t zn = an + bn - 1;
+    if (zh > zn) {
+        int64_t s0 = zl > zn ? zl : zn;
+        memset(z + (s0 - zl), 0, (size_t)(zh - s0) * sizeof(uint64_t));
+        if (zl >= zn) return FA_OK;
+        zh = zn;
+    }
+    uint64_t n = mod.n;
+    if (n == 1) {
+        memset(z, 0, (size_t)(zh - zl) * sizeof(uint64_t));
+        return FA_OK;
+    }
+    fa_tables* T = fa_get_tables(clog2_u64((uint64_t)zn));
+    if (!T) return FA_E_NODEV;
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+
+    // prime set (plan.c:356-398): the modulus itself when it is one of the
+    // FFT primes, otherwise the fewest primes with prod >= len * 2^(2 modbits)
+    int modbits = nbits_u64(n);
+    poly_job J;
+    J.np = 0;
+    J.offset = 0;
+    if (modbits == 50)
+        for (int i = 0; i < FA_NPRIMES; i++)
+            if (T->pr[i].n == n) { J.np = 1; J.offset = i; }
+    if (!J.np) {
+        uint64_t len = (uint64_t)(an < bn ? an : bn);
+        for (int np = 1; np <= FA_NPRIMES && !J.np; np++) {
+            fa_crt_consts C;
+            fa_crt_consts_init(&C, np, 0);
+            if (prod_ge(C, len, 2 * (uint64_t)modbits)) J.np = np;
+        }
+    }
+    if (!J.np) return fa_set_error(FA_E_RANGE, "no prime set for modulus");
+    fa_crt_consts_init(&J.C, J.np, J.offset);
+    J.depth = clog2_u64((uint64_t)zn);
+    if (J.depth < 4) J.depth = 4;
+    if (J.depth > FA_MAX_DEPTH)
+        return fa_set_error(FA_E_RANGE, "nmod product length %lld above 2^%d", (long long)zn,
+                            FA_MAX_DEPTH);
+
+    uint64_t cnt = (uint64_t)(zh - zl);
+    uint64_t *da, *db, *dz;
+    double *A, *B;
+    int rc = poly_buffers(st, an, bn, J.np, J.depth, cnt, &da, &db, &A, &B, &dz);
+    if (rc) return rc;
+    hipStream_t s = st->stream;
+    HIPTRY(hipMemcpyAsync(da, a, an * sizeof(uint64_t), hipMemcpyHostToDevice, s), "upload a");
+    HIPTRY(hipMemcpyAsync(db, b, bn * sizeof(uint64_t), hipMemcpyHostToDevice, s), "upload b");
+    fa_pack_desc pa = {FA_PACK_U64, da, (uint64_t)an, 0, 1};
+    fa_pack_desc pb = {FA_PACK_U64, db, (uint64_t)bn, 0, 1};
+    rc = fa_convolve(st, T, J.np, J.offset, J.depth, &pa, &pb, A, B, s);
+    if (rc) return rc;
+    int norm = 64 - modbits;
+    uint64_t d = n << norm;
+    uint64_t dinv = (uint64_t)(((u128)0 - 1) / d - ((u128)1 << 64));
+    FA_LAUNCH("fa_k_crt_nmod", fa_k_crt_nmod, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
+              A, UINT64_C(1) << J.depth, J.C, (uint64_t)zl, cnt, n, dinv, norm, dz);
+    HIPTRY(hipGetLastError(), "crt launch");
+    HIPTRY(hipMemcpyAsync(z, dz, cnt * sizeof(uint64_t), hipMemcpyDeviceToHost, s), "download");
+    HIPTRY(hipStreamSynchronize(s), "synchronize");
+    return FA_OK;
+}
+
+extern "C" void _nmod_poly_mul_mid_default_mpn_ctx(uint64_t* res, int64_t zl, int64_t zh,
+                                                   const uint64_t* a, int64_t an,
+                                                   const uint64_t* b, int64_t bn, fa_nmod_t mod)
+{
+    int rc = fa_nmod_poly_mul_mid(res, zl, zh, a, an, b, bn, mod);
+    if (rc != FA_OK) {
+        fprintf(stderr, "_nmod_poly_mul_mid_default_mpn_ctx: %s\n", fa_last_error());
+        abort();
+    }
+}
+
+// _fmpz_vec_max_bits: max bit size of |c|, negated if some c < 0
+static int64_t vec_max_bits(const uint64_t* v, uint64_t w, int64_t n)
+{
+    uint64_t mb = 0;
+    int neg = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const uint64_t* c = v + (uint64_t)i * w;
+        int s = (int64_t)c[w - 1] < 0;
+        neg |= s;
+        uint64_t bits = 0;
+        if (!s) {
+            for (int64_t k = (int64_t)w - 1; k >= 0; k--)
+                if (c[k]) { bits = 64 * (uint64_t)k + nbits_u64(c[k]); break; }
+        } else {
+            // |c| = ~c + 1, limb by limb from the bottom
+            uint64_t cy = 1;
+            for (uint64_t k = 0; k < w; k++) {
+                uint64_t t = ~c[k] + cy;
+                cy = (cy && t == 0);
+                if (t) bits = 64 * k + nbits_u64(t);
+            }
+        }
+        if (bits > mb) mb = bits;
+    }
+    return neg ? -(int64_t)mb : (int64_t)mb;
+}
+
+extern "C" int fa_fmpz_poly_mul_mid(uint64_t* z, uint64_t zw, int64_t zl, int64_t zh,
+                                    const uint64_t* a, uint64_t aw, int64_t an, const uint64_t* b,
+                                    uint64_t bw, int64_t bn)
+{
+    if (an <= 0 || bn <= 0 || !a || !b || aw == 0 || bw == 0 || zw == 0)
+        return fa_set_error(FA_E_ARG, "fa_fmpz_poly_mul_mid: bad arguments");
+    if (zl < 0 || zl >= zh) return 1;
+    if (!z) return fa_set_error(FA_E_ARG, "fa_fmpz_poly_mul_mid: null output");
+    int64_t zn = an + bn - 1;
+    if (zh > zn) {
+        int64_t s0 = zl > zn ? zl : zn;
+        memset(z + (uint64_t)(s0 - zl) * zw, 0, (size_t)(zh - s0) * zw * sizeof(uint64_t));
+        if (zl >= zn) return 1;
+        zh = zn;
+    }
+    int64_t b1 = vec_max_bits(a, aw, an);
+    int64_t b2 = vec_max_bits(b, bw, bn);
+    uint64_t modbits = (uint64_t)(b1 < 0 ? -b1 : b1) + (uint64_t)(b2 < 0 ? -b2 : b2) + 1;
+    // need prod_of_primes >= bn * 2^modbits (fmpz_poly_mul.c:374-379)
+    int np = 0;
+    for (int k = 1; k <= FA_NPRIMES && !np; k++) {
+        fa_crt_consts C;
+        fa_crt_consts_init(&C, k, 0);
+        if (prod_ge(C, (uint64_t)bn, modbits)) np = k;
+    }
+    if (!np) return 0;
+    int depth = clog2_u64((uint64_t)zn);
+    if (depth < 4) depth = 4;
+    if (depth > FA_MAX_DEPTH)
+        return fa_set_error(FA_E_RANGE, "fmpz product length %lld above 2^%d", (long long)zn,
+                            FA_MAX_DEPTH);
+    fa_tables* T = fa_get_tables(depth);
+    if (!T) return FA_E_NODEV;
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    fa_crt_consts C;
+    fa_crt_consts_init(&C, np, 0);
+    uint64_t cnt = (uint64_t)(zh - zl);
+    uint64_t *da, *db, *dz;
+    double *A, *B;
+    int rc = poly_buffers(st, (uint64_t)an * aw, (uint64_t)bn * bw, np, depth, cnt * zw, &da, &db,
+                          &A, &B, &dz);
+    if (rc) return rc;
+    hipStream_t s = st->stream;
+    HIPTRY(hipMemcpyAsync(da, a, (uint64_t)an * aw * sizeof(uint64_t), hipMemcpyHostToDevice, s),
+           "upload a");
+    HIPTRY(hipMemcpyAsync(db, b, (uint64_t)bn * bw * sizeof(uint64_t), hipMemcpyHostToDevice, s),
+           "upload b");
+    fa_pack_desc pa = {FA_PACK_S64W, da, (uint64_t)an, 0, (int)aw};
+    fa_pack_desc pb = {FA_PACK_S64W, db, (uint64_t)bn, 0, (int)bw};
+    rc = fa_convolve(st, T, np, 0, depth, &pa, &pb, A, B, s);
+    if (rc) return rc;
+    FA_LAUNCH("fa_k_crt_fmpz", fa_k_crt_fmpz, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
+              A, UINT64_C(1) << depth, C, (uint64_t)zl, cnt, zw, dz);
+    HIPTRY(hipGetLastError(), "crt launch");
+    HIPTRY(hipMemcpyAsync(z, dz, cnt * zw * sizeof(uint64_t), hipMemcpyDeviceToHost, s),
+           "download");
+    HIPTRY(hipStreamSynchronize(s), "synchronize");
+    return 1;
+}

@@ -5,8 +5,8 @@
  * Reference interfaces are cited as file:line of flintlib/flint 3.7.0-dev.
  * Plain pointers and sizes only.  Functions returning int report
  *   FA_OK (0) on success, a negative FA_E* code on failure
- * (fa_last_error() has the message); the fmpz entry points additionally
- * return FA_DECLINED (1... see below) exactly where the reference declines.
+ * (fa_last_error() has the message); the fmpz entry points return 1 on
+ * success and 0 exactly where the reference declines.
  *
  * Host-pointer entry points are synchronous: they return after the output
  * is in host memory.  Device-pointer entry points (suffix _device) enqueue
@@ -23,6 +23,8 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+#define FA_API __attribute__((visibility("default")))
 
 #define FA_OK 0
 #define FA_E_ARG (-1)       /* invalid argument */
@@ -45,12 +47,12 @@ typedef struct {
 /* replaces mpn_mul_default_mpn_ctx, fft_small.h:852, default_ctx.c:41-45:
    r1[0, n1+n2) = i1 * i2.  Aborts the process on device failure, as the
    reference's flint_throw does (sd_fft_ctx.c:63-74). */
-void mpn_mul_default_mpn_ctx(uint64_t* r1, const uint64_t* i1, int64_t n1,
+FA_API void mpn_mul_default_mpn_ctx(uint64_t* r1, const uint64_t* i1, int64_t n1,
                              const uint64_t* i2, int64_t n2);
 
 /* replaces _nmod_poly_mul_mid_default_mpn_ctx, fft_small.h:853,
    default_ctx.c:47-51: res[0, zh-zl) = coefficients [zl, zh) of a*b mod n */
-void _nmod_poly_mul_mid_default_mpn_ctx(uint64_t* res, int64_t zl, int64_t zh,
+FA_API void _nmod_poly_mul_mid_default_mpn_ctx(uint64_t* res, int64_t zl, int64_t zh,
                                         const uint64_t* a, int64_t an,
                                         const uint64_t* b, int64_t bn,
                                         fa_nmod_t mod);
@@ -60,18 +62,18 @@ void _nmod_poly_mul_mid_default_mpn_ctx(uint64_t* res, int64_t zl, int64_t zh,
 /* ---------------------------------------------------------------------- */
 
 /* mpn_ctx_mpn_mul, mpn_mul.c:1980-1983 (full product, an, bn >= 1) */
-int fa_mpn_mul(uint64_t* z, const uint64_t* a, int64_t an,
+FA_API int fa_mpn_mul(uint64_t* z, const uint64_t* a, int64_t an,
                const uint64_t* b, int64_t bn);
 
 /* same with operands and output in device memory (no PCIe copies) */
-int fa_mpn_mul_device(uint64_t* z, const uint64_t* a, int64_t an,
+FA_API int fa_mpn_mul_device(uint64_t* z, const uint64_t* a, int64_t an,
                       const uint64_t* b, int64_t bn, void* stream);
 
 /* plan the device call would use: out = {np, bits, depth, d1, d2, zlen} */
-int fa_mpn_plan(int64_t an, int64_t bn, int64_t* out);
+FA_API int fa_mpn_plan(int64_t an, int64_t bn, int64_t* out);
 
 /* _nmod_poly_mul_mid_mpn_ctx, nmod_poly_mul.c:904-972 */
-int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh,
+FA_API int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh,
                          const uint64_t* a, int64_t an,
                          const uint64_t* b, int64_t bn, fa_nmod_t mod);
 
@@ -81,7 +83,7 @@ int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh,
    (fmpz.h:238-239) convert to and from.  Returns 1 on success and 0 when
    the reference declines (more than 8 primes needed, fmpz_poly_mul.c:
    377-379), leaving z untouched; negative on error. */
-int fa_fmpz_poly_mul_mid(uint64_t* z, uint64_t zw, int64_t zl, int64_t zh,
+FA_API int fa_fmpz_poly_mul_mid(uint64_t* z, uint64_t zw, int64_t zl, int64_t zh,
                          const uint64_t* a, uint64_t aw, int64_t an,
                          const uint64_t* b, uint64_t bw, int64_t bn);
 
@@ -99,32 +101,48 @@ typedef struct {
 } fa_sd_fft_ctx_struct;
 
 /* sd_fft_ctx_init_prime, sd_fft_ctx.c:56-154 */
-int fa_sd_fft_ctx_init_prime(fa_sd_fft_ctx_struct* Q, uint64_t pp);
+FA_API int fa_sd_fft_ctx_init_prime(fa_sd_fft_ctx_struct* Q, uint64_t pp);
 /* sd_fft_ctx_clear, sd_fft_ctx.c:16-26 */
-void fa_sd_fft_ctx_clear(fa_sd_fft_ctx_struct* Q);
+FA_API void fa_sd_fft_ctx_clear(fa_sd_fft_ctx_struct* Q);
 /* sd_fft_trunc, sd_fft.c:1230-1274 (host data, in place) */
-int fa_sd_fft_trunc(fa_sd_fft_ctx_struct* Q, double* d, uint64_t L,
+FA_API int fa_sd_fft_trunc(fa_sd_fft_ctx_struct* Q, double* d, uint64_t L,
                     uint64_t itrunc, uint64_t otrunc);
 /* sd_ifft_trunc, sd_ifft.c:1503-1533 (host data, in place) */
-int fa_sd_ifft_trunc(fa_sd_fft_ctx_struct* Q, double* d, uint64_t L, uint64_t trunc);
+FA_API int fa_sd_ifft_trunc(fa_sd_fft_ctx_struct* Q, double* d, uint64_t L, uint64_t trunc);
 /* sd_fft_ctx_w, fft_small.h:308-312 */
-double fa_sd_fft_ctx_w(const fa_sd_fft_ctx_struct* Q, uint64_t j);
+FA_API double fa_sd_fft_ctx_w(const fa_sd_fft_ctx_struct* Q, uint64_t j);
 
 /* ---------------------------------------------------------------------- */
 /* library state                                                          */
 /* ---------------------------------------------------------------------- */
 
-const char* fa_last_error(void);
-int fa_init(void);                   /* optional: create device tables now */
-int fa_default_primes(uint64_t* out8);
+FA_API const char* fa_last_error(void);
+FA_API int fa_init(void);                   /* optional: create device tables now */
+FA_API int fa_default_primes(uint64_t* out8);
 /* synchronize the calling thread's stream */
-int fa_sync(void);
+FA_API int fa_sync(void);
 /* the calling thread's stream (hipStream_t) */
-void* fa_stream(void);
+FA_API void* fa_stream(void);
 
 /* seeded synthetic inputs (SplitMix64 -> xoshiro256**), identical to the
    generator the tests use, written on the host */
-void fa_synth_limbs(uint64_t seed, uint64_t* a, uint64_t n, int top_nonzero);
+FA_API void fa_synth_limbs(uint64_t seed, uint64_t* a, uint64_t n, int top_nonzero);
+/* both operands of one product from one stream: a, then b, top limbs nonzero */
+FA_API void fa_synth_mpn_pair(uint64_t seed, uint64_t* a, uint64_t an, uint64_t* b, uint64_t bn);
+
+/* ---------------------------------------------------------------------- */
+/* measurement support                                                    */
+/* ---------------------------------------------------------------------- */
+
+/* per-kernel timing with HIP events recorded around each launch on the
+   launching stream (calling thread only) */
+FA_API void fa_prof_enable(int on);
+FA_API void fa_prof_reset(void);
+/* names: max*64 chars; ms: total milliseconds; counts: launches */
+FA_API int fa_prof_read(char* names, double* ms, int64_t* counts, int max);
+/* plain device-to-device copy kernel (nbytes multiple of 16), used to
+   calibrate hardware byte counters */
+FA_API int fa_copy_device(void* dst, const void* src, uint64_t nbytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,119 @@
+"""GPU parity of _nmod_poly_mul_mid / _fmpz_poly_mul_mid (drop-ins)
+against the golden fixtures (Kronecker products on Python integers) and the
+oracle, including windows, zero padding past the product and the decline
+contract of the fmpz entry (fmpz_poly_mul.c:377-379)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.golden.make_fixtures import poly_ks
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_nmod_vs_fixtures_and_windows():
+    import flint_amd as fa
+    f = np.load(os.path.join(GOLDEN, "poly_small.npz"))
+    for k, ns in enumerate(f["nmod_moduli"]):
+        n = int(ns)
+        a, b, z = f["nmod_a%d" % k], f["nmod_b%d" % k], f["nmod_z%d" % k]
+        zn = len(a) + len(b) - 1
+        print("nmod", n, len(a), len(b), flush=True)
+        assert np.array_equal(fa.nmod_poly_mul_mid(a, b, 0, zn, n), z)
+        lo, hi = zn // 3, zn + 7
+        mid = fa.nmod_poly_mul_mid(a, b, lo, hi, n)
+        assert np.array_equal(mid[: zn - lo], z[lo:])
+        assert not mid[zn - lo:].any()
+
+
+@pytest.mark.parametrize("bits", [1, 2, 20, 31, 32, 33, 49, 50, 51, 52, 63, 64])
+def test_nmod_random_moduli(bits):
+    import flint_amd as fa
+    rng = np.random.default_rng(bits)
+    for rep in range(3):
+        n = int(rng.integers(1 << (bits - 1), (1 << bits) - 1, dtype=np.uint64, endpoint=True))
+        n = max(n, 1)
+        an, bn = int(rng.integers(1, 5000)), int(rng.integers(1, 5000))
+        a = (rng.integers(0, 2**63, an, dtype=np.uint64) % np.uint64(n)).astype(np.uint64)
+        b = (rng.integers(0, 2**63, bn, dtype=np.uint64) % np.uint64(n)).astype(np.uint64)
+        zn = an + bn - 1
+        z = fa.nmod_poly_mul_mid(a, b, 0, zn, n)
+        assert np.array_equal(z, O.nmod_poly_mul_mid(a, b, 0, zn, n))
+
+
+def test_nmod_fft_prime_moduli_and_long():
+    import flint_amd as fa
+    rng = np.random.default_rng(5)
+    for n in O.default_primes()[:3] + [7340033, 2**61 - 1]:
+        an, bn = 70000, 65000
+        a = (rng.integers(0, 2**63, an, dtype=np.uint64) % np.uint64(n)).astype(np.uint64)
+        b = (rng.integers(0, 2**63, bn, dtype=np.uint64) % np.uint64(n)).astype(np.uint64)
+        zn = an + bn - 1
+        z = fa.nmod_poly_mul_mid(a, b, 1000, zn - 1000, n)
+        ref = O.nmod_poly_mul_mid(a, b, 1000, zn - 1000, n)
+        assert np.array_equal(z, ref)
+
+
+def test_nmod_all_max_coefficients():
+    import flint_amd as fa
+    n = 2**64 - 59
+    a = np.full(3000, np.uint64(n - 1))
+    b = np.full(2000, np.uint64(n - 1))
+    zn = 4999
+    ref = [c % n for c in poly_ks([n - 1] * 3000, [n - 1] * 2000)]
+    assert [int(x) for x in fa.nmod_poly_mul_mid(a, b, 0, zn, n)] == ref
+
+
+def test_fmpz_vs_fixtures():
+    import flint_amd as fa
+    f = np.load(os.path.join(GOLDEN, "poly_small.npz"))
+    for k, (aw, bw, zw) in enumerate(f["fmpz_widths"]):
+        a, b, z = f["fmpz_a%d" % k], f["fmpz_b%d" % k], f["fmpz_z%d" % k]
+        an, bn = len(a) // aw, len(b) // bw
+        zn = an + bn - 1
+        print("fmpz", k, aw, an, bn, flush=True)
+        out = fa.fmpz_poly_mul_mid(a, int(aw), b, int(bw), 0, zn, int(zw))
+        assert out is not None
+        assert np.array_equal(out, z)
+
+
+def test_fmpz_declines_like_the_reference():
+    import flint_amd as fa
+    w = 4
+    an = bn = 10
+    a = np.zeros(an * w, dtype=np.uint64)
+    b = np.zeros(bn * w, dtype=np.uint64)
+    a[w - 1::w] = np.uint64(1 << 7)
+    b[w - 1::w] = np.uint64(1 << 7)
+    zw = 8
+    z = np.full((an + bn - 1) * zw, np.uint64(7))
+    assert fa.fmpz_poly_mul_mid(a, w, b, w, 0, an + bn - 1, zw) is None
+    ok, _ = O.fmpz_poly_mul_mid(a, w, b, w, 0, an + bn - 1, zw)
+    assert ok == 0
+
+
+@pytest.mark.parametrize("bits,signed", [(30, True), (62, False), (120, True), (180, True)])
+def test_fmpz_random_vs_oracle(bits, signed):
+    import flint_amd as fa
+    rng = np.random.default_rng(bits)
+    an, bn = 3000, 2500
+    w = bits // 64 + 1
+    zw = (2 * bits + 16) // 64 + 1
+
+    def draw(n):
+        v = []
+        for _ in range(n):
+            x = int.from_bytes(rng.bytes((bits + 7) // 8), "little") % (1 << bits)
+            v.append(-x if signed and rng.integers(0, 2) else x)
+        return v, np.frombuffer(b"".join((x % (1 << (64 * w))).to_bytes(8 * w, "little")
+                                         for x in v), dtype=np.uint64)
+    _, a = draw(an)
+    _, b = draw(bn)
+    zn = an + bn - 1
+    out = fa.fmpz_poly_mul_mid(a, w, b, w, 5, zn + 3, zw)
+    ok, ref = O.fmpz_poly_mul_mid(a, w, b, w, 5, zn + 3, zw)
+    assert ok == 1 and out is not None
+    assert np.array_equal(out, ref)
