@@ -43,10 +43,16 @@ WORKLOADS = {
 
 # array passes (8 bytes per point each) of every kernel of the mpn pipeline
 KERNEL_PASSES = {
-    "fa_k_col_fwd": 2,     # writes the transformed a and b columns; reads the limbs
-    "fa_k_row_conv": 3,    # reads a and b rows, writes the product row
-    "fa_k_col_inv": 2,     # reads and writes the product columns
-    "fa_k_crt_mpn": 1,     # reads the residues; writes the output limbs
+    "fa_k_col_fwd16w": 2,  # writes the transformed a and b columns; reads the limbs
+    "fa_k_col_inv16w": 2,  # reads and writes the product columns
+    "fa_k_col_fwd16": 2,
+    "fa_k_row_conv16": 3,  # reads a and b rows, writes the product row
+    "fa_k_col_inv16": 2,   # reads and writes the product columns
+    "fa_k_crt_mpn16": 1,   # reads the residues; writes the output limbs
+    "fa_k_col_fwd": 2,
+    "fa_k_row_conv": 3,
+    "fa_k_col_inv": 2,
+    "fa_k_crt_mpn": 1,
     "fa_k_small_conv": 1,
 }
 
@@ -54,9 +60,9 @@ KERNEL_PASSES = {
 def kernel_bytes(name, W):
     pts = W["ref_np"] * W["ref_T"]
     b = 8 * KERNEL_PASSES.get(name, 0) * pts
-    if name == "fa_k_col_fwd":
+    if name.startswith("fa_k_col_fwd"):
         b += 8 * (W["an"] + W["bn"])
-    if name == "fa_k_crt_mpn":
+    if name.startswith("fa_k_crt_mpn"):
         b += 8 * (W["an"] + W["bn"])
     return b
 
@@ -109,6 +115,9 @@ def main():
     ap.add_argument("--cpu-sample-log2", type=int, default=22)
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_mpn_2p24.json"))
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the library's plain copy kernel over MIB MiB, 5 times (the "
+                         "known-size reference for scaling HBM counter readings), and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -118,6 +127,27 @@ def main():
 
     import torch
     import flint_amd as fa
+
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        src = torch.ones(nbytes // 8, dtype=torch.int64, device="cuda:0")
+        dst = torch.empty_like(src)
+        stream = torch.cuda.Stream()
+        torch.cuda.set_stream(stream)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        fa.copy_device(dst.data_ptr(), src.data_ptr(), nbytes, stream.cuda_stream)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(5):
+            fa.copy_device(dst.data_ptr(), src.data_ptr(), nbytes, stream.cuda_stream)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / 5
+        assert bool(torch.equal(src, dst))
+        print(json.dumps({"copy_bytes": nbytes, "read_plus_write_bytes": 2 * nbytes,
+                          "ms": ms, "GB/s": 2 * nbytes / (ms * 1e-3) / 1e9}))
+        return
 
     W = WORKLOADS[args.workload]
     an, bn = W["an"], W["bn"]

@@ -48,6 +48,9 @@ SIGNATURES = {
     "fa_mpn_mul": (ctypes.c_int, [_P64, _P64, _s64, _P64, _s64]),
     "fa_mpn_mul_device": (ctypes.c_int, [_VP, _VP, _s64, _VP, _s64, _VP]),
     "fa_mpn_plan": (ctypes.c_int, [_s64, _s64, _P64]),
+    "fa_mpn_mul_plan": (ctypes.c_int, [_P64, _P64, _s64, _P64, _s64, ctypes.c_int, ctypes.c_int]),
+    "fa_mpn_mul_device_plan": (ctypes.c_int, [_VP, _VP, _s64, _VP, _s64, ctypes.c_int,
+                                              ctypes.c_int, _VP]),
     "fa_nmod_poly_mul_mid": (ctypes.c_int, [_P64, _s64, _s64, _P64, _s64, _P64, _s64, NmodT]),
     "fa_fmpz_poly_mul_mid": (ctypes.c_int, [_P64, _u64, _s64, _s64, _P64, _u64, _s64, _P64, _u64,
                                             _s64]),
@@ -67,6 +70,9 @@ SIGNATURES = {
     "fa_prof_reset": (None, []),
     "fa_prof_read": (ctypes.c_int, [ctypes.c_char_p, _PD, _P64, ctypes.c_int]),
     "fa_copy_device": (ctypes.c_int, [_VP, _VP, _u64, _VP]),
+    "fa_set_fast_kernels": (None, [ctypes.c_int]),
+    "fa_set_col_variant": (None, [ctypes.c_int]),
+    "fa_probe_colcopy": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP]),
 }
 
 _lib = None
@@ -144,6 +150,15 @@ def prof_read(max_entries=64):
     return out
 
 
+def set_fast_kernels(on=True):
+    lib().fa_set_fast_kernels(1 if on else 0)
+
+
+def set_col_variant(v):
+    """column-pass tiles at 2^24 points: 1 = 8 columns per workgroup, 0 = 4"""
+    lib().fa_set_col_variant(int(v))
+
+
 def copy_device(dst_ptr, src_ptr, nbytes, stream=None):
     return _check(lib().fa_copy_device(dst_ptr, src_ptr, nbytes, stream))
 
@@ -161,6 +176,19 @@ def mpn_mul(a, b):
     z = np.empty(len(a) + len(b), dtype=np.uint64)
     _check(lib().fa_mpn_mul(_p(z), _p(a), len(a), _p(b), len(b)))
     return z
+
+
+def mpn_mul_plan(a, b, np_, bits):
+    """product with a caller-chosen plan (np primes, bits-bit chunks)"""
+    a = _arr(a)
+    b = _arr(b)
+    z = np.empty(len(a) + len(b), dtype=np.uint64)
+    _check(lib().fa_mpn_mul_plan(_p(z), _p(a), len(a), _p(b), len(b), np_, bits))
+    return z
+
+
+def mpn_mul_device_plan(z_ptr, a_ptr, an, b_ptr, bn, np_, bits, stream=None):
+    return _check(lib().fa_mpn_mul_device_plan(z_ptr, a_ptr, an, b_ptr, bn, np_, bits, stream))
 
 
 def mpn_mul_device(z_ptr, a_ptr, an, b_ptr, bn, stream=None):

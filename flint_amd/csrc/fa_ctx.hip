@@ -186,8 +186,8 @@ int fa_prime_info(uint64_t n, fa_prime_host* pr)
     pr->pinv = 1.0 / pr->p;
     pr->v = __builtin_ctzll(n - 1);
     pr->g = primitive_2power_root(n);
-    for (int k = 0; k < 8; k++)
-        pr->two32pow[k] = fa_signed_rep(fa_powmod_u64(2, 32 * (uint64_t)k, n), n);
+    static const int e[8] = {32, 48, 96, 144, 192, 0, 0, 0};
+    for (int k = 0; k < 8; k++) pr->pow2[k] = fa_signed_rep(fa_powmod_u64(2, e[k], n), n);
     return FA_OK;
 }
 
@@ -469,7 +469,7 @@ void fa_prime_set_init(fa_prime_set* S, const fa_tables* T, int np, int depth, i
         const fa_prime_host* pr = &T->pr[offset + i];
         S->p[i] = pr->p;
         S->pinv[i] = pr->pinv;
-        for (int k = 0; k < 8; k++) S->two32pow[i][k] = pr->two32pow[k];
+        for (int k = 0; k < 8; k++) S->pow2[i][k] = pr->pow2[k];
         S->w2[i] = T->w2[offset + i];
         S->w2inv[i] = T->w2inv[offset + i];
         if (with_norm) {
@@ -550,6 +550,39 @@ int fa_mpn_choose_plan(uint64_t an, uint64_t bn, fa_mpn_plan_t* P)
                             (unsigned long long)an, (unsigned long long)bn, FA_MAX_DEPTH);
     P->d2 = P->depth <= FA_ROW_LG ? P->depth : FA_ROW_LG;
     P->d1 = P->depth - P->d2;
+    return FA_OK;
+}
+
+// a caller-chosen (np, bits) checked against the same exactness and
+// geometry constraints as the automatic choice
+int fa_mpn_fixed_plan(uint64_t an, uint64_t bn, int np, int bits, fa_mpn_plan_t* P)
+{
+    if (an == 0 || bn == 0) return fa_set_error(FA_E_ARG, "empty operand");
+    if (np < 2 || np > FA_NPRIMES || bits < 8 || bits > 200 || bits % 8)
+        return fa_set_error(FA_E_ARG, "plan (np=%d, bits=%d) outside np in [2,8], bits in "
+                            "8..200 step 8", np, bits);
+    fa_crt_consts C;
+    fa_crt_consts_init(&C, np, 0);
+    if (9 * bits < 64 * C.nl + 64)
+        return fa_set_error(FA_E_ARG, "bits=%d too small for %d primes", bits, np);
+    uint64_t alen = cdiv_u64(64 * an, bits), blen = cdiv_u64(64 * bn, bits);
+    uint64_t mlen = alen < blen ? alen : blen;
+    if (!ml_ge_ui_2exp(C.M, mlen, 2 * (uint64_t)bits))
+        return fa_set_error(FA_E_RANGE, "plan (np=%d, bits=%d) not exact for %llu x %llu limbs",
+                            np, bits, (unsigned long long)an, (unsigned long long)bn);
+    uint64_t zlen = alen + blen - 1;
+    int depth = clog2_u64(zlen);
+    if (depth < 4) depth = 4;
+    if (depth > FA_MAX_DEPTH)
+        return fa_set_error(FA_E_RANGE, "plan needs depth %d > %d", depth, FA_MAX_DEPTH);
+    P->np = np;
+    P->bits = bits;
+    P->depth = depth;
+    P->alen = alen;
+    P->blen = blen;
+    P->zlen = zlen;
+    P->d2 = depth <= FA_ROW_LG ? depth : FA_ROW_LG;
+    P->d1 = depth - P->d2;
     return FA_OK;
 }
 

@@ -1,0 +1,583 @@
+// fa_fast.hip -- register radix-16 kernels for the two-level transforms with
+// a 4096-point row (d2 = 12) and a column of 2^d1 points, d1 = 1..12, i.e.
+// transform lengths 2^13 .. 2^24.
+//
+// A 2^(4s)-point transform is s stages of radix 16: each thread holds 16
+// elements in registers and applies four levels (two radix-4 butterflies,
+// fa_fwd4 / fa_inv4 of fa_ntt.h) per stage; between stages the elements are
+// exchanged through LDS.  At stage j a thread owns, inside one block of
+// level 4j (block b, offset i0), the 16 positions
+//     (b << (D - 4j)) + i0 + k * 2^(D - 4j - 4),   k < 16,
+// so after the last stage thread t holds the 16 consecutive outputs
+// 16t .. 16t+15 (bit-reversed evaluation order).  LDS rows are padded by
+// one double per 16 so that all three access patterns are conflict-free.
+#include "fa_internal.h"
+#include "fa_ntt.h"
+#include "fa_pack.h"
+
+#define PADX(p) ((p) + ((p) >> 4))
+
+// radix-16 = four levels from the block Q at level m0 (fft_small.h block
+// convention): t[0..2] = (w2[Q], w2[2Q], w2[2Q+1]), t[3+3j..5+3j] the same
+// for Q' = 4Q + j
+FA_DEV void fa_fwd16(double* x, const double* t, double n, double ninv)
+{
+#pragma unroll
+    for (int k = 0; k < 4; k++) fa_fwd4(x[k], x[k + 4], x[k + 8], x[k + 12], t[0], t[1], t[2], n, ninv);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        fa_fwd4(x[4 * j], x[4 * j + 1], x[4 * j + 2], x[4 * j + 3], t[3 + 3 * j], t[4 + 3 * j],
+                t[5 + 3 * j], n, ninv);
+}
+
+FA_DEV void fa_inv16(double* x, const double* t, double n, double ninv)
+{
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        fa_inv4(x[4 * j], x[4 * j + 1], x[4 * j + 2], x[4 * j + 3], t[3 + 3 * j], t[4 + 3 * j],
+                t[5 + 3 * j], n, ninv);
+#pragma unroll
+    for (int k = 0; k < 4; k++) fa_inv4(x[k], x[k + 4], x[k + 8], x[k + 12], t[0], t[1], t[2], n, ninv);
+}
+
+// twiddles of block Q from a table indexed by global block number
+FA_DEV void fa_tw16(const double* __restrict__ w, uint64_t Q, double* t)
+{
+    t[0] = w[Q];
+    t[1] = w[2 * Q];
+    t[2] = w[2 * Q + 1];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        uint64_t Qp = 4 * Q + j;
+        t[3 + 3 * j] = w[Qp];
+        t[4 + 3 * j] = w[2 * Qp];
+        t[5 + 3 * j] = w[2 * Qp + 1];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-row twiddle table in LDS: entry (m, q) = w2[(r << m) + q], m < 12.
+// Levels m < 8 are stored at (2^m - 1) + q; levels m >= 8 transposed,
+// (2^m - 1) + (q mod 2^(m-8)) * 256 + (q >> (m-8)), so that the last stage
+// (thread t needs q = t*2^(m-8) + k) reads consecutive addresses.
+// ---------------------------------------------------------------------------
+
+FA_DEV int fa_tab_index(int m, int q)
+{
+    if (m < 8) return (1 << m) - 1 + q;
+    int s = m - 8;
+    return (1 << m) - 1 + ((q & ((1 << s) - 1)) << 8) + (q >> s);
+}
+
+// T(m, q) = rho_m * base[q] with rho_m = w[r << m] (multiplicative over the
+// disjoint bit ranges of (r << m) and q), reduced exactly to [-n/2, n/2]
+FA_DEV void fa_build_row_table(double* tab, const double* __restrict__ w, uint64_t r,
+                               double* rho, double n, double ninv)
+{
+    const int tid = threadIdx.x;
+    if (tid < 12) rho[tid] = w[r << tid];
+    __syncthreads();
+    const double hn = 0.5 * (n - 1.0);
+    for (int e = tid; e < 4095; e += blockDim.x) {
+        int m = 31 - __builtin_clz(e + 1);
+        int u = e + 1 - (1 << m);
+        int q = u;
+        if (m >= 8) {
+            int s = m - 8;
+            q = ((u & 255) << s) + (u >> 8);
+        }
+        double v = fa_mulmod(rho[m], w[q], n, ninv);
+        if (v > hn) v -= n;
+        else if (v < -hn) v += n;
+        tab[e] = v;
+    }
+    __syncthreads();
+}
+
+FA_DEV void fa_row_tw(const double* tab, int j, int bq, double* t)
+{
+    const int m = 4 * j;
+    t[0] = tab[fa_tab_index(m, bq)];
+    t[1] = tab[fa_tab_index(m + 1, 2 * bq)];
+    t[2] = tab[fa_tab_index(m + 1, 2 * bq + 1)];
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+        int Qp = 4 * bq + jj;
+        t[3 + 3 * jj] = tab[fa_tab_index(m + 2, Qp)];
+        t[4 + 3 * jj] = tab[fa_tab_index(m + 3, 2 * Qp)];
+        t[5 + 3 * jj] = tab[fa_tab_index(m + 3, 2 * Qp + 1)];
+    }
+}
+
+// 4096-point forward transform of one row, 256 threads, input in the
+// stage-0 layout (positions t + 256k), output in the stage-2 layout
+// (positions 16t + k)
+FA_DEV void fa_fwd4096_row(double* x, double* xch, const double* tab, double n, double ninv)
+{
+    const int t = threadIdx.x;
+    double tw[15];
+    fa_row_tw(tab, 0, 0, tw);
+    fa_fwd16(x, tw, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 16; k++) xch[PADX(t + 256 * k)] = x[k];
+    __syncthreads();
+    const int b = t >> 4, i0 = t & 15;
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = xch[PADX(256 * b + i0 + 16 * k)];
+    fa_row_tw(tab, 1, b, tw);
+    fa_fwd16(x, tw, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 16; k++) xch[PADX(256 * b + i0 + 16 * k)] = x[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = xch[PADX(16 * t + k)];
+    fa_row_tw(tab, 2, t, tw);
+    fa_fwd16(x, tw, n, ninv);
+    __syncthreads();
+}
+
+FA_DEV void fa_inv4096_row(double* x, double* xch, const double* tab, double n, double ninv)
+{
+    const int t = threadIdx.x;
+    double tw[15];
+    fa_row_tw(tab, 2, t, tw);
+    fa_inv16(x, tw, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 16; k++) xch[PADX(16 * t + k)] = x[k];
+    __syncthreads();
+    const int b = t >> 4, i0 = t & 15;
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = xch[PADX(256 * b + i0 + 16 * k)];
+    fa_row_tw(tab, 1, b, tw);
+    fa_inv16(x, tw, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 16; k++) xch[PADX(256 * b + i0 + 16 * k)] = x[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = xch[PADX(t + 256 * k)];
+    fa_row_tw(tab, 0, 0, tw);
+    fa_inv16(x, tw, n, ninv);
+}
+
+// row pass: both forward row transforms, pointwise product with the
+// normalizer, inverse row transform; in place in A.
+// blockIdx.x = row, blockIdx.y = prime
+__global__ void __launch_bounds__(256) fa_k_row_conv16(double* __restrict__ A,
+                                                       const double* __restrict__ B,
+                                                       uint64_t stride, fa_prime_set S)
+{
+    __shared__ double xch[4352];
+    __shared__ double tab[4096];
+    __shared__ double rho[16];
+    const int i = blockIdx.y;
+    const double n = S.p[i], ninv = S.pinv[i];
+    const uint64_t r = blockIdx.x;
+    const int t = threadIdx.x;
+    double* arow = A + (uint64_t)i * stride + (r << 12);
+    const double* brow = B + (uint64_t)i * stride + (r << 12);
+    double x[16], y[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        y[k] = brow[t + 256 * k];
+        x[k] = arow[t + 256 * k];
+    }
+    fa_build_row_table(tab, S.w2[i], r, rho, n, ninv);
+    fa_fwd4096_row(y, xch, tab, n, ninv);
+    fa_fwd4096_row(x, xch, tab, n, ninv);
+    const double m = S.norm[i];
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = fa_mulmod(fa_mulmod(x[k], m, n, ninv), y[k], n, ninv);
+    fa_build_row_table(tab, S.w2inv[i], r, rho, n, ninv);
+    fa_inv4096_row(x, xch, tab, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 16; k++) arow[t + 256 * k] = x[k];
+}
+
+// ---------------------------------------------------------------------------
+// column passes for any column depth D1 in [1, 12].  With D1 = 4s + r:
+//  - D1 >= 4: 2^(D1-4) threads per column, 16 elements each; a partial stage
+//    of r levels (if r > 0) on the stage-0 layout, then s radix-16 stages
+//    starting at levels r, r+4, ...;
+//  - D1 < 4: one thread per column, all 2^D1 elements in registers.
+// A stage starting at level m0 owns, in block b of level m0, the positions
+//     (b << (D1 - m0)) + i0 + (k << (D1 - m0 - 4)),  tc = b * 2^(D1-m0-4) + i0.
+// ---------------------------------------------------------------------------
+
+template <int D1>
+struct col_geom {
+    static constexpr int R = (D1 >= 4) ? (D1 % 4) : D1;   // levels of the partial stage
+    static constexpr int NS = (D1 >= 4) ? (D1 / 4) : 0;   // radix-16 stages
+    static constexpr int TPC = (D1 >= 4) ? (1 << (D1 - 4)) : 1;
+    static constexpr int CPW = 1024 / TPC;
+    static constexpr int LEN = 1 << D1;
+    static constexpr int PLEN = LEN + LEN / 16;
+    static constexpr int NEL = (D1 >= 4) ? 16 : (1 << D1);  // elements per thread
+    static constexpr int NSTAGE = NS + (R > 0 && D1 >= 4 ? 1 : 0);
+    static constexpr int LDS = (NSTAGE > 1) ? CPW * PLEN : 1;
+
+    // first level of stage j (stage 0 is the partial one when R > 0)
+    static FA_DEV int m0(int j) { return (R > 0) ? (j == 0 ? 0 : R + 4 * (j - 1)) : 4 * j; }
+    static FA_DEV int pos(int j, int tc, int k)
+    {
+        if (D1 < 4) return k;
+        const int sh = D1 - m0(j) - 4;
+        const int b = tc >> sh, i0 = tc & ((1 << sh) - 1);
+        return (b << (D1 - m0(j))) + i0 + (k << sh);
+    }
+    static FA_DEV int block(int j, int tc)
+    {
+        if (D1 < 4) return 0;
+        return tc >> (D1 - m0(j) - 4);
+    }
+};
+
+// the first r (< 4) levels of a radix-16 group, block Q: level 0 pairs
+// (k, k+8), level 1 (k, k+4), level 2 (k, k+2)
+FA_DEV void fa_fwd16_partial(double* x, int r, const double* __restrict__ w, uint64_t Q,
+                             double n, double ninv)
+{
+    if (r == 1) {
+        const double W = w[Q];
+#pragma unroll
+        for (int k = 0; k < 8; k++) fa_fwd2(x[k], x[k + 8], W, n, ninv);
+        return;
+    }
+    const double W2 = w[Q], W = w[2 * Q], IW = w[2 * Q + 1];
+#pragma unroll
+    for (int k = 0; k < 4; k++) fa_fwd4(x[k], x[k + 4], x[k + 8], x[k + 12], W2, W, IW, n, ninv);
+    if (r == 3) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const double Wj = w[4 * Q + j];
+            fa_fwd2(x[4 * j], x[4 * j + 2], Wj, n, ninv);
+            fa_fwd2(x[4 * j + 1], x[4 * j + 3], Wj, n, ninv);
+        }
+    }
+}
+
+FA_DEV void fa_inv16_partial(double* x, int r, const double* __restrict__ wi, uint64_t Q,
+                             double n, double ninv)
+{
+    if (r == 1) {
+        const double V = wi[Q];
+#pragma unroll
+        for (int k = 0; k < 8; k++) fa_inv2(x[k], x[k + 8], V, n, ninv);
+        return;
+    }
+    if (r == 3) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const double Vj = wi[4 * Q + j];
+            fa_inv2(x[4 * j], x[4 * j + 2], Vj, n, ninv);
+            fa_inv2(x[4 * j + 1], x[4 * j + 3], Vj, n, ninv);
+        }
+    }
+    const double V2 = wi[Q], V = wi[2 * Q], IV = wi[2 * Q + 1];
+#pragma unroll
+    for (int k = 0; k < 4; k++) fa_inv4(x[k], x[k + 4], x[k + 8], x[k + 12], V2, V, IV, n, ninv);
+}
+
+// a whole column of 2^D1 <= 8 points in one thread (block 0 at level 0)
+template <int D1>
+FA_DEV void fa_fwd_small_col(double* x, const double* __restrict__ w, double n, double ninv)
+{
+#pragma unroll
+    for (int m = 0; m < D1; m++) {
+        const int half = 1 << (D1 - m - 1);
+#pragma unroll
+        for (int q = 0; q < (1 << m); q++) {
+            const double W = w[q];
+#pragma unroll
+            for (int i = 0; i < half; i++) fa_fwd2(x[2 * half * q + i], x[2 * half * q + i + half], W, n, ninv);
+        }
+    }
+}
+
+template <int D1>
+FA_DEV void fa_inv_small_col(double* x, const double* __restrict__ wi, double n, double ninv)
+{
+#pragma unroll
+    for (int m = D1 - 1; m >= 0; m--) {
+        const int half = 1 << (D1 - m - 1);
+#pragma unroll
+        for (int q = 0; q < (1 << m); q++) {
+            const double V = wi[q];
+#pragma unroll
+            for (int i = 0; i < half; i++) fa_inv2(x[2 * half * q + i], x[2 * half * q + i + half], V, n, ninv);
+        }
+    }
+}
+
+// linear work index with consecutive indices on the same XCD (workgroups
+// are dealt round-robin to the 8 XCDs); grids are multiples of 8
+FA_DEV unsigned fa_xcd_linear()
+{
+    unsigned nb = gridDim.x, bid = blockIdx.x;
+    return (bid & 7) * (nb >> 3) + (bid >> 3);
+}
+
+template <int D1>
+__global__ void __launch_bounds__(1024) fa_k_col_fwd16(fa_operand a, fa_operand b, int d2,
+                                                       double* __restrict__ A,
+                                                       double* __restrict__ B, uint64_t stride,
+                                                       fa_prime_set S, int np, int ngroups,
+                                                       int nops)
+{
+    typedef col_geom<D1> G;
+    __shared__ double lds[G::LDS];
+    const unsigned L = fa_xcd_linear();
+    const int i = L % np;
+    const unsigned rest = L / np;
+    const int g = rest % ngroups;
+    const int z = rest / ngroups;
+    if (z >= nops) return;
+    const double n = S.p[i], ninv = S.pinv[i];
+    const fa_operand& op = z ? b : a;
+    double* dst = (z ? B : A) + (uint64_t)i * stride;
+    const int c = threadIdx.x % G::CPW, tc = threadIdx.x / G::CPW;
+    const uint64_t col = (uint64_t)g * G::CPW + c;
+    double* my = lds + (G::NSTAGE > 1 ? c * G::PLEN : 0);
+    const double* w2 = S.w2[i];
+    double x[16], tw[15];
+#pragma unroll
+    for (int k = 0; k < G::NEL; k++)
+        x[k] = fa_operand_value(op, ((uint64_t)G::pos(0, tc, k) << d2) + col, S.pow2[i], n, ninv);
+    if (D1 < 4) {
+        fa_fwd_small_col<(D1 < 4 ? D1 : 1)>(x, w2, n, ninv);
+    } else {
+#pragma unroll
+        for (int j = 0; j < G::NSTAGE; j++) {
+            if (j > 0) {
+                // each position of a stage layout belongs to one thread: a
+                // single barrier separates stage j-1's stores from stage j's loads
+#pragma unroll
+                for (int k = 0; k < 16; k++) my[PADX(G::pos(j - 1, tc, k))] = x[k];
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < 16; k++) x[k] = my[PADX(G::pos(j, tc, k))];
+            }
+            if (G::R > 0 && j == 0) {
+                fa_fwd16_partial(x, G::R, w2, (uint64_t)G::block(0, tc), n, ninv);
+            } else {
+                fa_tw16(w2, (uint64_t)G::block(j, tc), tw);
+                fa_fwd16(x, tw, n, ninv);
+            }
+        }
+    }
+    const int jl = (D1 < 4) ? 0 : G::NSTAGE - 1;
+#pragma unroll
+    for (int k = 0; k < G::NEL; k++) dst[((uint64_t)G::pos(jl, tc, k) << d2) + col] = x[k];
+}
+
+template <int D1>
+__global__ void __launch_bounds__(1024) fa_k_col_inv16(double* __restrict__ A, int d2,
+                                                       uint64_t stride, fa_prime_set S, int np,
+                                                       int ngroups)
+{
+    typedef col_geom<D1> G;
+    __shared__ double lds[G::LDS];
+    const unsigned L = fa_xcd_linear();
+    const int i = L % np;
+    const int g = L / np;
+    if (g >= ngroups) return;
+    const double n = S.p[i], ninv = S.pinv[i];
+    double* dst = A + (uint64_t)i * stride;
+    const int c = threadIdx.x % G::CPW, tc = threadIdx.x / G::CPW;
+    const uint64_t col = (uint64_t)g * G::CPW + c;
+    double* my = lds + (G::NSTAGE > 1 ? c * G::PLEN : 0);
+    const double* wi = S.w2inv[i];
+    double x[16], tw[15];
+    const int jl = (D1 < 4) ? 0 : G::NSTAGE - 1;
+#pragma unroll
+    for (int k = 0; k < G::NEL; k++) x[k] = dst[((uint64_t)G::pos(jl, tc, k) << d2) + col];
+    if (D1 < 4) {
+        fa_inv_small_col<(D1 < 4 ? D1 : 1)>(x, wi, n, ninv);
+    } else {
+#pragma unroll
+        for (int j = G::NSTAGE - 1; j >= 0; j--) {
+            if (j < G::NSTAGE - 1) {
+#pragma unroll
+                for (int k = 0; k < 16; k++) my[PADX(G::pos(j + 1, tc, k))] = x[k];
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < 16; k++) x[k] = my[PADX(G::pos(j, tc, k))];
+            }
+            if (G::R > 0 && j == 0) {
+                fa_inv16_partial(x, G::R, wi, (uint64_t)G::block(0, tc), n, ninv);
+            } else {
+                fa_tw16(wi, (uint64_t)G::block(j, tc), tw);
+                fa_inv16(x, tw, n, ninv);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < G::NEL; k++) dst[((uint64_t)G::pos(0, tc, k) << d2) + col] = x[k];
+}
+
+// ---------------------------------------------------------------------------
+// 8-column variant for D1 = 12: each thread carries columns c and c + 4 of
+// an 8-column tile (rows stage-layout as above), so a wave's pair of loads
+// covers 64 contiguous bytes of every row it touches; the LDS exchange of
+// the 4096-row columns runs in two halves of 4 columns.
+// ---------------------------------------------------------------------------
+
+FA_DEV void fa_exchange2(double* lds, double* x0, double* x1, int c, int tc, int jfrom, int jto)
+{
+    typedef col_geom<12> G;
+    double* my = lds + c * G::PLEN;
+#pragma unroll
+    for (int k = 0; k < 16; k++) my[PADX(G::pos(jfrom, tc, k))] = x0[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) x0[k] = my[PADX(G::pos(jto, tc, k))];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) my[PADX(G::pos(jfrom, tc, k))] = x1[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) x1[k] = my[PADX(G::pos(jto, tc, k))];
+}
+
+__global__ void __launch_bounds__(1024) fa_k_col_fwd16w(fa_operand a, fa_operand b, int d2,
+                                                        double* __restrict__ A,
+                                                        double* __restrict__ B, uint64_t stride,
+                                                        fa_prime_set S, int np, int ngroups,
+                                                        int nops)
+{
+    typedef col_geom<12> G;
+    __shared__ double lds[4 * G::PLEN];
+    const unsigned L = fa_xcd_linear();
+    const int i = L % np;
+    const unsigned rest = L / np;
+    const int g = rest % ngroups;
+    const int z = rest / ngroups;
+    if (z >= nops) return;
+    const double n = S.p[i], ninv = S.pinv[i];
+    const fa_operand& op = z ? b : a;
+    double* dst = (z ? B : A) + (uint64_t)i * stride;
+    const int c = threadIdx.x & 3, tc = threadIdx.x >> 2;
+    const uint64_t col0 = (uint64_t)g * 8 + c, col1 = col0 + 4;
+    const double* w2 = S.w2[i];
+    double x0[16], x1[16], tw[15];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint64_t r = (uint64_t)G::pos(0, tc, k) << d2;
+        x0[k] = fa_operand_value(op, r + col0, S.pow2[i], n, ninv);
+        x1[k] = fa_operand_value(op, r + col1, S.pow2[i], n, ninv);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        if (j > 0) {
+            fa_exchange2(lds, x0, x1, c, tc, j - 1, j);
+        }
+        fa_tw16(w2, (uint64_t)G::block(j, tc), tw);
+        fa_fwd16(x0, tw, n, ninv);
+        fa_fwd16(x1, tw, n, ninv);
+        if (j < 2) __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint64_t r = (uint64_t)G::pos(2, tc, k) << d2;
+        dst[r + col0] = x0[k];
+        dst[r + col1] = x1[k];
+    }
+}
+
+__global__ void __launch_bounds__(1024) fa_k_col_inv16w(double* __restrict__ A, int d2,
+                                                        uint64_t stride, fa_prime_set S, int np,
+                                                        int ngroups)
+{
+    typedef col_geom<12> G;
+    __shared__ double lds[4 * G::PLEN];
+    const unsigned L = fa_xcd_linear();
+    const int i = L % np;
+    const int g = L / np;
+    if (g >= ngroups) return;
+    const double n = S.p[i], ninv = S.pinv[i];
+    double* dst = A + (uint64_t)i * stride;
+    const int c = threadIdx.x & 3, tc = threadIdx.x >> 2;
+    const uint64_t col0 = (uint64_t)g * 8 + c, col1 = col0 + 4;
+    const double* wi = S.w2inv[i];
+    double x0[16], x1[16], tw[15];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint64_t r = (uint64_t)G::pos(2, tc, k) << d2;
+        x0[k] = dst[r + col0];
+        x1[k] = dst[r + col1];
+    }
+#pragma unroll
+    for (int j = 2; j >= 0; j--) {
+        if (j < 2) {
+            fa_exchange2(lds, x0, x1, c, tc, j + 1, j);
+        }
+        fa_tw16(wi, (uint64_t)G::block(j, tc), tw);
+        fa_inv16(x0, tw, n, ninv);
+        fa_inv16(x1, tw, n, ninv);
+        if (j > 0) __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint64_t r = (uint64_t)G::pos(0, tc, k) << d2;
+        dst[r + col0] = x0[k];
+        dst[r + col1] = x1[k];
+    }
+}
+
+int fa_col_variant = 1;   // 1: 8-column tiles at d1 = 12, 0: 4-column tiles
+extern "C" FA_API void fa_set_col_variant(int v) { fa_col_variant = v; }
+
+// ---------------------------------------------------------------------------
+// host launcher: transform lengths 2^13 .. 2^24 (row 2^12, column 2^1 ..
+// 2^12).  Returns 1 if it handled the depth, 0 otherwise.
+// ---------------------------------------------------------------------------
+
+extern int fa_col_variant;
+
+template <int D1>
+static int launch_fast(const fa_operand& oa, const fa_operand& ob, int nops, int np, double* A,
+                       double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s)
+{
+    typedef col_geom<D1> G;
+    const int d2 = FA_ROW_LG;
+    const bool wide = (D1 == 12) && fa_col_variant == 1;
+    const int ngroups = (1 << d2) / (wide ? 8 : G::CPW);
+    unsigned total = (unsigned)(np * ngroups * nops);
+    unsigned grid = (total + 7) & ~7u;
+    if (wide)
+        FA_LAUNCH("fa_k_col_fwd16w", fa_k_col_fwd16w, dim3(grid), dim3(1024), 0, s, oa, ob, d2, A, B,
+                  stride, S, np, ngroups, nops);
+    else
+        FA_LAUNCH("fa_k_col_fwd16", fa_k_col_fwd16<D1>, dim3(grid), dim3(1024), 0, s, oa, ob, d2, A,
+                  B, stride, S, np, ngroups, nops);
+    FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16, dim3(1u << D1, np), dim3(256), 0, s, A, B, stride,
+              S);
+    unsigned total2 = (unsigned)(np * ngroups);
+    unsigned grid2 = (total2 + 7) & ~7u;
+    if (wide)
+        FA_LAUNCH("fa_k_col_inv16w", fa_k_col_inv16w, dim3(grid2), dim3(1024), 0, s, A, d2, stride,
+                  S, np, ngroups);
+    else
+        FA_LAUNCH("fa_k_col_inv16", fa_k_col_inv16<D1>, dim3(grid2), dim3(1024), 0, s, A, d2, stride,
+                  S, np, ngroups);
+    return 1;
+}
+
+int fa_convolve_fast(int depth, const fa_operand& oa, const fa_operand& ob, int np, double* A,
+                     double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s)
+{
+    switch (depth - FA_ROW_LG) {
+    case 1: return launch_fast<1>(oa, ob, 2, np, A, B, stride, S, s);
+    case 2: return launch_fast<2>(oa, ob, 2, np, A, B, stride, S, s);
+    case 3: return launch_fast<3>(oa, ob, 2, np, A, B, stride, S, s);
+    case 4: return launch_fast<4>(oa, ob, 2, np, A, B, stride, S, s);
+    case 5: return launch_fast<5>(oa, ob, 2, np, A, B, stride, S, s);
+    case 6: return launch_fast<6>(oa, ob, 2, np, A, B, stride, S, s);
+    case 7: return launch_fast<7>(oa, ob, 2, np, A, B, stride, S, s);
+    case 8: return launch_fast<8>(oa, ob, 2, np, A, B, stride, S, s);
+    case 9: return launch_fast<9>(oa, ob, 2, np, A, B, stride, S, s);
+    case 10: return launch_fast<10>(oa, ob, 2, np, A, B, stride, S, s);
+    case 11: return launch_fast<11>(oa, ob, 2, np, A, B, stride, S, s);
+    case 12: return launch_fast<12>(oa, ob, 2, np, A, B, stride, S, s);
+    default: return 0;
+    }
+}

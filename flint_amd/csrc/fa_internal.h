@@ -19,7 +19,7 @@ struct fa_prime_host {
     double p, pinv;
     uint64_t g;              // primitive 2^v-th root (sd_fft_ctx.c:32-37)
     int v;                   // 2-valuation of n-1
-    double two32pow[8];      // 2^(32k) mod n in [-n/2, n/2]
+    double pow2[8];          // 2^32, 2^48, 2^96, 2^144, 2^192 mod n in [-n/2, n/2]
 };
 
 // device-wide tables (one set per device, shared by all host threads)
@@ -47,7 +47,7 @@ struct fa_prime_set {
     double p[FA_NPRIMES];
     double pinv[FA_NPRIMES];
     double norm[FA_NPRIMES];       // pointwise normalizer (cop*2^D)^-1
-    double two32pow[FA_NPRIMES][8];
+    double pow2[FA_NPRIMES][8];
     const double* w2[FA_NPRIMES];
     const double* w2inv[FA_NPRIMES];
 };
@@ -84,6 +84,7 @@ int fa_hbuf_reserve(fa_thread_state* st, size_t bytes, void** out);
 void fa_crt_consts_init(fa_crt_consts* C, int np, int offset);
 uint64_t fa_crt_cop(int np, int i);         // (M/p_i) mod p_i
 int fa_mpn_choose_plan(uint64_t an, uint64_t bn, fa_mpn_plan_t* P);
+int fa_mpn_fixed_plan(uint64_t an, uint64_t bn, int np, int bits, fa_mpn_plan_t* P);
 void fa_prime_set_init(fa_prime_set* S, const fa_tables* T, int np, int depth, int offset,
                        int with_norm);
 uint64_t fa_mulmod_u64(uint64_t a, uint64_t b, uint64_t n);

@@ -16,92 +16,7 @@
 #include "fa_internal.h"
 #include "fa_ntt.h"
 
-// ------------------------------------------------------------------------
-// limb splitting (mpn_mul.c:595-695, slow_mpn_to_fft :422-481)
-// ------------------------------------------------------------------------
-
-// 32-bit word of the little-endian limb array starting at byte offset
-// `byte` (zero beyond the array)
-FA_DEV uint32_t fa_load_u32_at(const uint32_t* __restrict__ w, uint64_t nwords, uint64_t byte)
-{
-    uint64_t i = byte >> 2;
-    int sh = (int)(byte & 3) * 8;
-    uint32_t lo = (i < nwords) ? w[i] : 0u;
-    if (sh == 0) return lo;
-    uint32_t hi = (i + 1 < nwords) ? w[i + 1] : 0u;
-    return (lo >> sh) | (hi << (32 - sh));
-}
-
-// residue mod p of chunk j = bits [j*bits, (j+1)*bits) of the limb array,
-// bits a multiple of 8; result in [-p/2, p/2]
-FA_DEV double fa_chunk_residue(const uint64_t* __restrict__ limbs, uint64_t nlimbs, int bits,
-                               uint64_t j, const double* __restrict__ two32pow, double n,
-                               double ninv)
-{
-    const uint32_t* w = (const uint32_t*)limbs;
-    uint64_t nwords = 2 * nlimbs;
-    uint64_t start = j * (uint64_t)(bits >> 3);
-    if (start >= 8 * nlimbs) return 0.0;
-    int nbytes = bits >> 3;
-    if (bits <= 48) {
-        uint32_t lo = fa_load_u32_at(w, nwords, start);
-        uint32_t hi = (nbytes > 4) ? fa_load_u32_at(w, nwords, start + 4) : 0u;
-        if (nbytes < 4) lo &= (1u << (8 * nbytes)) - 1u;
-        if (nbytes > 4 && nbytes < 8) hi &= (1u << (8 * (nbytes - 4))) - 1u;
-        return (double)lo + 4294967296.0 * (double)hi;
-    }
-    double acc = 0.0;
-    for (int k = 0; 4 * k < nbytes; k++) {
-        uint32_t v = fa_load_u32_at(w, nwords, start + 4 * (uint64_t)k);
-        int rem = nbytes - 4 * k;
-        if (rem < 4) v &= (1u << (8 * rem)) - 1u;
-        acc += (k == 0) ? (double)v : fa_mulmod((double)v, two32pow[k], n, ninv);
-    }
-    return fa_reduce_pm1n(acc, n, ninv);
-}
-
-// ------------------------------------------------------------------------
-// operand import: the value of element x (coefficient index) of an operand
-// ------------------------------------------------------------------------
-
-struct fa_operand {
-    int kind;                  // FA_PACK_*
-    const uint64_t* src;
-    uint64_t n;                // limbs (mpn) or coefficients
-    int bits;
-    int width;
-};
-
-// signed w-limb two's complement coefficient mod p: Horner over the 32-bit
-// words from the top, minus 2^(64 w) for a negative coefficient
-FA_DEV double fa_coeff_residue_s(const uint64_t* __restrict__ c, int w,
-                                 const double* __restrict__ two32pow, double n, double ninv)
-{
-    const uint32_t* u = (const uint32_t*)c;
-    const double c32 = two32pow[1];
-    double r = 0.0, t = 1.0;
-    for (int k = 2 * w - 1; k >= 0; k--) {
-        r = fa_reduce_pm1n(fa_mulmod(r, c32, n, ninv) + (double)u[k], n, ninv);
-        t = fa_reduce_pm1n(fa_mulmod(t, c32, n, ninv), n, ninv);
-    }
-    if ((int64_t)c[w - 1] < 0) r = fa_reduce_pm1n(r - t, n, ninv);
-    return r;
-}
-
-FA_DEV double fa_operand_value(const fa_operand& op, uint64_t x,
-                               const double* __restrict__ two32pow, double n, double ninv)
-{
-    if (op.kind == FA_PACK_MPN)
-        return fa_chunk_residue(op.src, op.n, op.bits, x, two32pow, n, ninv);
-    if (x >= op.n) return 0.0;
-    if (op.kind == FA_PACK_U64) {
-        uint64_t v = op.src[x];
-        double lo = (double)(uint32_t)v, hi = (double)(uint32_t)(v >> 32);
-        if (v < (UINT64_C(1) << 50)) return (double)v;
-        return fa_reduce_pm1n(lo + fa_mulmod(hi, two32pow[1], n, ninv), n, ninv);
-    }
-    return fa_coeff_residue_s(op.src + x * (uint64_t)op.width, op.width, two32pow, n, ninv);
-}
+#include "fa_pack.h"
 
 // ------------------------------------------------------------------------
 // transform kernels
@@ -120,8 +35,8 @@ __global__ void __launch_bounds__(256) fa_k_small_conv(fa_operand a, fa_operand 
     double* sa = lds;
     double* sb = lds + N;
     for (int x = threadIdx.x; x < N; x += blockDim.x) {
-        sa[x] = fa_operand_value(a, (uint64_t)x, S.two32pow[i], n, ninv);
-        sb[x] = fa_operand_value(b, (uint64_t)x, S.two32pow[i], n, ninv);
+        sa[x] = fa_operand_value(a, (uint64_t)x, S.pow2[i], n, ninv);
+        sb[x] = fa_operand_value(b, (uint64_t)x, S.pow2[i], n, ninv);
     }
     __syncthreads();
     fa_lds_fwd(sa, depth, 0, 0, S.w2[i], n, ninv);
@@ -153,7 +68,7 @@ __global__ void __launch_bounds__(256) fa_k_col_fwd(fa_operand a, fa_operand b, 
     for (int u = threadIdx.x; u < total; u += blockDim.x) {
         uint64_t row = (uint64_t)(u >> lgC), c = (uint64_t)(u & (C - 1));
         uint64_t x = (row << d2) + col0 + c;
-        lds[u] = fa_operand_value(op, x, S.two32pow[i], n, ninv);
+        lds[u] = fa_operand_value(op, x, S.pow2[i], n, ninv);
     }
     __syncthreads();
     fa_lds_fwd(lds, d1, lgC, 0, S.w2[i], n, ninv);
@@ -408,40 +323,41 @@ __global__ void __launch_bounds__(FA_CRT_WG) fa_k_crt_mpn(const double* __restri
     }
 }
 
-// exclusive scan of workgroup (generate, propagate) pairs -> carry-in
-__global__ void __launch_bounds__(256) fa_k_carry_scan(const uint8_t* __restrict__ agg,
-                                                       uint8_t* __restrict__ cin, uint64_t nblk)
+// exclusive scan of workgroup (generate, propagate) pairs -> carry-in.
+// One workgroup of 1024 threads: each thread folds a contiguous chunk of
+// aggregates, the 1024 chunk aggregates are scanned in LDS, then each thread
+// walks its chunk again writing the carry-ins.
+__global__ void __launch_bounds__(1024) fa_k_carry_scan(const uint8_t* __restrict__ agg,
+                                                        uint8_t* __restrict__ cin, uint64_t nblk)
 {
-    __shared__ uint8_t sg[256], sp[256];
-    __shared__ uint8_t running;
-    if (threadIdx.x == 0) running = 0;
+    __shared__ uint8_t sg[1024], sp[1024];
+    const int t = threadIdx.x;
+    const uint64_t chunk = (nblk + 1023) / 1024;
+    const uint64_t b0 = (uint64_t)t * chunk;
+    const uint64_t b1 = (b0 + chunk < nblk) ? b0 + chunk : nblk;
+    uint8_t g = 0, p = 1;
+    for (uint64_t b = b0; b < b1; b++) {
+        uint8_t a = agg[b];
+        g = (uint8_t)((a & 1) | (((a >> 1) & 1) & g));
+        p = (uint8_t)(p & ((a >> 1) & 1));
+    }
+    sg[t] = g;
+    sp[t] = p;
     __syncthreads();
-    for (uint64_t base = 0; base < nblk; base += 256) {
-        uint64_t b = base + threadIdx.x;
-        uint8_t a = (b < nblk) ? agg[b] : 2;
-        sg[threadIdx.x] = a & 1;
-        sp[threadIdx.x] = (a >> 1) & 1;
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint8_t g1 = 0, p1 = 1;
+        if (t >= off) { g1 = sg[t - off]; p1 = sp[t - off]; }
         __syncthreads();
-        // inclusive Hillis-Steele scan of the operator
-        // (g1,p1) then (g2,p2) -> (g2 | p2 & g1, p1 & p2)
-        for (int off = 1; off < 256; off <<= 1) {
-            uint8_t g1 = 0, p1 = 1;
-            if ((int)threadIdx.x >= off) { g1 = sg[threadIdx.x - off]; p1 = sp[threadIdx.x - off]; }
-            __syncthreads();
-            uint8_t g2 = sg[threadIdx.x], p2 = sp[threadIdx.x];
-            sg[threadIdx.x] = g2 | (p2 & g1);
-            sp[threadIdx.x] = p1 & p2;
-            __syncthreads();
-        }
-        uint8_t run = running;
-        // exclusive value for b: inclusive of b-1 applied to running
-        uint8_t ex;
-        if (threadIdx.x == 0) ex = run;
-        else ex = sg[threadIdx.x - 1] | (sp[threadIdx.x - 1] & run);
-        if (b < nblk) cin[b] = ex;
+        uint8_t g2 = sg[t], p2 = sp[t];
+        sg[t] = g2 | (p2 & g1);
+        sp[t] = p1 & p2;
         __syncthreads();
-        if (threadIdx.x == 255) running = sg[255] | (sp[255] & run);
-        __syncthreads();
+    }
+    uint8_t c = t ? sg[t - 1] : 0;
+    for (uint64_t b = b0; b < b1; b++) {
+        cin[b] = c;
+        uint8_t a = agg[b];
+        c = (uint8_t)((a & 1) | (((a >> 1) & 1) & c));
     }
 }
 
@@ -483,6 +399,364 @@ __global__ void __launch_bounds__(FA_CRT_WG) fa_k_carry_apply(const uint8_t* __r
 }
 
 // ------------------------------------------------------------------------
+// coalesced CRT: one thread per coefficient computes its integer value into
+// LDS, then one thread per group of 8 coefficients stitches the shifted
+// values into the group's bits/8 limbs, the overhang of each group is added
+// into the next one, and the workgroup writes its limbs contiguously.
+// A workgroup covers 32 groups; its first group (the last one of the
+// previous workgroup) is recomputed only for its overhang.
+// ------------------------------------------------------------------------
+
+template <int NP>
+struct fa_crt_limbs {
+    // limbs of sum_i x_i (M/p_i) < NP * M < 2^(50 NP + 3)
+    static constexpr int N = (50 * NP + 3 + 63) / 64;
+};
+
+template <int NP>
+FA_DEV void fa_crt_value(const double* __restrict__ Q, uint64_t stride, uint64_t j,
+                         const fa_crt_consts& C, uint64_t* s)
+{
+    constexpr int NL = fa_crt_limbs<NP>::N;
+#pragma unroll
+    for (int l = 0; l < NL; l++) s[l] = 0;
+    double qe = 0.0;
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+        uint64_t x = fa_reduce_0n_u64(Q[(uint64_t)i * stride + j], C.p[i], C.pinv[i]);
+        qe += (double)x * C.rp[i];
+        uint64_t carry = 0;
+#pragma unroll
+        for (int l = 0; l < NL; l++) {
+            uint64_t lo = x * C.cof[i][l];
+            uint64_t hi = __umul64hi(x, C.cof[i][l]);
+            uint64_t t = s[l] + lo;
+            hi += (t < lo);
+            t += carry;
+            hi += (t < carry);
+            s[l] = t;
+            carry = hi;
+        }
+    }
+    uint64_t q = (uint64_t)qe;
+    uint64_t borrow = 0, carry = 0;
+#pragma unroll
+    for (int l = 0; l < NL; l++) {
+        uint64_t lo = q * C.M[l];
+        uint64_t hi = __umul64hi(q, C.M[l]);
+        lo += carry;
+        hi += (lo < carry);
+        carry = hi;
+        uint64_t t = s[l] - lo;
+        uint64_t b1 = (s[l] < lo);
+        uint64_t t2 = t - borrow;
+        uint64_t b2 = (t < borrow);
+        s[l] = t2;
+        borrow = b1 | b2;
+    }
+    if (borrow) {
+        uint64_t c2 = 0;
+#pragma unroll
+        for (int l = 0; l < NL; l++) {
+            uint64_t t = s[l] + C.M[l];
+            uint64_t cc = (t < s[l]);
+            t += c2;
+            cc |= (t < c2);
+            s[l] = t;
+            c2 = cc;
+        }
+    } else {
+        int ge = 1;
+#pragma unroll
+        for (int l = NL - 1; l >= 0; l--)
+            if (ge == 1 && s[l] != C.M[l]) ge = (s[l] > C.M[l]) ? 2 : 0;
+        if (ge) {
+            uint64_t br = 0;
+#pragma unroll
+            for (int l = 0; l < NL; l++) {
+                uint64_t t = s[l] - C.M[l];
+                uint64_t b1 = (s[l] < C.M[l]);
+                uint64_t t2 = t - br;
+                uint64_t b2 = (t < br);
+                s[l] = t2;
+                br = b1 | b2;
+            }
+        }
+    }
+}
+
+#define FA_CRT16_GROUPS 32   // groups per workgroup, the first one a halo
+#define FA_OUT_STRIDE 25     // LDS row stride (doubles) of a group's limbs
+
+template <int NP>
+__global__ void __launch_bounds__(256) fa_k_crt_mpn16(const double* __restrict__ Q,
+                                                      uint64_t stride, fa_crt_consts C,
+                                                      uint64_t zlen, int bits,
+                                                      uint64_t* __restrict__ z, uint64_t zn,
+                                                      uint64_t ngroups,
+                                                      uint8_t* __restrict__ flags,
+                                                      uint8_t* __restrict__ agg)
+{
+    constexpr int NL = fa_crt_limbs<NP>::N;
+    __shared__ uint64_t sS[NL][264];
+    __shared__ uint64_t sOut[FA_CRT16_GROUPS * FA_OUT_STRIDE];
+    __shared__ uint64_t sOvh[FA_CRT16_GROUPS][FA_WIN];
+    __shared__ uint8_t sF[FA_CRT16_GROUPS];
+    const int t = threadIdx.x;
+    const int own = bits >> 3;
+    const int64_t g0 = (int64_t)blockIdx.x * (FA_CRT16_GROUPS - 1) - 1;
+
+    {   // coefficient c = t of the workgroup, stored at (c mod 8)*33 + c/8
+        int64_t j = g0 * 8 + t;
+        uint64_t s[NL];
+#pragma unroll
+        for (int l = 0; l < NL; l++) s[l] = 0;
+        if (j >= 0 && (uint64_t)j < zlen) fa_crt_value<NP>(Q, stride, (uint64_t)j, C, s);
+        const int pos = (t & 7) * 33 + (t >> 3);
+#pragma unroll
+        for (int l = 0; l < NL; l++) sS[l][pos] = s[l];
+    }
+    __syncthreads();
+
+    if (t < FA_CRT16_GROUPS) {
+        uint64_t win[FA_WIN];
+#pragma unroll
+        for (int l = 0; l < FA_WIN; l++) win[l] = 0;
+        int emitted = 0;
+        uint64_t prev_limb = 0;
+        uint64_t* out = sOut + t * FA_OUT_STRIDE;
+        for (int k = 0; k < 8; k++) {
+            uint64_t bitoff = (uint64_t)k * bits;
+            int sh = (int)(bitoff & 63);
+            uint64_t cur = bitoff >> 6;
+            while (prev_limb < cur) {
+                out[emitted++] = win[0];
+#pragma unroll
+                for (int l = 0; l < FA_WIN - 1; l++) win[l] = win[l + 1];
+                win[FA_WIN - 1] = 0;
+                prev_limb++;
+            }
+            const int pos = k * 33 + t;
+            uint64_t carry = 0;
+#pragma unroll
+            for (int l = 0; l < FA_WIN; l++) {
+                uint64_t cu = (l < NL) ? sS[l < NL ? l : 0][pos] : 0;
+                uint64_t pv = (l >= 1 && l - 1 < NL) ? sS[(l >= 1 && l - 1 < NL) ? l - 1 : 0][pos] : 0;
+                uint64_t v = sh ? ((cu << sh) | (pv >> (64 - sh))) : cu;
+                uint64_t tt = win[l] + v;
+                uint64_t c1 = (tt < v);
+                tt += carry;
+                c1 |= (tt < carry);
+                win[l] = tt;
+                carry = c1;
+            }
+        }
+        while (prev_limb < (uint64_t)own) {
+            out[emitted++] = win[0];
+#pragma unroll
+            for (int l = 0; l < FA_WIN - 1; l++) win[l] = win[l + 1];
+            win[FA_WIN - 1] = 0;
+            prev_limb++;
+        }
+#pragma unroll
+        for (int l = 0; l < FA_WIN; l++) sOvh[t][l] = win[l];
+    }
+    __syncthreads();
+
+    if (t >= 1 && t < FA_CRT16_GROUPS) {
+        const int64_t g = g0 + t;
+        uint8_t f = 2;   // identity of the carry operator: propagate, no generate
+        if (g >= 0 && (uint64_t)g < ngroups) {
+            uint64_t* out = sOut + t * FA_OUT_STRIDE;
+            uint64_t carry = 0;
+            int prop = 1;
+            for (int l = 0; l < own; l++) {
+                if ((uint64_t)g * own + l >= zn) { prop = 0; break; }
+                uint64_t add = (l < FA_WIN) ? sOvh[t - 1][l < FA_WIN ? l : 0] : 0;
+                uint64_t v = out[l];
+                uint64_t tt = v + add;
+                uint64_t c1 = (tt < add);
+                tt += carry;
+                c1 |= (tt < carry);
+                carry = c1;
+                out[l] = tt;
+                if (tt != ~UINT64_C(0)) prop = 0;
+            }
+            f = (uint8_t)(carry | (prop << 1));
+            flags[g] = f;
+        }
+        sF[t] = f;
+    }
+    __syncthreads();
+    if (t == 0) {
+        // bit 0: carry out with no carry in, bit 1: every group propagates,
+        // bit 2: some group generates a carry
+        uint8_t gg = 0, pp = 1, any = 0;
+        for (int k = 1; k < FA_CRT16_GROUPS; k++) {
+            uint8_t ft = sF[k];
+            gg = (uint8_t)((ft & 1) | (((ft >> 1) & 1) & gg));
+            pp = (uint8_t)(pp & ((ft >> 1) & 1));
+            any = (uint8_t)(any | (ft & 1));
+        }
+        agg[blockIdx.x] = (uint8_t)(gg | (pp << 1) | (any << 2));
+    }
+    // contiguous write of the limbs of groups 1..31
+    const uint64_t base = (uint64_t)(g0 + 1) * own;
+    const int nl = (FA_CRT16_GROUPS - 1) * own;
+    for (int e = t; e < nl; e += 256) {
+        uint64_t za = base + e;
+        if (za < zn) z[za] = sOut[(1 + e / own) * FA_OUT_STRIDE + e % own];
+    }
+}
+
+// carry-in of every group of a 31-group workgroup: scan of the group flags
+// seeded with the workgroup carry-in
+__global__ void __launch_bounds__(64) fa_k_carry_apply16(const uint8_t* __restrict__ flags,
+                                                         const uint8_t* __restrict__ cin,
+                                                         uint64_t* __restrict__ z, uint64_t zn,
+                                                         int own, uint64_t ngroups)
+{
+    __shared__ uint8_t sg[FA_CRT16_GROUPS], sp[FA_CRT16_GROUPS];
+    const int t = threadIdx.x;
+    const int64_t g = (int64_t)blockIdx.x * (FA_CRT16_GROUPS - 1) + t - 1;
+    const bool active = t >= 1 && t < FA_CRT16_GROUPS && g >= 0 && (uint64_t)g < ngroups;
+    if (t < FA_CRT16_GROUPS) {
+        uint8_t f = active ? flags[g] : 2;
+        if (t == 0) f = (uint8_t)(cin[blockIdx.x] & 1);
+        sg[t] = f & 1;
+        sp[t] = (f >> 1) & 1;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    // serial inclusive scan over 32 entries, then apply
+    uint8_t gg = sg[0];
+    for (int k = 1; k < FA_CRT16_GROUPS; k++) {
+        int64_t gk = (int64_t)blockIdx.x * (FA_CRT16_GROUPS - 1) + k - 1;
+        if (gk < 0 || (uint64_t)gk >= ngroups) break;
+        if (gg) {
+            uint64_t zb = (uint64_t)gk * own;
+            for (int l = 0; l < own && zb + l < zn; l++) {
+                uint64_t v = z[zb + l] + 1;
+                z[zb + l] = v;
+                if (v != 0) break;
+            }
+        }
+        gg = sg[k] | (sp[k] & gg);
+    }
+}
+
+// carry-in of every group from the per-group flags and the workgroup
+// aggregates, in one workgroup of 1024 threads.  Each thread owns a
+// contiguous range of CRT workgroups (a multiple of 8, read as 8-byte
+// words), folds their (generate, propagate) aggregates, the range folds are
+// scanned in LDS, and each thread then walks its range: a workgroup that
+// receives no carry and generates none inside (bit 2 of its aggregate) is
+// skipped, any other is resolved group by group from the group flags.
+__global__ void __launch_bounds__(1024) fa_k_carry_fused(const uint8_t* __restrict__ flags,
+                                                         const uint8_t* __restrict__ agg,
+                                                         uint64_t* __restrict__ z, uint64_t zn,
+                                                         int own, uint64_t ngroups, uint64_t nblk)
+{
+    __shared__ uint8_t sg[1024], sp[1024];
+    const int t = threadIdx.x;
+    const uint64_t chunk = ((nblk + 1023) / 1024 + 7) & ~UINT64_C(7);
+    const uint64_t b0 = (uint64_t)t * chunk;
+    const uint64_t b1 = (b0 + chunk < nblk) ? b0 + chunk : (b0 < nblk ? nblk : b0);
+    uint8_t g = 0, p = 1;
+    for (uint64_t b = b0; b < b1; b += 8) {
+        uint64_t w;
+        if (b + 8 <= b1) {
+            w = *(const uint64_t*)(agg + b);
+        } else {
+            w = 0;
+            for (uint64_t q = b; q < b1; q++) w |= (uint64_t)agg[q] << (8 * (q - b));
+            for (uint64_t q = b1; q < b + 8; q++) w |= (uint64_t)2 << (8 * (q - b));
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            uint8_t a = (uint8_t)(w >> (8 * q));
+            g = (uint8_t)((a & 1) | (((a >> 1) & 1) & g));
+            p = (uint8_t)(p & ((a >> 1) & 1));
+        }
+    }
+    sg[t] = g;
+    sp[t] = p;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint8_t g1 = 0, p1 = 1;
+        if (t >= off) { g1 = sg[t - off]; p1 = sp[t - off]; }
+        __syncthreads();
+        uint8_t g2 = sg[t], p2 = sp[t];
+        sg[t] = (uint8_t)(g2 | (p2 & g1));
+        sp[t] = (uint8_t)(p1 & p2);
+        __syncthreads();
+    }
+    uint8_t c = t ? sg[t - 1] : 0;
+    const uint64_t gper = FA_CRT16_GROUPS - 1;
+    for (uint64_t b = b0; b < b1; b += 8) {
+        uint64_t w;
+        if (b + 8 <= b1) {
+            w = *(const uint64_t*)(agg + b);
+        } else {
+            w = 0;
+            for (uint64_t q = b; q < b1; q++) w |= (uint64_t)agg[q] << (8 * (q - b));
+            for (uint64_t q = b1; q < b + 8; q++) w |= (uint64_t)2 << (8 * (q - b));
+        }
+        for (int q = 0; q < 8; q++) {
+            const uint8_t a = (uint8_t)(w >> (8 * q));
+            if (c == 0 && !(a & 4)) {
+                // no carry in and none generated inside: nothing to add,
+                // no carry out
+                continue;
+            }
+            const uint64_t gs = (b + q) * gper;
+            const uint64_t ge = (gs + gper < ngroups) ? gs + gper : ngroups;
+            for (uint64_t gi = gs; gi < ge; gi++) {
+                const uint8_t f = flags[gi];
+                if (c) {
+                    const uint64_t zb = gi * own;
+                    for (int l = 0; l < own && zb + l < zn; l++) {
+                        const uint64_t v = z[zb + l] + 1;
+                        z[zb + l] = v;
+                        if (v != 0) break;
+                    }
+                }
+                c = (uint8_t)((f & 1) | (((f >> 1) & 1) & c));
+            }
+        }
+    }
+}
+
+template <int NP>
+static void launch_crt16(const double* Q, uint64_t stride, const fa_crt_consts& C, uint64_t zlen,
+                         int bits, uint64_t* z, uint64_t zn, uint64_t ngroups, uint8_t* flags,
+                         uint8_t* agg, uint64_t nblk, hipStream_t s)
+{
+    FA_LAUNCH("fa_k_crt_mpn16", fa_k_crt_mpn16<NP>, dim3((unsigned)nblk), dim3(256), 0, s, Q,
+              stride, C, zlen, bits, z, zn, ngroups, flags, agg);
+}
+
+static int crt16(int np, const double* Q, uint64_t stride, const fa_crt_consts& C, uint64_t zlen,
+                 int bits, uint64_t* z, uint64_t zn, uint64_t ngroups, uint8_t* flags, uint8_t* agg,
+                 uint8_t* cin, uint64_t nblk, hipStream_t s)
+{
+    switch (np) {
+    case 2: launch_crt16<2>(Q, stride, C, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
+    case 3: launch_crt16<3>(Q, stride, C, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
+    case 4: launch_crt16<4>(Q, stride, C, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
+    case 5: launch_crt16<5>(Q, stride, C, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
+    case 6: launch_crt16<6>(Q, stride, C, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
+    case 7: launch_crt16<7>(Q, stride, C, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
+    case 8: launch_crt16<8>(Q, stride, C, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
+    default: return fa_set_error(FA_E_ARG, "crt16: np %d", np);
+    }
+    (void)cin;
+    FA_LAUNCH("fa_k_carry_fused", fa_k_carry_fused, dim3(1), dim3(1024), 0, s, flags, agg, z, zn,
+              bits / 8, ngroups, nblk);
+    return FA_OK;
+}
+
+// ------------------------------------------------------------------------
 // host drivers
 // ------------------------------------------------------------------------
 
@@ -504,6 +778,13 @@ static fa_operand to_operand(const fa_pack_desc* d)
     return o;
 }
 
+int fa_convolve_fast(int depth, const fa_operand& oa, const fa_operand& ob, int np, double* A,
+                     double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s);
+
+int fa_fast_enabled = 1;   // switched off by tests comparing the two kernel sets
+
+extern "C" FA_API void fa_set_fast_kernels(int on) { fa_fast_enabled = on; }
+
 static int col_lg(int d1, int d2)
 {
     int lgC = 0;
@@ -524,6 +805,10 @@ int fa_convolve(fa_thread_state* st, const fa_tables* T, int np, int offset, int
         size_t lds = 2 * stride * sizeof(double);
         FA_LAUNCH("fa_k_small_conv", fa_k_small_conv, dim3(np), dim3(256), lds, stream, oa, ob, depth, A,
                            stride, S);
+        LAUNCH_CHECK();
+        return FA_OK;
+    }
+    if (fa_fast_enabled && fa_convolve_fast(depth, oa, ob, np, A, B, stride, S, stream)) {
         LAUNCH_CHECK();
         return FA_OK;
     }
@@ -558,16 +843,18 @@ static size_t mpn_scratch_bytes(const fa_mpn_plan_t& P, uint64_t zn, uint64_t* n
     uint64_t own = (uint64_t)P.bits / 8;
     *ngroups = (zn + own - 1) / own;
     *nblk = (*ngroups + (FA_CRT_WG - 2)) / (FA_CRT_WG - 1);
+    uint64_t nblk16 = (*ngroups + 29) / 30;   // covers the 31-group workgroups
     size_t tr = ((size_t)P.np << P.depth) * sizeof(double);
-    size_t bytes = 2 * tr + *ngroups + 2 * (*nblk) + 4096;
+    size_t bytes = 2 * tr + *ngroups + 2 * nblk16 + 4096;
     return bytes;
 }
 
 static int mpn_mul_device(uint64_t* z, const uint64_t* a, uint64_t an, const uint64_t* b,
-                          uint64_t bn, hipStream_t stream)
+                          uint64_t bn, hipStream_t stream, int np_force = 0, int bits_force = 0)
 {
     fa_mpn_plan_t P;
-    int rc = fa_mpn_choose_plan(an, bn, &P);
+    int rc = np_force ? fa_mpn_fixed_plan(an, bn, np_force, bits_force, &P)
+                      : fa_mpn_choose_plan(an, bn, &P);
     if (rc) return rc;
     fa_tables* T = fa_get_tables(P.depth);
     if (!T) return FA_E_NODEV;
@@ -584,8 +871,10 @@ static int mpn_mul_device(uint64_t* z, const uint64_t* a, uint64_t an, const uin
     W.A = (double*)buf;
     W.B = (double*)((char*)buf + tr);
     W.flags = (uint8_t*)((char*)buf + 2 * tr);
-    W.agg = W.flags + ngroups;
-    W.cin = W.agg + nblk;
+    // aggregates start on an 8-byte boundary: the carry kernel reads them
+    // as 64-bit words
+    W.agg = W.flags + ((ngroups + 7) & ~UINT64_C(7));
+    W.cin = W.agg + (((ngroups + 29) / 30 + 7) & ~UINT64_C(7));
 
     fa_pack_desc da = {FA_PACK_MPN, a, an, P.bits, 1};
     fa_pack_desc db = {FA_PACK_MPN, b, bn, P.bits, 1};
@@ -595,13 +884,21 @@ static int mpn_mul_device(uint64_t* z, const uint64_t* a, uint64_t an, const uin
     fa_crt_consts C;
     fa_crt_consts_init(&C, P.np, 0);
     const uint64_t stride = UINT64_C(1) << P.depth;
+    if (fa_fast_enabled) {
+        uint64_t nblk16 = (ngroups + (FA_CRT16_GROUPS - 2)) / (FA_CRT16_GROUPS - 1);
+        rc = crt16(P.np, W.A, stride, C, P.zlen, P.bits, z, zn, ngroups, W.flags, W.agg, W.cin,
+                   nblk16, stream);
+        if (rc) return rc;
+        LAUNCH_CHECK();
+        return FA_OK;
+    }
     FA_LAUNCH("fa_k_crt_mpn", fa_k_crt_mpn, dim3((unsigned)nblk), dim3(FA_CRT_WG), 0, stream, W.A,
-                       stride, C, P.zlen, P.bits, z, zn, ngroups, W.flags, W.agg);
+              stride, C, P.zlen, P.bits, z, zn, ngroups, W.flags, W.agg);
     LAUNCH_CHECK();
-    FA_LAUNCH("fa_k_carry_scan", fa_k_carry_scan, dim3(1), dim3(256), 0, stream, W.agg, W.cin, nblk);
+    FA_LAUNCH("fa_k_carry_scan", fa_k_carry_scan, dim3(1), dim3(1024), 0, stream, W.agg, W.cin, nblk);
     LAUNCH_CHECK();
     FA_LAUNCH("fa_k_carry_apply", fa_k_carry_apply, dim3((unsigned)nblk), dim3(FA_CRT_WG), 0, stream,
-                       W.flags, W.cin, z, zn, P.bits / 8, ngroups);
+              W.flags, W.cin, z, zn, P.bits / 8, ngroups);
     LAUNCH_CHECK();
     return FA_OK;
 }
@@ -614,8 +911,8 @@ extern "C" int fa_mpn_mul_device(uint64_t* z, const uint64_t* a, int64_t an, con
     return mpn_mul_device(z, a, (uint64_t)an, b, (uint64_t)bn, (hipStream_t)stream);
 }
 
-extern "C" int fa_mpn_mul(uint64_t* z, const uint64_t* a, int64_t an, const uint64_t* b,
-                          int64_t bn)
+static int mpn_mul_host(uint64_t* z, const uint64_t* a, int64_t an, const uint64_t* b, int64_t bn,
+                        int np, int bits)
 {
     if (an <= 0 || bn <= 0 || !z || !a || !b)
         return fa_set_error(FA_E_ARG, "fa_mpn_mul: bad arguments");
@@ -632,9 +929,9 @@ extern "C" int fa_mpn_mul(uint64_t* z, const uint64_t* a, int64_t an, const uint
     if (hipMemcpyAsync(da, a, an * sizeof(uint64_t), hipMemcpyHostToDevice, st->stream) != hipSuccess ||
         hipMemcpyAsync(db, b, bn * sizeof(uint64_t), hipMemcpyHostToDevice, st->stream) != hipSuccess)
         return fa_set_error(FA_E_HIP, "operand upload failed");
-    rc = mpn_mul_device(dz, da, (uint64_t)an, db, (uint64_t)bn, st->stream);
+    rc = mpn_mul_device(dz, da, (uint64_t)an, db, (uint64_t)bn, st->stream, np, bits);
     if (rc) {
-        hipStreamSynchronize(st->stream);
+        (void)hipStreamSynchronize(st->stream);
         return rc;
     }
     if (hipMemcpyAsync(z, dz, zn * sizeof(uint64_t), hipMemcpyDeviceToHost, st->stream) != hipSuccess)
@@ -643,6 +940,28 @@ extern "C" int fa_mpn_mul(uint64_t* z, const uint64_t* a, int64_t an, const uint
         return fa_set_error(FA_E_HIP, "stream synchronize failed: %s",
                             hipGetErrorString(hipGetLastError()));
     return FA_OK;
+}
+
+extern "C" int fa_mpn_mul(uint64_t* z, const uint64_t* a, int64_t an, const uint64_t* b,
+                          int64_t bn)
+{
+    return mpn_mul_host(z, a, an, b, bn, 0, 0);
+}
+
+extern "C" int fa_mpn_mul_plan(uint64_t* z, const uint64_t* a, int64_t an, const uint64_t* b,
+                               int64_t bn, int np, int bits)
+{
+    if (np == 0) return fa_set_error(FA_E_ARG, "np must be given");
+    return mpn_mul_host(z, a, an, b, bn, np, bits);
+}
+
+extern "C" int fa_mpn_mul_device_plan(uint64_t* z, const uint64_t* a, int64_t an,
+                                      const uint64_t* b, int64_t bn, int np, int bits,
+                                      void* stream)
+{
+    if (an <= 0 || bn <= 0 || !z || !a || !b || np == 0)
+        return fa_set_error(FA_E_ARG, "fa_mpn_mul_device_plan: bad arguments");
+    return mpn_mul_device(z, a, (uint64_t)an, b, (uint64_t)bn, (hipStream_t)stream, np, bits);
 }
 
 extern "C" void mpn_mul_default_mpn_ctx(uint64_t* r1, const uint64_t* i1, int64_t n1,

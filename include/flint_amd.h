@@ -69,6 +69,14 @@ FA_API int fa_mpn_mul(uint64_t* z, const uint64_t* a, int64_t an,
 FA_API int fa_mpn_mul_device(uint64_t* z, const uint64_t* a, int64_t an,
                       const uint64_t* b, int64_t bn, void* stream);
 
+/* the same with a caller-chosen plan: np primes, bits-bit chunks (checked
+   against the exactness bound of mpn_mul.c:79-95); the autotuner's knob */
+FA_API int fa_mpn_mul_plan(uint64_t* z, const uint64_t* a, int64_t an,
+                           const uint64_t* b, int64_t bn, int np, int bits);
+FA_API int fa_mpn_mul_device_plan(uint64_t* z, const uint64_t* a, int64_t an,
+                                  const uint64_t* b, int64_t bn, int np, int bits,
+                                  void* stream);
+
 /* plan the device call would use: out = {np, bits, depth, d1, d2, zlen} */
 FA_API int fa_mpn_plan(int64_t an, int64_t bn, int64_t* out);
 
@@ -140,9 +148,17 @@ FA_API void fa_prof_enable(int on);
 FA_API void fa_prof_reset(void);
 /* names: max*64 chars; ms: total milliseconds; counts: launches */
 FA_API int fa_prof_read(char* names, double* ms, int64_t* counts, int max);
+/* selects the register radix-16 kernels (1, default) or the generic LDS
+   radix-4 kernels (0) for transform lengths 2^16, 2^20, 2^24 */
+FA_API void fa_set_fast_kernels(int on);
+/* column-pass tiles at 2^24 points: 1 = 8 columns (default), 0 = 4 columns */
+FA_API void fa_set_col_variant(int v);
 /* plain device-to-device copy kernel (nbytes multiple of 16), used to
    calibrate hardware byte counters */
 FA_API int fa_copy_device(void* dst, const void* src, uint64_t nbytes, void* stream);
+/* copy of a 2^d1 x 2^d2 double array with the column passes' access
+   pattern (tiles of 2^lgC columns, 16 rows per thread) and no arithmetic */
+FA_API int fa_probe_colcopy(const void* src, void* dst, int d1, int d2, int lgC, void* stream);
 
 #ifdef __cplusplus
 }

@@ -39,6 +39,21 @@ def test_mpn_mul_all_ones(an, bn):
     assert to_int(z) == to_int(a) * to_int(b)
 
 
+@pytest.mark.parametrize("n", [1 << 16, 1 << 20, 3 << 20])
+def test_mpn_mul_all_ones_closed_form(n):
+    """(2^(64n) - 1)^2 = 2^(128n) - 2^(64n+1) + 1: limb 0 is 1, limbs 1..n-1
+    are 0, limb n is 2^64 - 2, the rest all ones.  Every CRT workgroup
+    propagates, so the carry resolution crosses the whole product."""
+    import flint_amd as fa
+    a = np.full(n, np.uint64(2**64 - 1), dtype=np.uint64)
+    z = fa.mpn_mul(a, a)
+    want = np.full(2 * n, np.uint64(2**64 - 1), dtype=np.uint64)
+    want[0] = 1
+    want[1:n] = 0
+    want[n] = np.uint64(2**64 - 2)
+    assert np.array_equal(z, want)
+
+
 def test_mpn_mul_sparse_and_powers():
     import flint_amd as fa
     for an, bn in [(5000, 5000), (70000, 1000)]:
@@ -85,3 +100,67 @@ def test_config_checksums(name):
     print(name, fa.mpn_plan(rec["an"], rec["bn"]), flush=True)
     z = fa.mpn_mul(a, b)
     assert hashlib.sha256(z.tobytes()).hexdigest() == rec["sha256"]
+
+
+@pytest.mark.parametrize("an,bn,np_,bits", [(65536, 65536, 6, 136), (60000, 50000, 6, 136),
+                                           (1 << 20, 1 << 20, 6, 136), (1000000, 3, 4, 88),
+                                           (700000, 650000, 8, 184), (3 << 20, 2 << 20, 6, 136),
+                                           (30000, 30000, 3, 64), (5000, 4000, 2, 40)])
+def test_fast_kernels_match_generic(an, bn, np_, bits):
+    """the register radix-16 kernels and the generic LDS kernels give the
+    same product (and both the exact one)"""
+    import flint_amd as fa
+    rng = np.random.default_rng(an ^ bn)
+    a = rand_limbs(rng, an)
+    b = rand_limbs(rng, bn)
+    print("fast-vs-generic", an, bn, np_, bits, flush=True)
+    try:
+        fa.set_fast_kernels(False)
+        z0 = fa.mpn_mul_plan(a, b, np_, bits)
+    finally:
+        fa.set_fast_kernels(True)
+    z1 = fa.mpn_mul_plan(a, b, np_, bits)
+    assert np.array_equal(z0, z1)
+    if an * bn < (1 << 34):
+        assert to_int(z1) == to_int(a) * to_int(b)
+    else:
+        q = (1 << 61) - 1
+        assert (to_int(a) % q) * (to_int(b) % q) % q == to_int(z1) % q
+
+
+@pytest.mark.parametrize("depth", list(range(13, 25)))
+def test_fast_kernels_every_depth(depth):
+    """every column depth d1 = depth - 12 of the radix-16 kernels against the
+    generic kernels and an exact check, with the plan forced to np=6/136"""
+    import flint_amd as fa
+    n = max(1, int(0.7 * 2 ** depth * 136 / 128))
+    rng = np.random.default_rng(depth)
+    a = rand_limbs(rng, n)
+    b = rand_limbs(rng, n - 7 if n > 8 else n)
+    z1 = fa.mpn_mul_plan(a, b, 6, 136)
+    try:
+        fa.set_fast_kernels(False)
+        z0 = fa.mpn_mul_plan(a, b, 6, 136)
+    finally:
+        fa.set_fast_kernels(True)
+    assert np.array_equal(z0, z1)
+    q = (1 << 61) - 1
+    assert (to_int(a) % q) * (to_int(b) % q) % q == to_int(z1) % q
+
+
+@pytest.mark.parametrize("an,bn", [(1 << 24, 1 << 24), (15000000, 14000000)])
+def test_column_tile_variants_agree(an, bn):
+    """8-column and 4-column tiles of the 2^24-point column passes give the
+    same product"""
+    import flint_amd as fa
+    a, b = fa.synth_mpn_pair(0xC01, an, bn)
+    print("col-variants", an, bn, fa.mpn_plan(an, bn), flush=True)
+    try:
+        fa.set_col_variant(0)
+        z0 = fa.mpn_mul(a, b)
+    finally:
+        fa.set_col_variant(1)
+    z1 = fa.mpn_mul(a, b)
+    assert np.array_equal(z0, z1)
+    q = (1 << 61) - 1
+    assert (to_int(a) % q) * (to_int(b) % q) % q == to_int(z1) % q
