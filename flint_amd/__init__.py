@@ -72,6 +72,7 @@ SIGNATURES = {
     "fa_copy_device": (ctypes.c_int, [_VP, _VP, _u64, _VP]),
     "fa_set_fast_kernels": (None, [ctypes.c_int]),
     "fa_set_col_variant": (None, [ctypes.c_int]),
+    "fa_set_crt_variant": (None, [ctypes.c_int]),
     "fa_probe_colcopy": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP]),
 }
 
@@ -157,6 +158,11 @@ def set_fast_kernels(on=True):
 def set_col_variant(v):
     """column-pass tiles at 2^24 points: 1 = 8 columns per workgroup, 0 = 4"""
     lib().fa_set_col_variant(int(v))
+
+
+def set_crt_variant(v):
+    """CRT of integer products: 1 = FP64 radix-2^24 digits, 0 = 64-bit limbs"""
+    lib().fa_set_crt_variant(int(v))
 
 
 def copy_device(dst_ptr, src_ptr, nbytes, stream=None):

@@ -8,6 +8,7 @@
 
 #include "fa_arith.h"
 #include "fa_internal.h"
+#include "fa_crt_fp.h"
 
 typedef unsigned __int128 u128;
 
@@ -448,6 +449,30 @@ void fa_crt_consts_init(fa_crt_consts* C, int np, int offset)
     C->nl = FA_CRT_MAXL;
     while (C->nl > 1 && C->M[C->nl - 1] == 0) C->nl--;
     for (int i = 0; i < np; i++) ml_divexact_1(C->cof[i], C->M, FA_CRT_MAXL, P[offset + i]);
+}
+
+// 24-bit digits of a multi-limb number
+static void ml_digits24(double* out, int nd, const uint64_t* x, int nl)
+{
+    for (int d = 0; d < nd; d++) {
+        int pos = 24 * d, l = pos / 64, b = pos % 64;
+        uint64_t v = 0;
+        if (l < nl) v = x[l] >> b;
+        if (b > 40 && l + 1 < nl) v |= x[l + 1] << (64 - b);
+        out[d] = (double)(v & 0xffffff);
+    }
+}
+
+void fa_crt_fp_consts_init(fa_crt_fp_consts* F, const fa_crt_consts& C)
+{
+    memset(F, 0, sizeof(*F));
+    for (int i = 0; i < C.np; i++) {
+        F->p[i] = C.p[i];
+        F->pinv[i] = C.pinv[i];
+        F->rp[i] = C.rp[i];
+        ml_digits24(F->cof[i], 16, C.cof[i], FA_CRT_MAXL);
+    }
+    ml_digits24(F->M, 18, C.M, FA_CRT_MAXL);
 }
 
 uint64_t fa_crt_cop(int np, int i)

@@ -415,28 +415,27 @@ __global__ void __launch_bounds__(1024) fa_k_col_inv16(double* __restrict__ A, i
 }
 
 // ---------------------------------------------------------------------------
-// 8-column variant for D1 = 12: each thread carries columns c and c + 4 of
-// an 8-column tile (rows stage-layout as above), so a wave's pair of loads
-// covers 64 contiguous bytes of every row it touches; the LDS exchange of
-// the 4096-row columns runs in two halves of 4 columns.
+// 8-column tiles for D1 = 12.  Lane l of a workgroup takes column
+// c = l mod 8 and u = l / 8 < 128, and at every stage holds two radix-16
+// groups, tc = u and tc = u + 128 (x0, x1), so one wave instruction covers
+// 8 rows x 64 contiguous bytes.  A stage exchange runs in two halves of the
+// column (rows [0, 2048) and [2048, 4096)), each 8 columns x 2048 rows in
+// LDS:
+//  - stage 0 -> 1: a stage-0 group (rows tc + 256k) has k < 8 in the first
+//    half and k >= 8 in the second; the stage-1 group read back in half H is
+//    tc = 128H + u (blocks 8H .. 8H+7 of level 4);
+//  - stage 1 -> 2: stage-1 group u + 128H and stage-2 group u + 128H both
+//    lie in half H.
+// Column rows are 2184 doubles apart (8 mod 32), so the 8 columns of one
+// access fall on distinct banks in pairs.
 // ---------------------------------------------------------------------------
+#define FA_C8_PLH 2184
+#define FA_C8_LDS (8 * FA_C8_PLH)
 
-FA_DEV void fa_exchange2(double* lds, double* x0, double* x1, int c, int tc, int jfrom, int jto)
-{
-    typedef col_geom<12> G;
-    double* my = lds + c * G::PLEN;
-#pragma unroll
-    for (int k = 0; k < 16; k++) my[PADX(G::pos(jfrom, tc, k))] = x0[k];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; k++) x0[k] = my[PADX(G::pos(jto, tc, k))];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; k++) my[PADX(G::pos(jfrom, tc, k))] = x1[k];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; k++) x1[k] = my[PADX(G::pos(jto, tc, k))];
-}
+FA_DEV int fa_c8_addr(int c, int row) { return c * FA_C8_PLH + PADX(row & 2047); }
+
+// stage-1 row of group tc, element k
+FA_DEV int fa_c8_row1(int tc, int k) { return ((tc >> 4) << 8) + (tc & 15) + 16 * k; }
 
 __global__ void __launch_bounds__(1024) fa_k_col_fwd16w(fa_operand a, fa_operand b, int d2,
                                                         double* __restrict__ A,
@@ -444,8 +443,7 @@ __global__ void __launch_bounds__(1024) fa_k_col_fwd16w(fa_operand a, fa_operand
                                                         fa_prime_set S, int np, int ngroups,
                                                         int nops)
 {
-    typedef col_geom<12> G;
-    __shared__ double lds[4 * G::PLEN];
+    __shared__ double lds[FA_C8_LDS];
     const unsigned L = fa_xcd_linear();
     const int i = L % np;
     const unsigned rest = L / np;
@@ -455,31 +453,67 @@ __global__ void __launch_bounds__(1024) fa_k_col_fwd16w(fa_operand a, fa_operand
     const double n = S.p[i], ninv = S.pinv[i];
     const fa_operand& op = z ? b : a;
     double* dst = (z ? B : A) + (uint64_t)i * stride;
-    const int c = threadIdx.x & 3, tc = threadIdx.x >> 2;
-    const uint64_t col0 = (uint64_t)g * 8 + c, col1 = col0 + 4;
+    const int c = threadIdx.x & 7, u = threadIdx.x >> 3;
+    const uint64_t col = (uint64_t)g * 8 + c;
     const double* w2 = S.w2[i];
     double x0[16], x1[16], tw[15];
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-        const uint64_t r = (uint64_t)G::pos(0, tc, k) << d2;
-        x0[k] = fa_operand_value(op, r + col0, S.pow2[i], n, ninv);
-        x1[k] = fa_operand_value(op, r + col1, S.pow2[i], n, ninv);
+        x0[k] = fa_operand_value(op, ((uint64_t)(u + 256 * k) << d2) + col, S.pow2[i], n, ninv);
+        x1[k] = fa_operand_value(op, ((uint64_t)(u + 128 + 256 * k) << d2) + col, S.pow2[i], n,
+                                 ninv);
     }
+    // stage 0: levels 0..3, block 0
+    fa_tw16(w2, 0, tw);
+    fa_fwd16(x0, tw, n, ninv);
+    fa_fwd16(x1, tw, n, ninv);
+    // exchange 0 -> 1 in two halves
+    double y0[16], y1[16];
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-        if (j > 0) {
-            fa_exchange2(lds, x0, x1, c, tc, j - 1, j);
+    for (int H = 0; H < 2; H++) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            lds[fa_c8_addr(c, u + 256 * (8 * H + k))] = x0[8 * H + k];
+            lds[fa_c8_addr(c, u + 128 + 256 * (8 * H + k))] = x1[8 * H + k];
         }
-        fa_tw16(w2, (uint64_t)G::block(j, tc), tw);
-        fa_fwd16(x0, tw, n, ninv);
-        fa_fwd16(x1, tw, n, ninv);
-        if (j < 2) __syncthreads();
+        __syncthreads();
+        const int tc = 128 * H + u;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const double v = lds[fa_c8_addr(c, fa_c8_row1(tc, k))];
+            if (H == 0) y0[k] = v; else y1[k] = v;
+        }
+        __syncthreads();
     }
+    // stage 1: levels 4..7, block tc >> 4
+    fa_tw16(w2, (uint64_t)(u >> 4), tw);
+    fa_fwd16(y0, tw, n, ninv);
+    fa_tw16(w2, (uint64_t)((u + 128) >> 4), tw);
+    fa_fwd16(y1, tw, n, ninv);
+    // exchange 1 -> 2, half H holds y_H entirely
+#pragma unroll
+    for (int H = 0; H < 2; H++) {
+        const int tc = 128 * H + u;
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            lds[fa_c8_addr(c, fa_c8_row1(tc, k))] = H == 0 ? y0[k] : y1[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const double v = lds[fa_c8_addr(c, 16 * tc + k)];
+            if (H == 0) x0[k] = v; else x1[k] = v;
+        }
+        if (H == 0) __syncthreads();
+    }
+    // stage 2: levels 8..11, block tc
+    fa_tw16(w2, (uint64_t)u, tw);
+    fa_fwd16(x0, tw, n, ninv);
+    fa_tw16(w2, (uint64_t)(u + 128), tw);
+    fa_fwd16(x1, tw, n, ninv);
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-        const uint64_t r = (uint64_t)G::pos(2, tc, k) << d2;
-        dst[r + col0] = x0[k];
-        dst[r + col1] = x1[k];
+        dst[((uint64_t)(16 * u + k) << d2) + col] = x0[k];
+        dst[((uint64_t)(16 * (u + 128) + k) << d2) + col] = x1[k];
     }
 }
 
@@ -487,39 +521,71 @@ __global__ void __launch_bounds__(1024) fa_k_col_inv16w(double* __restrict__ A, 
                                                         uint64_t stride, fa_prime_set S, int np,
                                                         int ngroups)
 {
-    typedef col_geom<12> G;
-    __shared__ double lds[4 * G::PLEN];
+    __shared__ double lds[FA_C8_LDS];
     const unsigned L = fa_xcd_linear();
     const int i = L % np;
     const int g = L / np;
     if (g >= ngroups) return;
     const double n = S.p[i], ninv = S.pinv[i];
     double* dst = A + (uint64_t)i * stride;
-    const int c = threadIdx.x & 3, tc = threadIdx.x >> 2;
-    const uint64_t col0 = (uint64_t)g * 8 + c, col1 = col0 + 4;
+    const int c = threadIdx.x & 7, u = threadIdx.x >> 3;
+    const uint64_t col = (uint64_t)g * 8 + c;
     const double* wi = S.w2inv[i];
     double x0[16], x1[16], tw[15];
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-        const uint64_t r = (uint64_t)G::pos(2, tc, k) << d2;
-        x0[k] = dst[r + col0];
-        x1[k] = dst[r + col1];
+        x0[k] = dst[((uint64_t)(16 * u + k) << d2) + col];
+        x1[k] = dst[((uint64_t)(16 * (u + 128) + k) << d2) + col];
     }
+    // stage 2 (levels 11..8), block tc
+    fa_tw16(wi, (uint64_t)u, tw);
+    fa_inv16(x0, tw, n, ninv);
+    fa_tw16(wi, (uint64_t)(u + 128), tw);
+    fa_inv16(x1, tw, n, ninv);
+    // exchange 2 -> 1, half H holds x_H entirely
+    double y0[16], y1[16];
 #pragma unroll
-    for (int j = 2; j >= 0; j--) {
-        if (j < 2) {
-            fa_exchange2(lds, x0, x1, c, tc, j + 1, j);
+    for (int H = 0; H < 2; H++) {
+        const int tc = 128 * H + u;
+#pragma unroll
+        for (int k = 0; k < 16; k++) lds[fa_c8_addr(c, 16 * tc + k)] = H == 0 ? x0[k] : x1[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const double v = lds[fa_c8_addr(c, fa_c8_row1(tc, k))];
+            if (H == 0) y0[k] = v; else y1[k] = v;
         }
-        fa_tw16(wi, (uint64_t)G::block(j, tc), tw);
-        fa_inv16(x0, tw, n, ninv);
-        fa_inv16(x1, tw, n, ninv);
-        if (j > 0) __syncthreads();
+        __syncthreads();
     }
+    // stage 1 (levels 7..4), block tc >> 4
+    fa_tw16(wi, (uint64_t)(u >> 4), tw);
+    fa_inv16(y0, tw, n, ninv);
+    fa_tw16(wi, (uint64_t)((u + 128) >> 4), tw);
+    fa_inv16(y1, tw, n, ninv);
+    // exchange 1 -> 0: half H receives stage-1 group 128H + u and returns
+    // elements 8H .. 8H+7 of both stage-0 groups
+#pragma unroll
+    for (int H = 0; H < 2; H++) {
+        const int tc = 128 * H + u;
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            lds[fa_c8_addr(c, fa_c8_row1(tc, k))] = H == 0 ? y0[k] : y1[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            x0[8 * H + k] = lds[fa_c8_addr(c, u + 256 * (8 * H + k))];
+            x1[8 * H + k] = lds[fa_c8_addr(c, u + 128 + 256 * (8 * H + k))];
+        }
+        if (H == 0) __syncthreads();
+    }
+    // stage 0 (levels 3..0), block 0
+    fa_tw16(wi, 0, tw);
+    fa_inv16(x0, tw, n, ninv);
+    fa_inv16(x1, tw, n, ninv);
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-        const uint64_t r = (uint64_t)G::pos(0, tc, k) << d2;
-        dst[r + col0] = x0[k];
-        dst[r + col1] = x1[k];
+        dst[((uint64_t)(u + 256 * k) << d2) + col] = x0[k];
+        dst[((uint64_t)(u + 128 + 256 * k) << d2) + col] = x1[k];
     }
 }
 

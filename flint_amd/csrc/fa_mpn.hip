@@ -17,6 +17,7 @@
 #include "fa_ntt.h"
 
 #include "fa_pack.h"
+#include "fa_crt_fp.h"
 
 // ------------------------------------------------------------------------
 // transform kernels
@@ -727,6 +728,155 @@ __global__ void __launch_bounds__(1024) fa_k_carry_fused(const uint8_t* __restri
     }
 }
 
+// ------------------------------------------------------------------------
+// CRT on the FP64 lanes (fa_crt_fp.h), 31 groups of 8 coefficients per
+// workgroup of 256 threads plus one halo group in front (its overhang
+// reaches the first group).
+//   1. thread t: coefficient 8 g0 + t -> NL limbs, shifted left by
+//      (bits * (t mod 8)) mod 64 into NL + 1 limbs in LDS;
+//   2. thread per (group, limb p): sum of the <= 5 shifted coefficients
+//      overlapping p, as a 64-bit value and a small carry count;
+//   3. thread per group: carries through the group's TL limbs;
+//   4. thread per group >= 1: adds the previous group's overhang (limbs
+//      own .. TL-1, which fit in own limbs) into its own limbs, records the
+//      (generate, propagate) flag of the carry into the next group;
+//   5. coalesced store of the owned limbs.
+// ------------------------------------------------------------------------
+#define FA_CRTFP_TL 32   // max limbs of a group's sum: 7*200/64 + 8 + 1
+
+template <int NP>
+__global__ void __launch_bounds__(256) fa_k_crt_fp(const double* __restrict__ Q, uint64_t stride,
+                                                   fa_crt_fp_consts F, uint64_t zlen, int bits,
+                                                   uint64_t* __restrict__ z, uint64_t zn,
+                                                   uint64_t ngroups, uint8_t* __restrict__ flags,
+                                                   uint8_t* __restrict__ agg)
+{
+    constexpr int NL = fa_crt_fp_dims<NP>::NL;
+    constexpr int NLS = NL + 1;
+    __shared__ uint64_t sC[256 * NLS];
+    __shared__ uint64_t sLo[FA_CRT16_GROUPS][FA_CRTFP_TL + 1];
+    __shared__ uint8_t sHi[FA_CRT16_GROUPS][FA_CRTFP_TL];
+    __shared__ uint8_t sF[FA_CRT16_GROUPS];
+    const int t = threadIdx.x;
+    const int own = bits >> 3;
+    const int TL = ((7 * bits) >> 6) + NLS;
+    const int64_t g0 = (int64_t)blockIdx.x * (FA_CRT16_GROUPS - 1) - 1;
+
+    {   // 1. the coefficient of thread t, shifted to its bit offset in the group
+        const int64_t j = g0 * 8 + t;
+        uint64_t L[NL];
+        if (j >= 0 && (uint64_t)j < zlen) {
+            fa_crt_fp_value<NP>(Q, stride, (uint64_t)j, F, L);
+        } else {
+#pragma unroll
+            for (int l = 0; l < NL; l++) L[l] = 0;
+        }
+        const int sh = (bits * (t & 7)) & 63;
+        uint64_t* o = sC + t * NLS;
+        uint64_t prev = 0;
+#pragma unroll
+        for (int l = 0; l < NL; l++) {
+            o[l] = (L[l] << sh) | (sh ? (prev >> (64 - sh)) : 0);
+            prev = L[l];
+        }
+        o[NL] = sh ? (prev >> (64 - sh)) : 0;
+    }
+    __syncthreads();
+
+    // 2. limb p of group gl: coefficients k with o_k <= p < o_k + NLS,
+    //    o_k = (bits k) >> 6
+    for (int e = t; e < FA_CRT16_GROUPS * TL; e += 256) {
+        const int gl = e / TL, p = e - gl * TL;
+        int kmax = (64 * p + 63) / bits;
+        if (kmax > 7) kmax = 7;
+        int kmin = 0;
+        if (p >= NLS) kmin = (64 * (p - NLS + 1) + bits - 1) / bits;
+        uint64_t lo = 0;
+        unsigned hi = 0;
+        for (int k = kmin; k <= kmax; k++) {
+            const int idx = p - ((bits * k) >> 6);
+            if (idx < 0 || idx >= NLS) continue;
+            const uint64_t v = sC[(gl * 8 + k) * NLS + idx];
+            lo += v;
+            hi += (lo < v);
+        }
+        sLo[gl][p] = lo;
+        sHi[gl][p] = (uint8_t)hi;
+    }
+    __syncthreads();
+
+    // 3. carries inside each group
+    if (t < FA_CRT16_GROUPS) {
+        uint64_t c = 0;
+        for (int p = 0; p < TL; p++) {
+            const uint64_t v = sLo[t][p] + c;
+            c = (uint64_t)sHi[t][p] + (v < c);
+            sLo[t][p] = v;
+        }
+        sLo[t][TL] = c;   // zero: the group's sum fits in TL limbs
+    }
+    __syncthreads();
+
+    // 4. overhang of the previous group into the owned limbs
+    if (t >= 1 && t < FA_CRT16_GROUPS) {
+        const int64_t g = g0 + t;
+        uint8_t f = 2;   // identity of the carry operator: propagate, no generate
+        if (g >= 0 && (uint64_t)g < ngroups) {
+            const int ovh = TL - own;
+            uint64_t carry = 0;
+            int prop = 1;
+            for (int l = 0; l < own; l++) {
+                if ((uint64_t)g * own + l >= zn) { prop = 0; break; }
+                const uint64_t add = (l < ovh) ? sLo[t - 1][own + l] : 0;
+                uint64_t v = sLo[t][l];
+                uint64_t tt = v + add;
+                uint64_t c1 = (tt < add);
+                tt += carry;
+                c1 |= (tt < carry);
+                carry = c1;
+                sLo[t][l] = tt;
+                if (tt != ~UINT64_C(0)) prop = 0;
+            }
+            f = (uint8_t)(carry | (prop << 1));
+            flags[g] = f;
+        }
+        sF[t] = f;
+    }
+    __syncthreads();
+    if (t == 0) {
+        // bit 0: carry out with no carry in, bit 1: every group propagates,
+        // bit 2: some group generates a carry
+        uint8_t gg = 0, pp = 1, any = 0;
+        for (int k = 1; k < FA_CRT16_GROUPS; k++) {
+            const uint8_t ft = sF[k];
+            gg = (uint8_t)((ft & 1) | (((ft >> 1) & 1) & gg));
+            pp = (uint8_t)(pp & ((ft >> 1) & 1));
+            any = (uint8_t)(any | (ft & 1));
+        }
+        agg[blockIdx.x] = (uint8_t)(gg | (pp << 1) | (any << 2));
+    }
+    // 5. owned limbs of groups 1..31, contiguous in z
+    const uint64_t base = (uint64_t)(g0 + 1) * own;
+    const int nl = (FA_CRT16_GROUPS - 1) * own;
+    for (int e = t; e < nl; e += 256) {
+        const uint64_t za = base + e;
+        const int gl = 1 + e / own;
+        if (za < zn) z[za] = sLo[gl][e - (gl - 1) * own];
+    }
+}
+
+int fa_crt_fp_enabled = 1;
+extern "C" FA_API void fa_set_crt_variant(int v) { fa_crt_fp_enabled = v; }
+
+template <int NP>
+static void launch_crt_fp(const double* Q, uint64_t stride, const fa_crt_fp_consts& F,
+                          uint64_t zlen, int bits, uint64_t* z, uint64_t zn, uint64_t ngroups,
+                          uint8_t* flags, uint8_t* agg, uint64_t nblk, hipStream_t s)
+{
+    FA_LAUNCH("fa_k_crt_fp", fa_k_crt_fp<NP>, dim3((unsigned)nblk), dim3(256), 0, s, Q, stride, F,
+              zlen, bits, z, zn, ngroups, flags, agg);
+}
+
 template <int NP>
 static void launch_crt16(const double* Q, uint64_t stride, const fa_crt_consts& C, uint64_t zlen,
                          int bits, uint64_t* z, uint64_t zn, uint64_t ngroups, uint8_t* flags,
@@ -740,6 +890,23 @@ static int crt16(int np, const double* Q, uint64_t stride, const fa_crt_consts& 
                  int bits, uint64_t* z, uint64_t zn, uint64_t ngroups, uint8_t* flags, uint8_t* agg,
                  uint8_t* cin, uint64_t nblk, hipStream_t s)
 {
+    if (fa_crt_fp_enabled) {
+        fa_crt_fp_consts F;
+        fa_crt_fp_consts_init(&F, C);
+        switch (np) {
+        case 2: launch_crt_fp<2>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
+        case 3: launch_crt_fp<3>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
+        case 4: launch_crt_fp<4>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
+        case 5: launch_crt_fp<5>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
+        case 6: launch_crt_fp<6>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
+        case 7: launch_crt_fp<7>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
+        case 8: launch_crt_fp<8>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
+        default: return fa_set_error(FA_E_ARG, "crt: np %d", np);
+        }
+        FA_LAUNCH("fa_k_carry_fused", fa_k_carry_fused, dim3(1), dim3(1024), 0, s, flags, agg, z,
+                  zn, bits / 8, ngroups, nblk);
+        return FA_OK;
+    }
     switch (np) {
     case 2: launch_crt16<2>(Q, stride, C, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
     case 3: launch_crt16<3>(Q, stride, C, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;

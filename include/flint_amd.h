@@ -153,6 +153,9 @@ FA_API int fa_prof_read(char* names, double* ms, int64_t* counts, int max);
 FA_API void fa_set_fast_kernels(int on);
 /* column-pass tiles at 2^24 points: 1 = 8 columns (default), 0 = 4 columns */
 FA_API void fa_set_col_variant(int v);
+/* CRT of integer products: 1 = radix-2^24 digits on the FP64 lanes
+   (default), 0 = 64-bit limb arithmetic */
+FA_API void fa_set_crt_variant(int v);
 /* plain device-to-device copy kernel (nbytes multiple of 16), used to
    calibrate hardware byte counters */
 FA_API int fa_copy_device(void* dst, const void* src, uint64_t nbytes, void* stream);

@@ -164,3 +164,26 @@ def test_column_tile_variants_agree(an, bn):
     assert np.array_equal(z0, z1)
     q = (1 << 61) - 1
     assert (to_int(a) % q) * (to_int(b) % q) % q == to_int(z1) % q
+
+
+@pytest.mark.parametrize("an,bn,np_,bits", [(65536, 65536, 6, 136), (5000, 4000, 2, 40),
+                                           (30000, 30000, 3, 64), (1000000, 3, 4, 88),
+                                           (200000, 190000, 5, 112), (700000, 650000, 8, 184),
+                                           (300000, 300000, 7, 160), (1 << 20, 1 << 20, 6, 136)])
+def test_crt_variants_agree(an, bn, np_, bits):
+    """FP64-digit CRT and 64-bit-limb CRT give the same (exact) product for
+    every prime count"""
+    import flint_amd as fa
+    rng = np.random.default_rng(an + 7 * bn + np_)
+    a = rand_limbs(rng, an)
+    b = rand_limbs(rng, bn)
+    print("crt-variants", an, bn, np_, bits, flush=True)
+    try:
+        fa.set_crt_variant(0)
+        z0 = fa.mpn_mul_plan(a, b, np_, bits)
+    finally:
+        fa.set_crt_variant(1)
+    z1 = fa.mpn_mul_plan(a, b, np_, bits)
+    assert np.array_equal(z0, z1)
+    q = (1 << 61) - 1
+    assert (to_int(a) % q) * (to_int(b) % q) % q == to_int(z1) % q
