@@ -10,7 +10,9 @@
 //     (b << (D - 4j)) + i0 + k * 2^(D - 4j - 4),   k < 16,
 // so after the last stage thread t holds the 16 consecutive outputs
 // 16t .. 16t+15 (bit-reversed evaluation order).  LDS rows are padded by
-// one double per 16 so that all three access patterns are conflict-free.
+// one double per 16 so that all three access patterns are conflict-free;
+// the column passes interleave their columns in LDS (element (c, r) of a
+// tile at PADX(r) * columns + c) for the same reason.
 #include "fa_internal.h"
 #include "fa_ntt.h"
 #include "fa_pack.h"
@@ -160,7 +162,31 @@ FA_DEV void fa_inv4096_row(double* x, double* xch, const double* tab, double n, 
 }
 
 // row pass: both forward row transforms, pointwise product with the
-// normalizer, inverse row transform; in place in A.
+// normalizer m, inverse row transform; in place in arow.  Row r of the
+// two-level transform (root offset r at level 12).
+FA_DEV void fa_row_conv_tile(double* __restrict__ arow, const double* __restrict__ brow,
+                             uint64_t r, double n, double ninv, double m,
+                             const double* __restrict__ w2, const double* __restrict__ w2inv,
+                             double* xch, double* tab, double* rho)
+{
+    const int t = threadIdx.x;
+    double x[16], y[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        y[k] = brow[t + 256 * k];
+        x[k] = arow[t + 256 * k];
+    }
+    fa_build_row_table(tab, w2, r, rho, n, ninv);
+    fa_fwd4096_row(y, xch, tab, n, ninv);
+    fa_fwd4096_row(x, xch, tab, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = fa_mulmod(fa_mulmod(x[k], m, n, ninv), y[k], n, ninv);
+    fa_build_row_table(tab, w2inv, r, rho, n, ninv);
+    fa_inv4096_row(x, xch, tab, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 16; k++) arow[t + 256 * k] = x[k];
+}
+
 // blockIdx.x = row, blockIdx.y = prime
 __global__ void __launch_bounds__(256) fa_k_row_conv16(double* __restrict__ A,
                                                        const double* __restrict__ B,
@@ -170,27 +196,9 @@ __global__ void __launch_bounds__(256) fa_k_row_conv16(double* __restrict__ A,
     __shared__ double tab[4096];
     __shared__ double rho[16];
     const int i = blockIdx.y;
-    const double n = S.p[i], ninv = S.pinv[i];
     const uint64_t r = blockIdx.x;
-    const int t = threadIdx.x;
-    double* arow = A + (uint64_t)i * stride + (r << 12);
-    const double* brow = B + (uint64_t)i * stride + (r << 12);
-    double x[16], y[16];
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        y[k] = brow[t + 256 * k];
-        x[k] = arow[t + 256 * k];
-    }
-    fa_build_row_table(tab, S.w2[i], r, rho, n, ninv);
-    fa_fwd4096_row(y, xch, tab, n, ninv);
-    fa_fwd4096_row(x, xch, tab, n, ninv);
-    const double m = S.norm[i];
-#pragma unroll
-    for (int k = 0; k < 16; k++) x[k] = fa_mulmod(fa_mulmod(x[k], m, n, ninv), y[k], n, ninv);
-    fa_build_row_table(tab, S.w2inv[i], r, rho, n, ninv);
-    fa_inv4096_row(x, xch, tab, n, ninv);
-#pragma unroll
-    for (int k = 0; k < 16; k++) arow[t + 256 * k] = x[k];
+    fa_row_conv_tile(A + (uint64_t)i * stride + (r << 12), B + (uint64_t)i * stride + (r << 12), r,
+                     S.p[i], S.pinv[i], S.norm[i], S.w2[i], S.w2inv[i], xch, tab, rho);
 }
 
 // ---------------------------------------------------------------------------
@@ -316,32 +324,21 @@ FA_DEV unsigned fa_xcd_linear()
     return (bid & 7) * (nb >> 3) + (bid >> 3);
 }
 
-template <int D1>
-__global__ void __launch_bounds__(1024) fa_k_col_fwd16(fa_operand a, fa_operand b, int d2,
-                                                       double* __restrict__ A,
-                                                       double* __restrict__ B, uint64_t stride,
-                                                       fa_prime_set S, int np, int ngroups,
-                                                       int nops)
+// first D1 levels of column group g (CPW columns) of one residue array:
+// import, forward, store.  lds holds CPW * PLEN doubles.
+template <int D1, int KIND>
+FA_DEV void fa_col_fwd_tile(const fa_operand& op, double* __restrict__ dst, int d2, int g,
+                            double n, double ninv, const double* __restrict__ w2,
+                            const double* __restrict__ pow2, double* lds)
 {
     typedef col_geom<D1> G;
-    __shared__ double lds[G::LDS];
-    const unsigned L = fa_xcd_linear();
-    const int i = L % np;
-    const unsigned rest = L / np;
-    const int g = rest % ngroups;
-    const int z = rest / ngroups;
-    if (z >= nops) return;
-    const double n = S.p[i], ninv = S.pinv[i];
-    const fa_operand& op = z ? b : a;
-    double* dst = (z ? B : A) + (uint64_t)i * stride;
     const int c = threadIdx.x % G::CPW, tc = threadIdx.x / G::CPW;
     const uint64_t col = (uint64_t)g * G::CPW + c;
-    double* my = lds + (G::NSTAGE > 1 ? c * G::PLEN : 0);
-    const double* w2 = S.w2[i];
+    double* my = lds + c;   // column c of the interleaved LDS rows
     double x[16], tw[15];
 #pragma unroll
     for (int k = 0; k < G::NEL; k++)
-        x[k] = fa_operand_value(op, ((uint64_t)G::pos(0, tc, k) << d2) + col, S.pow2[i], n, ninv);
+        x[k] = fa_operand_value_k<KIND>(op, ((uint64_t)G::pos(0, tc, k) << d2) + col, pow2, n, ninv);
     if (D1 < 4) {
         fa_fwd_small_col<(D1 < 4 ? D1 : 1)>(x, w2, n, ninv);
     } else {
@@ -351,10 +348,10 @@ __global__ void __launch_bounds__(1024) fa_k_col_fwd16(fa_operand a, fa_operand 
                 // each position of a stage layout belongs to one thread: a
                 // single barrier separates stage j-1's stores from stage j's loads
 #pragma unroll
-                for (int k = 0; k < 16; k++) my[PADX(G::pos(j - 1, tc, k))] = x[k];
+                for (int k = 0; k < 16; k++) my[PADX(G::pos(j - 1, tc, k)) * G::CPW] = x[k];
                 __syncthreads();
 #pragma unroll
-                for (int k = 0; k < 16; k++) x[k] = my[PADX(G::pos(j, tc, k))];
+                for (int k = 0; k < 16; k++) x[k] = my[PADX(G::pos(j, tc, k)) * G::CPW];
             }
             if (G::R > 0 && j == 0) {
                 fa_fwd16_partial(x, G::R, w2, (uint64_t)G::block(0, tc), n, ninv);
@@ -370,22 +367,13 @@ __global__ void __launch_bounds__(1024) fa_k_col_fwd16(fa_operand a, fa_operand 
 }
 
 template <int D1>
-__global__ void __launch_bounds__(1024) fa_k_col_inv16(double* __restrict__ A, int d2,
-                                                       uint64_t stride, fa_prime_set S, int np,
-                                                       int ngroups)
+FA_DEV void fa_col_inv_tile(double* __restrict__ dst, int d2, int g, double n, double ninv,
+                            const double* __restrict__ wi, double* lds)
 {
     typedef col_geom<D1> G;
-    __shared__ double lds[G::LDS];
-    const unsigned L = fa_xcd_linear();
-    const int i = L % np;
-    const int g = L / np;
-    if (g >= ngroups) return;
-    const double n = S.p[i], ninv = S.pinv[i];
-    double* dst = A + (uint64_t)i * stride;
     const int c = threadIdx.x % G::CPW, tc = threadIdx.x / G::CPW;
     const uint64_t col = (uint64_t)g * G::CPW + c;
-    double* my = lds + (G::NSTAGE > 1 ? c * G::PLEN : 0);
-    const double* wi = S.w2inv[i];
+    double* my = lds + c;   // column c of the interleaved LDS rows
     double x[16], tw[15];
     const int jl = (D1 < 4) ? 0 : G::NSTAGE - 1;
 #pragma unroll
@@ -397,10 +385,10 @@ __global__ void __launch_bounds__(1024) fa_k_col_inv16(double* __restrict__ A, i
         for (int j = G::NSTAGE - 1; j >= 0; j--) {
             if (j < G::NSTAGE - 1) {
 #pragma unroll
-                for (int k = 0; k < 16; k++) my[PADX(G::pos(j + 1, tc, k))] = x[k];
+                for (int k = 0; k < 16; k++) my[PADX(G::pos(j + 1, tc, k)) * G::CPW] = x[k];
                 __syncthreads();
 #pragma unroll
-                for (int k = 0; k < 16; k++) x[k] = my[PADX(G::pos(j, tc, k))];
+                for (int k = 0; k < 16; k++) x[k] = my[PADX(G::pos(j, tc, k)) * G::CPW];
             }
             if (G::R > 0 && j == 0) {
                 fa_inv16_partial(x, G::R, wi, (uint64_t)G::block(0, tc), n, ninv);
@@ -412,6 +400,37 @@ __global__ void __launch_bounds__(1024) fa_k_col_inv16(double* __restrict__ A, i
     }
 #pragma unroll
     for (int k = 0; k < G::NEL; k++) dst[((uint64_t)G::pos(0, tc, k) << d2) + col] = x[k];
+}
+
+template <int D1, int KIND>
+__global__ void __launch_bounds__(1024) fa_k_col_fwd16(fa_operand a, fa_operand b, int d2,
+                                                       double* __restrict__ A,
+                                                       double* __restrict__ B, uint64_t stride,
+                                                       fa_prime_set S, int np, int ngroups,
+                                                       int nops)
+{
+    __shared__ double lds[col_geom<D1>::LDS];
+    const unsigned L = fa_xcd_linear();
+    const int i = L % np;
+    const unsigned rest = L / np;
+    const int g = rest % ngroups;
+    const int z = rest / ngroups;
+    if (z >= nops) return;
+    fa_col_fwd_tile<D1, KIND>(z ? b : a, (z ? B : A) + (uint64_t)i * stride, d2, g, S.p[i],
+                              S.pinv[i], S.w2[i], S.pow2[i], lds);
+}
+
+template <int D1>
+__global__ void __launch_bounds__(1024) fa_k_col_inv16(double* __restrict__ A, int d2,
+                                                       uint64_t stride, fa_prime_set S, int np,
+                                                       int ngroups)
+{
+    __shared__ double lds[col_geom<D1>::LDS];
+    const unsigned L = fa_xcd_linear();
+    const int i = L % np;
+    const int g = L / np;
+    if (g >= ngroups) return;
+    fa_col_inv_tile<D1>(A + (uint64_t)i * stride, d2, g, S.p[i], S.pinv[i], S.w2inv[i], lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -426,42 +445,31 @@ __global__ void __launch_bounds__(1024) fa_k_col_inv16(double* __restrict__ A, i
 //    tc = 128H + u (blocks 8H .. 8H+7 of level 4);
 //  - stage 1 -> 2: stage-1 group u + 128H and stage-2 group u + 128H both
 //    lie in half H.
-// Column rows are 2184 doubles apart (8 mod 32), so the 8 columns of one
-// access fall on distinct banks in pairs.
+// LDS rows interleave the 8 columns (element (c, r) at 8 PADX(r) + c), so
+// the lanes of a wave touch consecutive doubles when their rows are
+// consecutive, and the pad of one row per 16 separates the 16-row strides
+// of the last stage.
 // ---------------------------------------------------------------------------
-#define FA_C8_PLH 2184
-#define FA_C8_LDS (8 * FA_C8_PLH)
+#define FA_C8_LDS (8 * 2176)
 
-FA_DEV int fa_c8_addr(int c, int row) { return c * FA_C8_PLH + PADX(row & 2047); }
+FA_DEV int fa_c8_addr(int c, int row) { return 8 * PADX(row & 2047) + c; }
 
 // stage-1 row of group tc, element k
 FA_DEV int fa_c8_row1(int tc, int k) { return ((tc >> 4) << 8) + (tc & 15) + 16 * k; }
 
-__global__ void __launch_bounds__(1024) fa_k_col_fwd16w(fa_operand a, fa_operand b, int d2,
-                                                        double* __restrict__ A,
-                                                        double* __restrict__ B, uint64_t stride,
-                                                        fa_prime_set S, int np, int ngroups,
-                                                        int nops)
+template <int KIND>
+FA_DEV void fa_col_fwd8_tile(const fa_operand& op, double* __restrict__ dst, int d2, int g,
+                             double n, double ninv, const double* __restrict__ w2,
+                             const double* __restrict__ pow2, double* lds)
 {
-    __shared__ double lds[FA_C8_LDS];
-    const unsigned L = fa_xcd_linear();
-    const int i = L % np;
-    const unsigned rest = L / np;
-    const int g = rest % ngroups;
-    const int z = rest / ngroups;
-    if (z >= nops) return;
-    const double n = S.p[i], ninv = S.pinv[i];
-    const fa_operand& op = z ? b : a;
-    double* dst = (z ? B : A) + (uint64_t)i * stride;
     const int c = threadIdx.x & 7, u = threadIdx.x >> 3;
     const uint64_t col = (uint64_t)g * 8 + c;
-    const double* w2 = S.w2[i];
     double x0[16], x1[16], tw[15];
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-        x0[k] = fa_operand_value(op, ((uint64_t)(u + 256 * k) << d2) + col, S.pow2[i], n, ninv);
-        x1[k] = fa_operand_value(op, ((uint64_t)(u + 128 + 256 * k) << d2) + col, S.pow2[i], n,
-                                 ninv);
+        x0[k] = fa_operand_value_k<KIND>(op, ((uint64_t)(u + 256 * k) << d2) + col, pow2, n, ninv);
+        x1[k] = fa_operand_value_k<KIND>(op, ((uint64_t)(u + 128 + 256 * k) << d2) + col, pow2, n,
+                                         ninv);
     }
     // stage 0: levels 0..3, block 0
     fa_tw16(w2, 0, tw);
@@ -517,20 +525,11 @@ __global__ void __launch_bounds__(1024) fa_k_col_fwd16w(fa_operand a, fa_operand
     }
 }
 
-__global__ void __launch_bounds__(1024) fa_k_col_inv16w(double* __restrict__ A, int d2,
-                                                        uint64_t stride, fa_prime_set S, int np,
-                                                        int ngroups)
+FA_DEV void fa_col_inv8_tile(double* __restrict__ dst, int d2, int g, double n, double ninv,
+                             const double* __restrict__ wi, double* lds)
 {
-    __shared__ double lds[FA_C8_LDS];
-    const unsigned L = fa_xcd_linear();
-    const int i = L % np;
-    const int g = L / np;
-    if (g >= ngroups) return;
-    const double n = S.p[i], ninv = S.pinv[i];
-    double* dst = A + (uint64_t)i * stride;
     const int c = threadIdx.x & 7, u = threadIdx.x >> 3;
     const uint64_t col = (uint64_t)g * 8 + c;
-    const double* wi = S.w2inv[i];
     double x0[16], x1[16], tw[15];
 #pragma unroll
     for (int k = 0; k < 16; k++) {
@@ -589,6 +588,242 @@ __global__ void __launch_bounds__(1024) fa_k_col_inv16w(double* __restrict__ A, 
     }
 }
 
+template <int KIND>
+__global__ void __launch_bounds__(1024) fa_k_col_fwd16w(fa_operand a, fa_operand b, int d2,
+                                                        double* __restrict__ A,
+                                                        double* __restrict__ B, uint64_t stride,
+                                                        fa_prime_set S, int np, int ngroups,
+                                                        int nops)
+{
+    __shared__ double lds[FA_C8_LDS];
+    const unsigned L = fa_xcd_linear();
+    const int i = L % np;
+    const unsigned rest = L / np;
+    const int g = rest % ngroups;
+    const int z = rest / ngroups;
+    if (z >= nops) return;
+    fa_col_fwd8_tile<KIND>(z ? b : a, (z ? B : A) + (uint64_t)i * stride, d2, g, S.p[i], S.pinv[i],
+                           S.w2[i], S.pow2[i], lds);
+}
+
+__global__ void __launch_bounds__(1024) fa_k_col_inv16w(double* __restrict__ A, int d2,
+                                                        uint64_t stride, fa_prime_set S, int np,
+                                                        int ngroups)
+{
+    __shared__ double lds[FA_C8_LDS];
+    const unsigned L = fa_xcd_linear();
+    const int i = L % np;
+    const int g = L / np;
+    if (g >= ngroups) return;
+    fa_col_inv8_tile(A + (uint64_t)i * stride, d2, g, S.p[i], S.pinv[i], S.w2inv[i], lds);
+}
+
+// ---------------------------------------------------------------------------
+// batched kernels: the same tiles, each workgroup taking one work item
+// ---------------------------------------------------------------------------
+struct fa_witem {
+    unsigned job;
+    int prime, op, tile;
+};
+
+FA_DEV fa_witem fa_decode(uint64_t w)
+{
+    fa_witem r;
+    r.job = (unsigned)(w & 0xffffffffu);
+    r.prime = (int)((w >> 32) & 15);
+    r.op = (int)((w >> 36) & 1);
+    r.tile = (int)(w >> 40);
+    return r;
+}
+
+FA_DEV fa_operand fa_job_operand(const fa_bjob& j, int z)
+{
+    fa_operand o;
+    o.kind = j.kind;
+    o.src = z ? j.b : j.a;
+    o.n = z ? j.bn : j.an;
+    o.bits = j.bits;
+    o.width = z ? j.bw : j.aw;
+    return o;
+}
+
+template <int D1, int KIND>
+__global__ void __launch_bounds__(1024) fa_kb_col_fwd(const fa_bjob* __restrict__ J,
+                                                      const uint64_t* __restrict__ work,
+                                                      unsigned nwork, fa_prime_set S)
+{
+    __shared__ double lds[(D1 == 12) ? FA_C8_LDS : col_geom<D1>::LDS];
+    const unsigned L = fa_xcd_linear();
+    if (L >= nwork) return;
+    const fa_witem w = fa_decode(work[L]);
+    const fa_bjob& j = J[w.job];
+    const int pi = j.offset + w.prime;
+    const fa_operand op = fa_job_operand(j, w.op);
+    double* dst = (w.op ? j.B : j.A) + (uint64_t)w.prime * j.stride;
+    if constexpr (D1 == 12)
+        fa_col_fwd8_tile<KIND>(op, dst, FA_ROW_LG, w.tile, S.p[pi], S.pinv[pi], S.w2[pi],
+                               S.pow2[pi], lds);
+    else
+        fa_col_fwd_tile<D1, KIND>(op, dst, FA_ROW_LG, w.tile, S.p[pi], S.pinv[pi], S.w2[pi],
+                                  S.pow2[pi], lds);
+}
+
+template <int D1>
+__global__ void __launch_bounds__(1024) fa_kb_col_inv(const fa_bjob* __restrict__ J,
+                                                      const uint64_t* __restrict__ work,
+                                                      unsigned nwork, fa_prime_set S)
+{
+    __shared__ double lds[(D1 == 12) ? FA_C8_LDS : col_geom<D1>::LDS];
+    const unsigned L = fa_xcd_linear();
+    if (L >= nwork) return;
+    const fa_witem w = fa_decode(work[L]);
+    const fa_bjob& j = J[w.job];
+    const int pi = j.offset + w.prime;
+    double* dst = j.A + (uint64_t)w.prime * j.stride;
+    if constexpr (D1 == 12)
+        fa_col_inv8_tile(dst, FA_ROW_LG, w.tile, S.p[pi], S.pinv[pi], S.w2inv[pi], lds);
+    else
+        fa_col_inv_tile<D1>(dst, FA_ROW_LG, w.tile, S.p[pi], S.pinv[pi], S.w2inv[pi], lds);
+}
+
+__global__ void __launch_bounds__(256) fa_kb_row_conv(const fa_bjob* __restrict__ J,
+                                                      const uint64_t* __restrict__ work,
+                                                      unsigned nwork, fa_prime_set S)
+{
+    __shared__ double xch[4352];
+    __shared__ double tab[4096];
+    __shared__ double rho[16];
+    const unsigned L = fa_xcd_linear();
+    if (L >= nwork) return;
+    const fa_witem w = fa_decode(work[L]);
+    const fa_bjob& j = J[w.job];
+    const int pi = j.offset + w.prime;
+    const uint64_t r = (uint64_t)w.tile;
+    const uint64_t off = (uint64_t)w.prime * j.stride + (r << 12);
+    fa_row_conv_tile(j.A + off, j.B + off, r, S.p[pi], S.pinv[pi], j.norm[w.prime], S.w2[pi],
+                     S.w2inv[pi], xch, tab, rho);
+}
+
+// whole product of one prime in LDS (depth <= 12): import, both forward
+// transforms, pointwise, inverse; the result goes to row `prime` of A
+template <int KIND>
+__global__ void __launch_bounds__(256) fa_kb_small_conv(const fa_bjob* __restrict__ J,
+                                                        const uint64_t* __restrict__ work,
+                                                        unsigned nwork, fa_prime_set S)
+{
+    extern __shared__ double lds[];
+    const unsigned L = fa_xcd_linear();
+    if (L >= nwork) return;
+    const fa_witem w = fa_decode(work[L]);
+    const fa_bjob& j = J[w.job];
+    const int pi = j.offset + w.prime;
+    const double n = S.p[pi], ninv = S.pinv[pi];
+    const int depth = j.depth;
+    const int N = 1 << depth;
+    const fa_operand oa = fa_job_operand(j, 0), ob = fa_job_operand(j, 1);
+    double* sa = lds;
+    double* sb = lds + N;
+    for (int x = threadIdx.x; x < N; x += blockDim.x) {
+        sa[x] = fa_operand_value_k<KIND>(oa, (uint64_t)x, S.pow2[pi], n, ninv);
+        sb[x] = fa_operand_value_k<KIND>(ob, (uint64_t)x, S.pow2[pi], n, ninv);
+    }
+    __syncthreads();
+    fa_lds_fwd(sa, depth, 0, 0, S.w2[pi], n, ninv);
+    fa_lds_fwd(sb, depth, 0, 0, S.w2[pi], n, ninv);
+    const double m = j.norm[w.prime];
+    for (int x = threadIdx.x; x < N; x += blockDim.x)
+        sa[x] = fa_mulmod(fa_mulmod(sa[x], m, n, ninv), sb[x], n, ninv);
+    __syncthreads();
+    fa_lds_inv(sa, depth, 0, 0, S.w2inv[pi], n, ninv);
+    double* o = j.A + (uint64_t)w.prime * j.stride;
+    for (int x = threadIdx.x; x < N; x += blockDim.x) o[x] = sa[x];
+}
+
+static unsigned grid8(unsigned n) { return (n + 7) & ~7u; }
+
+int fa_batch_col_groups(int depth)
+{
+    const int d1 = depth - FA_ROW_LG;
+    if (d1 == 12) return 4096 / 8;
+    if (d1 >= 4) return 4096 / (1024 >> (d1 - 4));
+    return 4096 / 1024;
+}
+
+int fa_batch_small_bin(int depth, int kind, const fa_bjob* J, const uint64_t* work,
+                       unsigned nwork, const fa_prime_set& S, hipStream_t s)
+{
+    if (!nwork) return FA_OK;
+    const size_t lds = 2 * ((size_t)1 << depth) * sizeof(double);
+    switch (kind) {
+    case FA_PACK_MPN8:
+        FA_LAUNCH("fa_kb_small_conv", fa_kb_small_conv<FA_PACK_MPN8>, dim3(grid8(nwork)),
+                  dim3(256), lds, s, J, work, nwork, S);
+        break;
+    case FA_PACK_U64:
+        FA_LAUNCH("fa_kb_small_conv", fa_kb_small_conv<FA_PACK_U64>, dim3(grid8(nwork)),
+                  dim3(256), lds, s, J, work, nwork, S);
+        break;
+    case FA_PACK_S64W:
+        FA_LAUNCH("fa_kb_small_conv", fa_kb_small_conv<FA_PACK_S64W>, dim3(grid8(nwork)),
+                  dim3(256), lds, s, J, work, nwork, S);
+        break;
+    default:
+        return fa_set_error(FA_E_ARG, "batch: import kind %d", kind);
+    }
+    return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "batch launch");
+}
+
+template <int D1, int KIND>
+static void batch_bin(const fa_bjob* J, const uint64_t* wf, unsigned nf, const uint64_t* wr,
+                      unsigned nr, const uint64_t* wi, unsigned ni, const fa_prime_set& S,
+                      hipStream_t s)
+{
+    if (nf)
+        FA_LAUNCH("fa_kb_col_fwd", (fa_kb_col_fwd<D1, KIND>), dim3(grid8(nf)), dim3(1024), 0, s,
+                  J, wf, nf, S);
+    if (nr)
+        FA_LAUNCH("fa_kb_row_conv", fa_kb_row_conv, dim3(grid8(nr)), dim3(256), 0, s, J, wr, nr,
+                  S);
+    if (ni)
+        FA_LAUNCH("fa_kb_col_inv", fa_kb_col_inv<D1>, dim3(grid8(ni)), dim3(1024), 0, s, J, wi,
+                  ni, S);
+}
+
+template <int D1>
+static int batch_bin_kind(int kind, const fa_bjob* J, const uint64_t* wf, unsigned nf,
+                          const uint64_t* wr, unsigned nr, const uint64_t* wi, unsigned ni,
+                          const fa_prime_set& S, hipStream_t s)
+{
+    switch (kind) {
+    case FA_PACK_MPN8: batch_bin<D1, FA_PACK_MPN8>(J, wf, nf, wr, nr, wi, ni, S, s); break;
+    case FA_PACK_U64: batch_bin<D1, FA_PACK_U64>(J, wf, nf, wr, nr, wi, ni, S, s); break;
+    case FA_PACK_S64W: batch_bin<D1, FA_PACK_S64W>(J, wf, nf, wr, nr, wi, ni, S, s); break;
+    default: return fa_set_error(FA_E_ARG, "batch: import kind %d", kind);
+    }
+    return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "batch launch");
+}
+
+int fa_batch_transform_bin(int depth, int kind, const fa_bjob* J, const uint64_t* wf,
+                           unsigned nf, const uint64_t* wr, unsigned nr, const uint64_t* wi,
+                           unsigned ni, const fa_prime_set& S, hipStream_t s)
+{
+    switch (depth - FA_ROW_LG) {
+    case 1: return batch_bin_kind<1>(kind, J, wf, nf, wr, nr, wi, ni, S, s);
+    case 2: return batch_bin_kind<2>(kind, J, wf, nf, wr, nr, wi, ni, S, s);
+    case 3: return batch_bin_kind<3>(kind, J, wf, nf, wr, nr, wi, ni, S, s);
+    case 4: return batch_bin_kind<4>(kind, J, wf, nf, wr, nr, wi, ni, S, s);
+    case 5: return batch_bin_kind<5>(kind, J, wf, nf, wr, nr, wi, ni, S, s);
+    case 6: return batch_bin_kind<6>(kind, J, wf, nf, wr, nr, wi, ni, S, s);
+    case 7: return batch_bin_kind<7>(kind, J, wf, nf, wr, nr, wi, ni, S, s);
+    case 8: return batch_bin_kind<8>(kind, J, wf, nf, wr, nr, wi, ni, S, s);
+    case 9: return batch_bin_kind<9>(kind, J, wf, nf, wr, nr, wi, ni, S, s);
+    case 10: return batch_bin_kind<10>(kind, J, wf, nf, wr, nr, wi, ni, S, s);
+    case 11: return batch_bin_kind<11>(kind, J, wf, nf, wr, nr, wi, ni, S, s);
+    case 12: return batch_bin_kind<12>(kind, J, wf, nf, wr, nr, wi, ni, S, s);
+    default: return fa_set_error(FA_E_ARG, "batch: depth %d outside 13..24", depth);
+    }
+}
+
 int fa_col_variant = 1;   // 1: 8-column tiles at d1 = 12, 0: 4-column tiles
 extern "C" FA_API void fa_set_col_variant(int v) { fa_col_variant = v; }
 
@@ -599,7 +834,7 @@ extern "C" FA_API void fa_set_col_variant(int v) { fa_col_variant = v; }
 
 extern int fa_col_variant;
 
-template <int D1>
+template <int D1, int KIND>
 static int launch_fast(const fa_operand& oa, const fa_operand& ob, int nops, int np, double* A,
                        double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s)
 {
@@ -610,11 +845,11 @@ static int launch_fast(const fa_operand& oa, const fa_operand& ob, int nops, int
     unsigned total = (unsigned)(np * ngroups * nops);
     unsigned grid = (total + 7) & ~7u;
     if (wide)
-        FA_LAUNCH("fa_k_col_fwd16w", fa_k_col_fwd16w, dim3(grid), dim3(1024), 0, s, oa, ob, d2, A, B,
-                  stride, S, np, ngroups, nops);
+        FA_LAUNCH("fa_k_col_fwd16w", fa_k_col_fwd16w<KIND>, dim3(grid), dim3(1024), 0, s, oa, ob,
+                  d2, A, B, stride, S, np, ngroups, nops);
     else
-        FA_LAUNCH("fa_k_col_fwd16", fa_k_col_fwd16<D1>, dim3(grid), dim3(1024), 0, s, oa, ob, d2, A,
-                  B, stride, S, np, ngroups, nops);
+        FA_LAUNCH("fa_k_col_fwd16", (fa_k_col_fwd16<D1, KIND>), dim3(grid), dim3(1024), 0, s, oa, ob,
+                  d2, A, B, stride, S, np, ngroups, nops);
     FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16, dim3(1u << D1, np), dim3(256), 0, s, A, B, stride,
               S);
     unsigned total2 = (unsigned)(np * ngroups);
@@ -628,22 +863,34 @@ static int launch_fast(const fa_operand& oa, const fa_operand& ob, int nops, int
     return 1;
 }
 
+template <int D1>
+static int launch_fast_any(const fa_operand& oa, const fa_operand& ob, int np, double* A,
+                           double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s)
+{
+    switch (fa_pack_kind(oa, ob)) {
+    case FA_PACK_MPN8: return launch_fast<D1, FA_PACK_MPN8>(oa, ob, 2, np, A, B, stride, S, s);
+    case FA_PACK_U64: return launch_fast<D1, FA_PACK_U64>(oa, ob, 2, np, A, B, stride, S, s);
+    case FA_PACK_S64W: return launch_fast<D1, FA_PACK_S64W>(oa, ob, 2, np, A, B, stride, S, s);
+    default: return launch_fast<D1, FA_PACK_ANY>(oa, ob, 2, np, A, B, stride, S, s);
+    }
+}
+
 int fa_convolve_fast(int depth, const fa_operand& oa, const fa_operand& ob, int np, double* A,
                      double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s)
 {
     switch (depth - FA_ROW_LG) {
-    case 1: return launch_fast<1>(oa, ob, 2, np, A, B, stride, S, s);
-    case 2: return launch_fast<2>(oa, ob, 2, np, A, B, stride, S, s);
-    case 3: return launch_fast<3>(oa, ob, 2, np, A, B, stride, S, s);
-    case 4: return launch_fast<4>(oa, ob, 2, np, A, B, stride, S, s);
-    case 5: return launch_fast<5>(oa, ob, 2, np, A, B, stride, S, s);
-    case 6: return launch_fast<6>(oa, ob, 2, np, A, B, stride, S, s);
-    case 7: return launch_fast<7>(oa, ob, 2, np, A, B, stride, S, s);
-    case 8: return launch_fast<8>(oa, ob, 2, np, A, B, stride, S, s);
-    case 9: return launch_fast<9>(oa, ob, 2, np, A, B, stride, S, s);
-    case 10: return launch_fast<10>(oa, ob, 2, np, A, B, stride, S, s);
-    case 11: return launch_fast<11>(oa, ob, 2, np, A, B, stride, S, s);
-    case 12: return launch_fast<12>(oa, ob, 2, np, A, B, stride, S, s);
+    case 1: return launch_fast_any<1>(oa, ob, np, A, B, stride, S, s);
+    case 2: return launch_fast_any<2>(oa, ob, np, A, B, stride, S, s);
+    case 3: return launch_fast_any<3>(oa, ob, np, A, B, stride, S, s);
+    case 4: return launch_fast_any<4>(oa, ob, np, A, B, stride, S, s);
+    case 5: return launch_fast_any<5>(oa, ob, np, A, B, stride, S, s);
+    case 6: return launch_fast_any<6>(oa, ob, np, A, B, stride, S, s);
+    case 7: return launch_fast_any<7>(oa, ob, np, A, B, stride, S, s);
+    case 8: return launch_fast_any<8>(oa, ob, np, A, B, stride, S, s);
+    case 9: return launch_fast_any<9>(oa, ob, np, A, B, stride, S, s);
+    case 10: return launch_fast_any<10>(oa, ob, np, A, B, stride, S, s);
+    case 11: return launch_fast_any<11>(oa, ob, np, A, B, stride, S, s);
+    case 12: return launch_fast_any<12>(oa, ob, np, A, B, stride, S, s);
     default: return 0;
     }
 }

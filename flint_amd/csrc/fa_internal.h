@@ -102,10 +102,49 @@ struct fa_pack_desc {
     int width;                 // limbs per coefficient (signed coefficients)
 };
 enum { FA_PACK_MPN = 0, FA_PACK_U64 = 1, FA_PACK_S64W = 2 };
+// kernel specializations of the import: a limb array cut at byte
+// boundaries (bits = 8 own, own <= 25), and any kind chosen at run time
+enum { FA_PACK_MPN8 = 3, FA_PACK_ANY = 4 };
 
 int fa_convolve(fa_thread_state* st, const fa_tables* T, int np, int offset, int depth,
                 const fa_pack_desc* a, const fa_pack_desc* b, double* A, double* B,
                 hipStream_t stream);
+}
+
+// ---------------------------------------------------------------------------
+// batched products: one device-side job record per product; the transform
+// kernels of a bin (equal depth and import kind) walk a work list whose
+// 64-bit items are  job | prime << 32 | operand << 36 | tile << 40
+// (tile = column group or row)
+// ---------------------------------------------------------------------------
+struct fa_bjob {
+    const uint64_t* a;         // operands (device)
+    const uint64_t* b;
+    uint64_t an, bn;           // limbs (mpn) or coefficients
+    int aw, bw;                // limbs per coefficient (signed coefficients)
+    int kind;                  // FA_PACK_*
+    int bits;                  // chunk bits (mpn)
+    double* A;                 // residue rows, np x stride each
+    double* B;
+    uint64_t stride;           // 2^depth
+    int np, offset, depth, pad;
+    double norm[FA_NPRIMES];   // pointwise normalizer of prime offset + i
+};
+
+#define FA_WORK(job, prime, op, tile)                                                     \
+    ((uint64_t)(job) | ((uint64_t)(prime) << 32) | ((uint64_t)(op) << 36) |              \
+     ((uint64_t)(tile) << 40))
+
+extern "C++" {
+// launches of one bin; return FA_OK or an error code
+int fa_batch_small_bin(int depth, int kind, const fa_bjob* J, const uint64_t* work,
+                       unsigned nwork, const fa_prime_set& S, hipStream_t s);
+int fa_batch_transform_bin(int depth, int kind, const fa_bjob* J, const uint64_t* wfwd,
+                           unsigned nfwd, const uint64_t* wrow, unsigned nrow,
+                           const uint64_t* winv, unsigned ninv, const fa_prime_set& S,
+                           hipStream_t s);
+// column groups per residue array in the batched column passes at depth D
+int fa_batch_col_groups(int depth);
 }
 
 // ---------------------------------------------------------------------------

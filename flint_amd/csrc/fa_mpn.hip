@@ -733,16 +733,25 @@ __global__ void __launch_bounds__(1024) fa_k_carry_fused(const uint8_t* __restri
 // workgroup of 256 threads plus one halo group in front (its overhang
 // reaches the first group).
 //   1. thread t: coefficient 8 g0 + t -> NL limbs, shifted left by
-//      (bits * (t mod 8)) mod 64 into NL + 1 limbs in LDS;
-//   2. thread per (group, limb p): sum of the <= 5 shifted coefficients
-//      overlapping p, as a 64-bit value and a small carry count;
-//   3. thread per group: carries through the group's TL limbs;
-//   4. thread per group >= 1: adds the previous group's overhang (limbs
-//      own .. TL-1, which fit in own limbs) into its own limbs, records the
-//      (generate, propagate) flag of the carry into the next group;
-//   5. coalesced store of the owned limbs.
+//      (bits * (t mod 8)) mod 64 into NLS = NL + 1 limbs in LDS;
+//   2. lanes (group gl, limb q), 32 lanes per group: the sum of the <= 8
+//      shifted coefficients overlapping limb q of the group, then the
+//      group's carries: limb q gets the carry count of limb q-1, and the
+//      single-bit overflows ripple through all-ones limbs, resolved at once
+//      from the wave's ballot masks (generate G, propagate P: the carries
+//      in are (G + (G|P)) ^ (G|P) ^ G);
+//   3. lanes (group gl >= 1, limb q < own): owned limb q plus limb own + q
+//      of the previous group (its overhang, which fits in own limbs), the
+//      same mask carry resolution, the store, and the group's flag
+//      (generate: carry out of the owned limbs, propagate: all ones).
 // ------------------------------------------------------------------------
 #define FA_CRTFP_TL 32   // max limbs of a group's sum: 7*200/64 + 8 + 1
+
+FA_DEV uint32_t fa_seg_ballot(bool pred)
+{
+    const unsigned long long m = __ballot(pred);
+    return (uint32_t)(m >> (threadIdx.x & 32));
+}
 
 template <int NP>
 __global__ void __launch_bounds__(256) fa_k_crt_fp(const double* __restrict__ Q, uint64_t stride,
@@ -754,12 +763,12 @@ __global__ void __launch_bounds__(256) fa_k_crt_fp(const double* __restrict__ Q,
     constexpr int NL = fa_crt_fp_dims<NP>::NL;
     constexpr int NLS = NL + 1;
     __shared__ uint64_t sC[256 * NLS];
-    __shared__ uint64_t sLo[FA_CRT16_GROUPS][FA_CRTFP_TL + 1];
-    __shared__ uint8_t sHi[FA_CRT16_GROUPS][FA_CRTFP_TL];
+    __shared__ uint64_t sR[FA_CRT16_GROUPS][FA_CRTFP_TL];
     __shared__ uint8_t sF[FA_CRT16_GROUPS];
     const int t = threadIdx.x;
     const int own = bits >> 3;
     const int TL = ((7 * bits) >> 6) + NLS;
+    const int ovh = TL - own;
     const int64_t g0 = (int64_t)blockIdx.x * (FA_CRT16_GROUPS - 1) - 1;
 
     {   // 1. the coefficient of thread t, shifted to its bit offset in the group
@@ -781,66 +790,77 @@ __global__ void __launch_bounds__(256) fa_k_crt_fp(const double* __restrict__ Q,
         }
         o[NL] = sh ? (prev >> (64 - sh)) : 0;
     }
+
+    // the coefficients k overlapping limb q: o_k <= q < o_k + NLS with
+    // o_k = (bits k) >> 6, k in [kmin, kmax] (at most 8 of them)
+    const int q = t & 31, seg = t >> 5;
+    const bool qin = q < TL;
+    int kmax = (64 * q + 63) / bits;
+    if (kmax > 7) kmax = 7;
+    int kmin = 0;
+    if (q >= NLS) kmin = (64 * (q - NLS + 1) + bits - 1) / bits;
+    int offs[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int k = kmin + e;
+        offs[e] = k * NLS + q - ((bits * k) >> 6);
+    }
+    const int nk = qin ? kmax - kmin + 1 : 0;
     __syncthreads();
 
-    // 2. limb p of group gl: coefficients k with o_k <= p < o_k + NLS,
-    //    o_k = (bits k) >> 6
-    for (int e = t; e < FA_CRT16_GROUPS * TL; e += 256) {
-        const int gl = e / TL, p = e - gl * TL;
-        int kmax = (64 * p + 63) / bits;
-        if (kmax > 7) kmax = 7;
-        int kmin = 0;
-        if (p >= NLS) kmin = (64 * (p - NLS + 1) + bits - 1) / bits;
+    // 2. group sums and the carries inside each group
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+        const int gl = seg + 8 * it;
+        const uint64_t* base = sC + gl * 8 * NLS;
         uint64_t lo = 0;
-        unsigned hi = 0;
-        for (int k = kmin; k <= kmax; k++) {
-            const int idx = p - ((bits * k) >> 6);
-            if (idx < 0 || idx >= NLS) continue;
-            const uint64_t v = sC[(gl * 8 + k) * NLS + idx];
-            lo += v;
-            hi += (lo < v);
-        }
-        sLo[gl][p] = lo;
-        sHi[gl][p] = (uint8_t)hi;
-    }
-    __syncthreads();
-
-    // 3. carries inside each group
-    if (t < FA_CRT16_GROUPS) {
-        uint64_t c = 0;
-        for (int p = 0; p < TL; p++) {
-            const uint64_t v = sLo[t][p] + c;
-            c = (uint64_t)sHi[t][p] + (v < c);
-            sLo[t][p] = v;
-        }
-        sLo[t][TL] = c;   // zero: the group's sum fits in TL limbs
-    }
-    __syncthreads();
-
-    // 4. overhang of the previous group into the owned limbs
-    if (t >= 1 && t < FA_CRT16_GROUPS) {
-        const int64_t g = g0 + t;
-        uint8_t f = 2;   // identity of the carry operator: propagate, no generate
-        if (g >= 0 && (uint64_t)g < ngroups) {
-            const int ovh = TL - own;
-            uint64_t carry = 0;
-            int prop = 1;
-            for (int l = 0; l < own; l++) {
-                if ((uint64_t)g * own + l >= zn) { prop = 0; break; }
-                const uint64_t add = (l < ovh) ? sLo[t - 1][own + l] : 0;
-                uint64_t v = sLo[t][l];
-                uint64_t tt = v + add;
-                uint64_t c1 = (tt < add);
-                tt += carry;
-                c1 |= (tt < carry);
-                carry = c1;
-                sLo[t][l] = tt;
-                if (tt != ~UINT64_C(0)) prop = 0;
+        uint32_t hi = 0;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            if (e < nk) {
+                const uint64_t v = base[offs[e]];
+                lo += v;
+                hi += (lo < v);
             }
-            f = (uint8_t)(carry | (prop << 1));
-            flags[g] = f;
         }
-        sF[t] = f;
+        uint32_t hprev = __shfl_up(hi, 1, 32);
+        if (q == 0) hprev = 0;
+        const uint64_t v = lo + hprev;
+        const uint32_t G = fa_seg_ballot(qin && v < lo);
+        const uint32_t P = fa_seg_ballot(qin && v == ~UINT64_C(0));
+        const uint64_t X = (uint64_t)(G | P);
+        const uint64_t cin = ((uint64_t)G + X) ^ X ^ (uint64_t)G;
+        if (qin) sR[gl][q] = v + ((cin >> q) & 1);
+    }
+    __syncthreads();
+
+    // 3. the previous group's overhang into the owned limbs, store, flags
+#pragma unroll
+    for (int it = 0; it < 4; it++) {
+        const int gl = seg + 8 * it;
+        const int64_t g = g0 + gl;
+        const bool valid = gl >= 1 && g >= 0 && (uint64_t)g < ngroups;
+        const uint64_t zi = (uint64_t)g * own + q;
+        const bool lin = valid && q < own && zi < zn;
+        uint64_t S = 0, add = 0;
+        if (lin) {
+            add = (q < ovh) ? sR[gl - 1][own + q] : 0;
+            S = sR[gl][q] + add;
+        }
+        const uint32_t G = fa_seg_ballot(lin && S < add);
+        const uint32_t P = fa_seg_ballot(lin && S == ~UINT64_C(0));
+        const uint64_t X = (uint64_t)(G | P);
+        const uint64_t cin = ((uint64_t)G + X) ^ X ^ (uint64_t)G;
+        if (lin) z[zi] = S + ((cin >> q) & 1);
+        if (q == 0 && gl >= 1) {
+            uint8_t f = 2;   // identity of the carry operator: propagate, no generate
+            if (valid) {
+                const uint32_t full = (own >= 32) ? 0xffffffffu : ((1u << own) - 1);
+                f = (uint8_t)(((cin >> own) & 1) | ((P == full) ? 2 : 0));
+                flags[g] = f;
+            }
+            sF[gl] = f;
+        }
     }
     __syncthreads();
     if (t == 0) {
@@ -854,14 +874,6 @@ __global__ void __launch_bounds__(256) fa_k_crt_fp(const double* __restrict__ Q,
             any = (uint8_t)(any | (ft & 1));
         }
         agg[blockIdx.x] = (uint8_t)(gg | (pp << 1) | (any << 2));
-    }
-    // 5. owned limbs of groups 1..31, contiguous in z
-    const uint64_t base = (uint64_t)(g0 + 1) * own;
-    const int nl = (FA_CRT16_GROUPS - 1) * own;
-    for (int e = t; e < nl; e += 256) {
-        const uint64_t za = base + e;
-        const int gl = 1 + e / own;
-        if (za < zn) z[za] = sLo[gl][e - (gl - 1) * own];
     }
 }
 

@@ -65,6 +65,62 @@ FA_DEV double fa_chunk_residue(const uint64_t* __restrict__ limbs, uint64_t nlim
     return fa_reduce_pm1n(acc, n, ninv);
 }
 
+// the same for bits = 8 own (own <= 25 bytes per chunk): the chunk's bytes
+// [own j, own j + own) come from two 16-byte loads at the enclosing 4-byte
+// boundary, realigned with v_alignbyte, masked to own bytes and cut into
+// 48-bit pieces of 32-bit words.  The result is in (-2n, 2.25n) when
+// own <= 18 (at most two products) and in [-n/2, n/2] otherwise -- inside
+// the (-3n, 3n) input range of the forward transform.  The last chunks,
+// whose 32-byte window reaches past the array, read it word by word.
+typedef uint32_t fa_u32x4 __attribute__((ext_vector_type(4)));
+typedef fa_u32x4 fa_u32x4_a4 __attribute__((aligned(4)));
+
+FA_DEV double fa_chunk_residue8(const uint64_t* __restrict__ limbs, uint64_t nlimbs, int own,
+                                uint64_t j, const double* __restrict__ pow2, double n,
+                                double ninv)
+{
+    const uint64_t B = j * (uint64_t)own;
+    const uint64_t nbytes = nlimbs * 8;
+    if (B >= nbytes) return 0.0;
+    const uint64_t B4 = B & ~UINT64_C(3);
+    uint32_t w[8];
+    if (B4 + 32 <= nbytes) {
+        const fa_u32x4_a4* q = (const fa_u32x4_a4*)((const char*)limbs + B4);
+        const fa_u32x4 lo4 = q[0], hi4 = q[1];
+        w[0] = lo4.x; w[1] = lo4.y; w[2] = lo4.z; w[3] = lo4.w;
+        w[4] = hi4.x; w[5] = hi4.y; w[6] = hi4.z; w[7] = hi4.w;
+    } else {
+        const uint32_t* q = (const uint32_t*)((const char*)limbs + B4);
+#pragma unroll
+        for (int m = 0; m < 8; m++) w[m] = (B4 + 4 * m < nbytes) ? q[m] : 0u;
+    }
+    const uint32_t s = (uint32_t)(B & 3);
+    uint32_t c[7];
+#pragma unroll
+    for (int m = 0; m < 7; m++) {
+        c[m] = __builtin_amdgcn_alignbyte(w[m + 1], w[m], s);
+        const int vb = own - 4 * m;   // valid bytes of word m
+        if (vb < 4) c[m] = (vb <= 0) ? 0u : (c[m] & ((1u << (8 * vb)) - 1u));
+    }
+    const double T16 = 65536.0, T32 = 4294967296.0;
+    double acc = __builtin_fma((double)(c[1] & 0xffffu), T32, (double)c[0]);
+    if (own > 6) {
+        const double p1 = __builtin_fma((double)c[2], T16, (double)(c[1] >> 16));
+        acc += fa_mulmod(p1, pow2[1], n, ninv);
+    }
+    if (own > 12) {
+        const double p2 = __builtin_fma((double)(c[4] & 0xffffu), T32, (double)c[3]);
+        acc += fa_mulmod(p2, pow2[2], n, ninv);
+    }
+    if (own > 18) {
+        const double p3 = __builtin_fma((double)c[5], T16, (double)(c[4] >> 16));
+        acc += fa_mulmod(p3, pow2[3], n, ninv);
+        if (own > 24) acc += fa_mulmod((double)c[6], pow2[4], n, ninv);
+        acc = fa_reduce_pm1n(acc, n, ninv);
+    }
+    return acc;
+}
+
 // ------------------------------------------------------------------------
 // operand import: the value of element x (coefficient index) of an operand
 // ------------------------------------------------------------------------
@@ -96,8 +152,11 @@ FA_DEV double fa_coeff_residue_s(const uint64_t* __restrict__ c, int w,
 FA_DEV double fa_operand_value(const fa_operand& op, uint64_t x,
                                const double* __restrict__ pow2, double n, double ninv)
 {
-    if (op.kind == FA_PACK_MPN)
+    if (op.kind == FA_PACK_MPN) {
+        if ((op.bits & 7) == 0 && op.bits <= 200)
+            return fa_chunk_residue8(op.src, op.n, op.bits >> 3, x, pow2, n, ninv);
         return fa_chunk_residue(op.src, op.n, op.bits, x, pow2, n, ninv);
+    }
     if (x >= op.n) return 0.0;
     if (op.kind == FA_PACK_U64) {
         uint64_t v = op.src[x];
@@ -108,3 +167,36 @@ FA_DEV double fa_operand_value(const fa_operand& op, uint64_t x,
     return fa_coeff_residue_s(op.src + x * (uint64_t)op.width, op.width, pow2, n, ninv);
 }
 
+// import specialized at compile time (KIND = FA_PACK_*), FA_PACK_ANY
+// dispatching on op.kind
+template <int KIND>
+FA_DEV double fa_operand_value_k(const fa_operand& op, uint64_t x,
+                                 const double* __restrict__ pow2, double n, double ninv)
+{
+    if constexpr (KIND == FA_PACK_MPN8) {
+        return fa_chunk_residue8(op.src, op.n, op.bits >> 3, x, pow2, n, ninv);
+    } else if constexpr (KIND == FA_PACK_MPN) {
+        return fa_chunk_residue(op.src, op.n, op.bits, x, pow2, n, ninv);
+    } else if constexpr (KIND == FA_PACK_U64) {
+        if (x >= op.n) return 0.0;
+        uint64_t v = op.src[x];
+        double lo = (double)(uint32_t)v, hi = (double)(uint32_t)(v >> 32);
+        if (v < (UINT64_C(1) << 50)) return (double)v;
+        return fa_reduce_pm1n(lo + fa_mulmod(hi, pow2[0], n, ninv), n, ninv);
+    } else if constexpr (KIND == FA_PACK_S64W) {
+        if (x >= op.n) return 0.0;
+        return fa_coeff_residue_s(op.src + x * (uint64_t)op.width, op.width, pow2, n, ninv);
+    } else {
+        return fa_operand_value(op, x, pow2, n, ninv);
+    }
+}
+
+// the specialization a pair of operands can share
+static inline int fa_pack_kind(const fa_operand& a, const fa_operand& b)
+{
+    if (a.kind != b.kind) return FA_PACK_ANY;
+    if (a.kind == FA_PACK_MPN)
+        return (a.bits == b.bits && (a.bits & 7) == 0 && a.bits <= 200) ? FA_PACK_MPN8
+                                                                         : FA_PACK_MPN;
+    return a.kind;
+}
