@@ -58,63 +58,76 @@ FA_DEV void fa_tw16(const double* __restrict__ w, uint64_t Q, double* t)
 }
 
 // ---------------------------------------------------------------------------
-// per-row twiddle table in LDS: entry (m, q) = w2[(r << m) + q], m < 12.
-// Levels m < 8 are stored at (2^m - 1) + q; levels m >= 8 transposed,
-// (2^m - 1) + (q mod 2^(m-8)) * 256 + (q >> (m-8)), so that the last stage
-// (thread t needs q = t*2^(m-8) + k) reads consecutive addresses.
+// row twiddles: for row r (root offset r at level 12) the entry of local
+// level m, block q is T(m, q) = w2[(r << m) + q] = rho_m * w2[q] with
+// rho_m = w2[r << m] (the index bits of r << m and q are disjoint), reduced
+// exactly to [-n/2, n/2].  Levels 0..7 (255 entries, shared by the first
+// two stages) live in LDS at (2^m - 1) + q; the 15 entries of the last
+// stage (levels 8..11 of block t) are used by thread t alone and are
+// computed in registers.
 // ---------------------------------------------------------------------------
 
-FA_DEV int fa_tab_index(int m, int q)
+FA_DEV double fa_row_entry(double rho, const double* __restrict__ w, int q, double n,
+                           double ninv)
 {
-    if (m < 8) return (1 << m) - 1 + q;
-    int s = m - 8;
-    return (1 << m) - 1 + ((q & ((1 << s) - 1)) << 8) + (q >> s);
+    const double hn = 0.5 * (n - 1.0);
+    double v = fa_mulmod(rho, w[q], n, ninv);
+    v = (v > hn) ? v - n : v;
+    v = (v < -hn) ? v + n : v;
+    return v;
 }
 
-// T(m, q) = rho_m * base[q] with rho_m = w[r << m] (multiplicative over the
-// disjoint bit ranges of (r << m) and q), reduced exactly to [-n/2, n/2]
+// rho[m] for m < 12 and the LDS entries of levels 0..7
 FA_DEV void fa_build_row_table(double* tab, const double* __restrict__ w, uint64_t r,
                                double* rho, double n, double ninv)
 {
     const int tid = threadIdx.x;
     if (tid < 12) rho[tid] = w[r << tid];
     __syncthreads();
-    const double hn = 0.5 * (n - 1.0);
-    for (int e = tid; e < 4095; e += blockDim.x) {
-        int m = 31 - __builtin_clz(e + 1);
-        int u = e + 1 - (1 << m);
-        int q = u;
-        if (m >= 8) {
-            int s = m - 8;
-            q = ((u & 255) << s) + (u >> 8);
-        }
-        double v = fa_mulmod(rho[m], w[q], n, ninv);
-        if (v > hn) v -= n;
-        else if (v < -hn) v += n;
-        tab[e] = v;
+    if (tid < 255) {
+        const int m = 31 - __builtin_clz(tid + 1);
+        tab[tid] = fa_row_entry(rho[m], w, tid + 1 - (1 << m), n, ninv);
     }
     __syncthreads();
 }
 
+// twiddles of stage j < 2 (levels 4j .. 4j+3) of block bq from the table
 FA_DEV void fa_row_tw(const double* tab, int j, int bq, double* t)
 {
     const int m = 4 * j;
-    t[0] = tab[fa_tab_index(m, bq)];
-    t[1] = tab[fa_tab_index(m + 1, 2 * bq)];
-    t[2] = tab[fa_tab_index(m + 1, 2 * bq + 1)];
+    t[0] = tab[(1 << m) - 1 + bq];
+    t[1] = tab[(2 << m) - 1 + 2 * bq];
+    t[2] = tab[(2 << m) - 1 + 2 * bq + 1];
 #pragma unroll
     for (int jj = 0; jj < 4; jj++) {
-        int Qp = 4 * bq + jj;
-        t[3 + 3 * jj] = tab[fa_tab_index(m + 2, Qp)];
-        t[4 + 3 * jj] = tab[fa_tab_index(m + 3, 2 * Qp)];
-        t[5 + 3 * jj] = tab[fa_tab_index(m + 3, 2 * Qp + 1)];
+        const int Qp = 4 * bq + jj;
+        t[3 + 3 * jj] = tab[(4 << m) - 1 + Qp];
+        t[4 + 3 * jj] = tab[(8 << m) - 1 + 2 * Qp];
+        t[5 + 3 * jj] = tab[(8 << m) - 1 + 2 * Qp + 1];
+    }
+}
+
+// twiddles of the last stage (levels 8..11) of block t, in registers
+FA_DEV void fa_row_tw_last(const double* rho, const double* __restrict__ w, int t, double* tw,
+                           double n, double ninv)
+{
+    tw[0] = fa_row_entry(rho[8], w, t, n, ninv);
+    tw[1] = fa_row_entry(rho[9], w, 2 * t, n, ninv);
+    tw[2] = fa_row_entry(rho[9], w, 2 * t + 1, n, ninv);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int Qp = 4 * t + j;
+        tw[3 + 3 * j] = fa_row_entry(rho[10], w, Qp, n, ninv);
+        tw[4 + 3 * j] = fa_row_entry(rho[11], w, 2 * Qp, n, ninv);
+        tw[5 + 3 * j] = fa_row_entry(rho[11], w, 2 * Qp + 1, n, ninv);
     }
 }
 
 // 4096-point forward transform of one row, 256 threads, input in the
 // stage-0 layout (positions t + 256k), output in the stage-2 layout
-// (positions 16t + k)
-FA_DEV void fa_fwd4096_row(double* x, double* xch, const double* tab, double n, double ninv)
+// (positions 16t + k); twl = the thread's last-stage twiddles
+FA_DEV void fa_fwd4096_row(double* x, double* xch, const double* tab, const double* twl,
+                           double n, double ninv)
 {
     const int t = threadIdx.x;
     double tw[15];
@@ -133,17 +146,16 @@ FA_DEV void fa_fwd4096_row(double* x, double* xch, const double* tab, double n, 
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 16; k++) x[k] = xch[PADX(16 * t + k)];
-    fa_row_tw(tab, 2, t, tw);
-    fa_fwd16(x, tw, n, ninv);
+    fa_fwd16(x, twl, n, ninv);
     __syncthreads();
 }
 
-FA_DEV void fa_inv4096_row(double* x, double* xch, const double* tab, double n, double ninv)
+FA_DEV void fa_inv4096_row(double* x, double* xch, const double* tab, const double* twl,
+                           double n, double ninv)
 {
     const int t = threadIdx.x;
     double tw[15];
-    fa_row_tw(tab, 2, t, tw);
-    fa_inv16(x, tw, n, ninv);
+    fa_inv16(x, twl, n, ninv);
 #pragma unroll
     for (int k = 0; k < 16; k++) xch[PADX(16 * t + k)] = x[k];
     __syncthreads();
@@ -170,30 +182,31 @@ FA_DEV void fa_row_conv_tile(double* __restrict__ arow, const double* __restrict
                              double* xch, double* tab, double* rho)
 {
     const int t = threadIdx.x;
-    double x[16], y[16];
+    double x[16], y[16], twl[15];
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        y[k] = brow[t + 256 * k];
-        x[k] = arow[t + 256 * k];
-    }
+    for (int k = 0; k < 16; k++) y[k] = brow[t + 256 * k];
     fa_build_row_table(tab, w2, r, rho, n, ninv);
-    fa_fwd4096_row(y, xch, tab, n, ninv);
-    fa_fwd4096_row(x, xch, tab, n, ninv);
+    fa_row_tw_last(rho, w2, t, twl, n, ninv);
+    fa_fwd4096_row(y, xch, tab, twl, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = arow[t + 256 * k];
+    fa_fwd4096_row(x, xch, tab, twl, n, ninv);
 #pragma unroll
     for (int k = 0; k < 16; k++) x[k] = fa_mulmod(fa_mulmod(x[k], m, n, ninv), y[k], n, ninv);
     fa_build_row_table(tab, w2inv, r, rho, n, ninv);
-    fa_inv4096_row(x, xch, tab, n, ninv);
+    fa_row_tw_last(rho, w2inv, t, twl, n, ninv);
+    fa_inv4096_row(x, xch, tab, twl, n, ninv);
 #pragma unroll
     for (int k = 0; k < 16; k++) arow[t + 256 * k] = x[k];
 }
 
 // blockIdx.x = row, blockIdx.y = prime
 __global__ void __launch_bounds__(256) fa_k_row_conv16(double* __restrict__ A,
-                                                       const double* __restrict__ B,
-                                                       uint64_t stride, fa_prime_set S)
+                                                          const double* __restrict__ B,
+                                                          uint64_t stride, fa_prime_set S)
 {
     __shared__ double xch[4352];
-    __shared__ double tab[4096];
+    __shared__ double tab[256];
     __shared__ double rho[16];
     const int i = blockIdx.y;
     const uint64_t r = blockIdx.x;
@@ -686,12 +699,12 @@ __global__ void __launch_bounds__(1024) fa_kb_col_inv(const fa_bjob* __restrict_
         fa_col_inv_tile<D1>(dst, FA_ROW_LG, w.tile, S.p[pi], S.pinv[pi], S.w2inv[pi], lds);
 }
 
-__global__ void __launch_bounds__(256) fa_kb_row_conv(const fa_bjob* __restrict__ J,
-                                                      const uint64_t* __restrict__ work,
-                                                      unsigned nwork, fa_prime_set S)
+__global__ void __launch_bounds__(256, 3) fa_kb_row_conv(const fa_bjob* __restrict__ J,
+                                                         const uint64_t* __restrict__ work,
+                                                         unsigned nwork, fa_prime_set S)
 {
     __shared__ double xch[4352];
-    __shared__ double tab[4096];
+    __shared__ double tab[256];
     __shared__ double rho[16];
     const unsigned L = fa_xcd_linear();
     if (L >= nwork) return;
