@@ -34,6 +34,19 @@ class NmodT(ctypes.Structure):
     _fields_ = [("n", _u64), ("ninv", _u64), ("norm", _u64)]
 
 
+class Request(ctypes.Structure):
+    """fa_request (include/flint_amd.h): one product of a batch."""
+    _fields_ = [("kind", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("a", _VP), ("b", _VP), ("an", _s64), ("bn", _s64), ("aw", _s64), ("bw", _s64),
+                ("z", _VP), ("zl", _s64), ("zh", _s64), ("zw", _s64), ("mod", NmodT),
+                ("abits", _s64), ("bbits", _s64), ("np", ctypes.c_int32), ("bits", ctypes.c_int32),
+                ("depth", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+REQ_MPN, REQ_NMOD, REQ_FMPZ = 0, 1, 2
+BATCH_DEVICE, BATCH_GRAPH = 1, 2
+
+
 class SdFftCtx(ctypes.Structure):
     """fa_sd_fft_ctx_struct (include/flint_amd.h)."""
     _fields_ = [("p", ctypes.c_double), ("pinv", ctypes.c_double), ("mod", NmodT),
@@ -74,6 +87,10 @@ SIGNATURES = {
     "fa_set_col_variant": (None, [ctypes.c_int]),
     "fa_set_crt_variant": (None, [ctypes.c_int]),
     "fa_probe_colcopy": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP]),
+    "fa_batch_create": (ctypes.c_int, [ctypes.POINTER(_VP), _VP, _s64, ctypes.c_int]),
+    "fa_batch_launch": (ctypes.c_int, [_VP, _VP]),
+    "fa_batch_destroy": (None, [_VP]),
+    "fa_batch_mul": (ctypes.c_int, [_VP, _s64, ctypes.c_int]),
 }
 
 _lib = None
@@ -229,6 +246,90 @@ def fmpz_poly_mul_mid(a, aw, b, bw, zl, zh, zw):
     rc = _check(lib().fa_fmpz_poly_mul_mid(_p(z), zw, zl, zh, _p(a), aw, len(a) // aw,
                                            _p(b), bw, len(b) // bw))
     return z[: (zh - zl) * zw] if rc == 1 else None
+
+
+def batch_mul(requests):
+    """Runs a list of host-side products as one batch (fa_batch_mul).
+
+    Each request is a dict: {"kind": "mpn", "a": limbs, "b": limbs},
+    {"kind": "nmod", "a", "b", "n", "zl", "zh"} or
+    {"kind": "fmpz", "a", "aw", "b", "bw", "zl", "zh", "zw"} (coefficients as
+    flat w-limb two's complement arrays).  Returns a list of (status, output)
+    with output None unless status == 1."""
+    n = len(requests)
+    R = (Request * max(n, 1))()
+    keep = []
+    outs = []
+    for k, q in enumerate(requests):
+        r = R[k]
+        a = _arr(q["a"])
+        b = _arr(q["b"])
+        keep += [a, b]
+        kind = q["kind"]
+        if kind == "mpn":
+            r.kind = REQ_MPN
+            r.an, r.bn = len(a), len(b)
+            z = np.zeros(len(a) + len(b), dtype=np.uint64)
+            r.np = q.get("np", 0)
+            r.bits = q.get("bits", 0)
+        elif kind == "nmod":
+            r.kind = REQ_NMOD
+            r.an, r.bn = len(a), len(b)
+            r.zl, r.zh = q["zl"], q["zh"]
+            r.mod = nmod_init(q["n"])
+            z = np.zeros(max(q["zh"] - q["zl"], 1), dtype=np.uint64)
+        elif kind == "fmpz":
+            r.kind = REQ_FMPZ
+            r.aw, r.bw, r.zw = q["aw"], q["bw"], q["zw"]
+            r.an, r.bn = len(a) // q["aw"], len(b) // q["bw"]
+            r.zl, r.zh = q["zl"], q["zh"]
+            r.abits, r.bbits = q.get("abits", 0), q.get("bbits", 0)
+            z = np.zeros(max((q["zh"] - q["zl"]) * q["zw"], 1), dtype=np.uint64)
+        else:
+            raise ValueError("kind must be mpn, nmod or fmpz")
+        r.a = a.ctypes.data
+        r.b = b.ctypes.data
+        r.z = z.ctypes.data
+        outs.append(z)
+    _check(lib().fa_batch_mul(ctypes.cast(R, _VP), n, 0))
+    res = []
+    for k, q in enumerate(requests):
+        st = R[k].status
+        z = outs[k]
+        if q["kind"] == "nmod":
+            z = z[: max(q["zh"] - q["zl"], 0)]
+        elif q["kind"] == "fmpz":
+            z = z[: (q["zh"] - q["zl"]) * q["zw"]]
+        res.append((st, z if st == 1 else None))
+    return res
+
+
+class DeviceBatch:
+    """A batch over device buffers, planned once and replayed
+    (fa_batch_create with FA_BATCH_DEVICE, optionally FA_BATCH_GRAPH).
+    `reqs` is a ctypes array of Request whose a, b, z are device addresses;
+    it must stay alive as long as the batch."""
+
+    def __init__(self, reqs, graph=True):
+        self.reqs = reqs
+        self.h = _VP()
+        flags = BATCH_DEVICE | (BATCH_GRAPH if graph else 0)
+        _check(lib().fa_batch_create(ctypes.byref(self.h), ctypes.cast(reqs, _VP), len(reqs),
+                                     flags))
+
+    def launch(self, stream=None):
+        _check(lib().fa_batch_launch(self.h, stream))
+
+    def close(self):
+        if self.h:
+            lib().fa_batch_destroy(self.h)
+            self.h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class SdFft:

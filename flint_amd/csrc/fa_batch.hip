@@ -1,0 +1,532 @@
+// fa_batch.hip -- batches of heterogeneous products (integer, nmod, fmpz)
+// executed as one recorded launch sequence.
+//
+// fa_batch_create plans every request (the same plans as the single-product
+// entry points), lays out one device allocation for all residue rows, job
+// records and work lists, and bins the products:
+//   - depth <= 12: one workgroup per (product, prime) does the whole
+//     convolution in LDS, binned by (depth, import kind);
+//   - depth 13..24: the column / row / column passes of the two-level
+//     transform, binned by (depth, import kind), every workgroup taking one
+//     (product, prime, operand, tile) item of its bin's work list;
+//   - CRT: integer products binned by prime count (then one carry
+//     workgroup per product), nmod and fmpz windows in 256-coefficient chunks.
+// The launch sequence is captured once into a hipGraph (FA_BATCH_GRAPH) and
+// replayed by fa_batch_launch.  No memset node is recorded: every output
+// word is written by a kernel (outputs past the product are written as 0).
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <utility>
+#include <vector>
+
+#include "fa_internal.h"
+#include "fa_crt_fp.h"
+#include "../../include/flint_amd.h"
+
+namespace {
+
+struct copy_op {
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+
+struct bin_key {
+    int depth, kind;
+    bool operator<(const bin_key& o) const
+    {
+        return depth != o.depth ? depth < o.depth : kind < o.kind;
+    }
+};
+
+struct tbin {
+    std::vector<uint64_t> fwd, row, inv;
+};
+
+// device-offset bookkeeping: everything lives in one allocation
+struct layout {
+    size_t size = 0;
+    size_t take(size_t bytes)
+    {
+        size_t off = size;
+        size += (bytes + 255) & ~(size_t)255;
+        return off;
+    }
+};
+
+int clog2_u64(uint64_t x) { return x <= 1 ? 0 : 64 - __builtin_clzll(x - 1); }
+int nbits_u64(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
+
+}  // namespace
+
+struct fa_batch_s {
+    int flags;
+    int64_t nreq;
+    fa_request* reqs;
+    hipStream_t stream;
+    char* dmem;
+    size_t dsize;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    std::vector<copy_op> h2d, d2h;
+    // recorded launch parameters (device addresses inside dmem)
+    fa_prime_set S;
+    std::vector<std::pair<bin_key, std::pair<size_t, unsigned>>> small;   // key -> (work, n)
+    struct tlaunch {
+        bin_key key;
+        size_t fwd, row, inv;
+        unsigned nf, nr, ni;
+    };
+    std::vector<tlaunch> trans;
+    size_t jobs;                                  // fa_bjob[]
+    size_t mpn_rec, fp_consts[FA_NPRIMES + 1];    // fa_bmpn[], fa_crt_fp_consts per np
+    std::vector<std::pair<int, std::pair<size_t, unsigned>>> crt_mpn;     // np -> (work, n)
+    size_t carry_jobs;
+    unsigned ncarry;
+    size_t poly_rec;
+    size_t nmod_work, fmpz_work;
+    unsigned nnmod, nfmpz;
+};
+
+static int batch_record(fa_batch_s* B, hipStream_t s)
+{
+    char* d = B->dmem;
+    const fa_bjob* J = (const fa_bjob*)(d + B->jobs);
+    int rc;
+    for (auto& e : B->small) {
+        rc = fa_batch_small_bin(e.first.depth, e.first.kind, J, (const uint64_t*)(d + e.second.first),
+                                e.second.second, B->S, s);
+        if (rc) return rc;
+    }
+    for (auto& t : B->trans) {
+        rc = fa_batch_transform_bin(t.key.depth, t.key.kind, J, (const uint64_t*)(d + t.fwd), t.nf,
+                                    (const uint64_t*)(d + t.row), t.nr,
+                                    (const uint64_t*)(d + t.inv), t.ni, B->S, s);
+        if (rc) return rc;
+    }
+    const fa_bmpn* M = (const fa_bmpn*)(d + B->mpn_rec);
+    for (auto& c : B->crt_mpn) {
+        rc = fa_batch_crt_mpn(c.first, M, (const uint64_t*)(d + c.second.first), c.second.second,
+                              d + B->fp_consts[c.first], s);
+        if (rc) return rc;
+    }
+    rc = fa_batch_carry(M, (const unsigned*)(d + B->carry_jobs), B->ncarry, s);
+    if (rc) return rc;
+    const fa_bpoly* P = (const fa_bpoly*)(d + B->poly_rec);
+    rc = fa_batch_crt_poly(0, P, (const uint64_t*)(d + B->nmod_work), B->nnmod, s);
+    if (rc) return rc;
+    return fa_batch_crt_poly(1, P, (const uint64_t*)(d + B->fmpz_work), B->nfmpz, s);
+}
+
+extern "C" void fa_batch_destroy(fa_batch_s* B)
+{
+    if (!B) return;
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    if (B->dmem) (void)hipFree(B->dmem);
+    if (B->stream) (void)hipStreamDestroy(B->stream);
+    delete B;
+}
+
+extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, int flags)
+{
+    if (!out || (nreq > 0 && !R) || nreq < 0)
+        return fa_set_error(FA_E_ARG, "fa_batch_create: bad arguments");
+    *out = nullptr;
+    const bool dev = (flags & FA_BATCH_DEVICE) != 0;
+    fa_batch_s* B = new fa_batch_s();
+    B->flags = flags;
+    B->nreq = nreq;
+    B->reqs = R;
+    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete B;
+        return fa_set_error(FA_E_HIP, "fa_batch_create: stream");
+    }
+
+    // ---- plan every request
+    struct plan {
+        int live;           // has device work
+        int np, offset, depth, bits, kind;
+        uint64_t zn, zlen, cnt, ngroups, nblk;
+        int64_t b1, b2;
+    };
+    std::vector<plan> P((size_t)nreq);
+    int maxdepth = 4;
+    for (int64_t r = 0; r < nreq; r++) {
+        fa_request& q = R[r];
+        plan& p = P[(size_t)r];
+        memset(&p, 0, sizeof(p));
+        q.status = 1;
+        if (q.an <= 0 || q.bn <= 0 || !q.a || !q.b || !q.z) {
+            q.status = FA_E_ARG;
+            continue;
+        }
+        if (q.kind == FA_REQ_MPN) {
+            fa_mpn_plan_t M;
+            int rc = (q.np || q.bits) ? fa_mpn_fixed_plan(q.an, q.bn, q.np, q.bits, &M)
+                                      : fa_mpn_choose_plan(q.an, q.bn, &M);
+            if (rc) { q.status = rc; continue; }
+            p.np = M.np; p.bits = M.bits; p.depth = M.depth; p.zlen = M.zlen;
+            p.kind = FA_PACK_MPN8;
+            p.zn = (uint64_t)(q.an + q.bn);
+            const uint64_t own = (uint64_t)M.bits / 8;
+            p.ngroups = (p.zn + own - 1) / own;
+            p.nblk = (p.ngroups + 30) / 31;
+            p.live = 1;
+        } else if (q.kind == FA_REQ_NMOD || q.kind == FA_REQ_FMPZ) {
+            if (q.zl < 0 || q.zh < q.zl) { q.status = FA_E_ARG; continue; }
+            p.zn = (uint64_t)(q.an + q.bn - 1);
+            p.cnt = (uint64_t)(q.zh - q.zl);
+            if (p.cnt == 0) { p.live = 0; continue; }
+            p.depth = clog2_u64(p.zn);
+            if (p.depth < 4) p.depth = 4;
+            if (q.kind == FA_REQ_NMOD) {
+                if (q.mod.n == 0) { q.status = FA_E_ARG; continue; }
+                p.kind = FA_PACK_U64;
+                if (q.mod.n == 1) {
+                    p.np = 0;        // zero product: only the CRT stage writes zeros
+                } else {
+                    fa_tables* T = fa_get_tables(p.depth < 20 ? 20 : p.depth);
+                    if (!T) { q.status = FA_E_NODEV; continue; }
+                    int rc = fa_nmod_primes(T, (uint64_t)(q.an < q.bn ? q.an : q.bn), q.mod.n,
+                                            &p.np, &p.offset);
+                    if (rc) { q.status = rc; continue; }
+                }
+            } else {
+                if (q.aw <= 0 || q.bw <= 0 || q.zw <= 0) { q.status = FA_E_ARG; continue; }
+                p.kind = FA_PACK_S64W;
+                p.b1 = q.abits;
+                p.b2 = q.bbits;
+                if (p.b1 == 0 && p.b2 == 0) {
+                    if (dev) { q.status = FA_E_ARG; continue; }   // bounds needed up front
+                    p.b1 = fa_vec_max_bits(q.a, (uint64_t)q.aw, q.an);
+                    p.b2 = fa_vec_max_bits(q.b, (uint64_t)q.bw, q.bn);
+                }
+                p.np = fa_fmpz_primes((uint64_t)q.bn, p.b1, p.b2);
+                if (!p.np) { q.status = 0; continue; }   // declined, as the reference does
+            }
+            p.live = 1;
+        } else {
+            q.status = FA_E_ARG;
+            continue;
+        }
+        if (p.depth > FA_MAX_DEPTH) {
+            q.status = fa_set_error(FA_E_RANGE, "batch request %lld: depth %d", (long long)r,
+                                    p.depth);
+            p.live = 0;
+            continue;
+        }
+        q.np = p.np;
+        q.bits = p.bits;
+        q.depth = p.depth;
+        if (p.depth > maxdepth) maxdepth = p.depth;
+    }
+    fa_tables* T = fa_get_tables(maxdepth < 20 ? 20 : maxdepth);
+    if (!T) { fa_batch_destroy(B); return FA_E_NODEV; }
+    fa_prime_set_init(&B->S, T, FA_NPRIMES, maxdepth, 0, 0);
+
+    // ---- layout
+    layout L;
+    std::vector<fa_bjob> jobs;
+    std::vector<fa_bmpn> mrec;
+    std::vector<fa_bpoly> prec;
+    std::vector<int> job_of((size_t)nreq, -1), rec_of((size_t)nreq, -1);
+    struct offs { size_t A, B, a, b, z, flags, agg; };
+    std::vector<offs> O((size_t)nreq);
+    std::map<std::pair<int, int>, size_t> pconsts;   // (np, offset) -> device offset
+    for (int64_t r = 0; r < nreq; r++) {
+        fa_request& q = R[r];
+        plan& p = P[(size_t)r];
+        if (!p.live || q.status != 1) continue;
+        offs& o = O[(size_t)r];
+        const uint64_t stride = UINT64_C(1) << p.depth;
+        if (p.np) {
+            o.A = L.take((size_t)p.np * stride * sizeof(double));
+            if (p.depth > FA_ROW_LG) o.B = L.take((size_t)p.np * stride * sizeof(double));
+        }
+        if (!dev) {
+            const size_t aw = q.kind == FA_REQ_FMPZ ? (size_t)q.aw : 1;
+            const size_t bw = q.kind == FA_REQ_FMPZ ? (size_t)q.bw : 1;
+            o.a = L.take((size_t)q.an * aw * 8);
+            o.b = L.take((size_t)q.bn * bw * 8);
+        }
+        size_t zwords = q.kind == FA_REQ_MPN ? (size_t)p.zn
+                        : q.kind == FA_REQ_NMOD ? (size_t)p.cnt : (size_t)p.cnt * (size_t)q.zw;
+        if (!dev) o.z = L.take(zwords * 8);
+        if (q.kind == FA_REQ_MPN) {
+            o.flags = L.take((size_t)p.ngroups + 8);
+            o.agg = L.take((size_t)p.nblk + 8);
+        }
+        if (q.kind != FA_REQ_MPN && p.np && !pconsts.count({p.np, p.offset}))
+            pconsts[{p.np, p.offset}] = L.take(sizeof(fa_crt_consts));
+    }
+    size_t fpc[FA_NPRIMES + 1];
+    for (int k = 0; k <= FA_NPRIMES; k++) fpc[k] = (k >= 2) ? L.take(sizeof(fa_crt_fp_consts)) : 0;
+
+    // job records and work lists (host images first; offsets resolved after
+    // the allocation)
+    std::map<bin_key, std::vector<uint64_t>> small_bins;
+    std::map<bin_key, tbin> trans_bins;
+    std::map<int, std::vector<uint64_t>> crt_bins;
+    std::vector<unsigned> carry;
+    std::vector<uint64_t> nmod_w, fmpz_w;
+    for (int64_t r = 0; r < nreq; r++) {
+        fa_request& q = R[r];
+        plan& p = P[(size_t)r];
+        if (!p.live || q.status != 1) continue;
+        if (p.np) {
+            const unsigned jid = (unsigned)jobs.size();
+            job_of[(size_t)r] = (int)jid;
+            fa_bjob j;
+            memset(&j, 0, sizeof(j));
+            j.an = (uint64_t)q.an;
+            j.bn = (uint64_t)q.bn;
+            j.aw = q.kind == FA_REQ_FMPZ ? (int)q.aw : 1;
+            j.bw = q.kind == FA_REQ_FMPZ ? (int)q.bw : 1;
+            j.kind = p.kind == FA_PACK_MPN8 ? FA_PACK_MPN : p.kind;
+            j.bits = p.bits;
+            j.stride = UINT64_C(1) << p.depth;
+            j.np = p.np;
+            j.offset = p.offset;
+            j.depth = p.depth;
+            fa_prime_set Sj;
+            fa_prime_set_init(&Sj, T, p.np, p.depth, p.offset, 1);
+            for (int i = 0; i < p.np; i++) j.norm[i] = Sj.norm[i];
+            jobs.push_back(j);
+            const bin_key key{p.depth, p.kind};
+            if (p.depth <= FA_ROW_LG) {
+                auto& w = small_bins[key];
+                for (int i = 0; i < p.np; i++) w.push_back(FA_WORK(jid, i, 0, 0));
+            } else {
+                auto& tb = trans_bins[key];
+                const int ng = fa_batch_col_groups(p.depth);
+                const int nrow = 1 << (p.depth - FA_ROW_LG);
+                for (int z = 0; z < 2; z++)
+                    for (int g = 0; g < ng; g++)
+                        for (int i = 0; i < p.np; i++) tb.fwd.push_back(FA_WORK(jid, i, z, g));
+                for (int i = 0; i < p.np; i++)
+                    for (int row = 0; row < nrow; row++) tb.row.push_back(FA_WORK(jid, i, 0, row));
+                for (int g = 0; g < ng; g++)
+                    for (int i = 0; i < p.np; i++) tb.inv.push_back(FA_WORK(jid, i, 0, g));
+            }
+        }
+        if (q.kind == FA_REQ_MPN) {
+            const unsigned mid = (unsigned)mrec.size();
+            rec_of[(size_t)r] = (int)mid;
+            fa_bmpn m;
+            memset(&m, 0, sizeof(m));
+            m.stride = UINT64_C(1) << p.depth;
+            m.zlen = p.zlen;
+            m.zn = p.zn;
+            m.ngroups = p.ngroups;
+            m.nblk = p.nblk;
+            m.bits = p.bits;
+            m.np = p.np;
+            mrec.push_back(m);
+            auto& w = crt_bins[p.np];
+            for (uint64_t blk = 0; blk < p.nblk; blk++) w.push_back((uint64_t)mid | (blk << 32));
+            carry.push_back(mid);
+        } else {
+            const unsigned pid = (unsigned)prec.size();
+            rec_of[(size_t)r] = (int)pid;
+            fa_bpoly b;
+            memset(&b, 0, sizeof(b));
+            b.stride = UINT64_C(1) << p.depth;
+            b.zl = (uint64_t)q.zl;
+            b.cnt = p.cnt;
+            b.zn = p.zn;
+            if (q.kind == FA_REQ_NMOD) {
+                const int mb = nbits_u64(q.mod.n);
+                b.n = q.mod.n;
+                b.norm = 64 - mb;
+                const unsigned __int128 d = (unsigned __int128)(q.mod.n << b.norm);
+                b.dinv = (uint64_t)((~(unsigned __int128)0) / d - ((unsigned __int128)1 << 64));
+            } else {
+                b.zw = (int)q.zw;
+            }
+            prec.push_back(b);
+            auto& w = q.kind == FA_REQ_NMOD ? nmod_w : fmpz_w;
+            for (uint64_t c = 0; c < (p.cnt + 255) / 256; c++) w.push_back((uint64_t)pid | (c << 32));
+        }
+    }
+    B->jobs = L.take(jobs.size() * sizeof(fa_bjob) + 8);
+    B->mpn_rec = L.take(mrec.size() * sizeof(fa_bmpn) + 8);
+    B->poly_rec = L.take(prec.size() * sizeof(fa_bpoly) + 8);
+    for (auto& e : small_bins) {
+        size_t w = L.take(e.second.size() * 8);
+        B->small.push_back({e.first, {w, (unsigned)e.second.size()}});
+    }
+    for (auto& e : trans_bins) {
+        fa_batch_s::tlaunch t;
+        t.key = e.first;
+        t.fwd = L.take(e.second.fwd.size() * 8);
+        t.row = L.take(e.second.row.size() * 8);
+        t.inv = L.take(e.second.inv.size() * 8);
+        t.nf = (unsigned)e.second.fwd.size();
+        t.nr = (unsigned)e.second.row.size();
+        t.ni = (unsigned)e.second.inv.size();
+        B->trans.push_back(t);
+    }
+    for (auto& e : crt_bins) B->crt_mpn.push_back({e.first, {L.take(e.second.size() * 8),
+                                                             (unsigned)e.second.size()}});
+    B->carry_jobs = L.take(carry.size() * 4 + 8);
+    B->ncarry = (unsigned)carry.size();
+    B->nmod_work = L.take(nmod_w.size() * 8 + 8);
+    B->fmpz_work = L.take(fmpz_w.size() * 8 + 8);
+    B->nnmod = (unsigned)nmod_w.size();
+    B->nfmpz = (unsigned)fmpz_w.size();
+    for (int k = 0; k <= FA_NPRIMES; k++) B->fp_consts[k] = fpc[k];
+
+    B->dsize = L.size + 256;
+    if (hipMalloc((void**)&B->dmem, B->dsize) != hipSuccess) {
+        fa_batch_destroy(B);
+        return fa_set_error(FA_E_NOMEM, "fa_batch_create: %zu bytes of device memory", L.size);
+    }
+    char* d = B->dmem;
+
+    // resolve device addresses into the records
+    for (int64_t r = 0; r < nreq; r++) {
+        fa_request& q = R[r];
+        plan& p = P[(size_t)r];
+        if (!p.live || q.status != 1) continue;
+        offs& o = O[(size_t)r];
+        const uint64_t* da = dev ? q.a : (const uint64_t*)(d + o.a);
+        const uint64_t* db = dev ? q.b : (const uint64_t*)(d + o.b);
+        uint64_t* dz = dev ? q.z : (uint64_t*)(d + o.z);
+        if (!dev) {
+            const size_t aw = q.kind == FA_REQ_FMPZ ? (size_t)q.aw : 1;
+            const size_t bw = q.kind == FA_REQ_FMPZ ? (size_t)q.bw : 1;
+            B->h2d.push_back({(void*)da, q.a, (size_t)q.an * aw * 8});
+            B->h2d.push_back({(void*)db, q.b, (size_t)q.bn * bw * 8});
+            size_t zwords = q.kind == FA_REQ_MPN ? (size_t)p.zn
+                            : q.kind == FA_REQ_NMOD ? (size_t)p.cnt : (size_t)p.cnt * (size_t)q.zw;
+            B->d2h.push_back({q.z, dz, zwords * 8});
+        }
+        if (job_of[(size_t)r] >= 0) {
+            fa_bjob& j = jobs[(size_t)job_of[(size_t)r]];
+            j.a = da;
+            j.b = db;
+            j.A = (double*)(d + o.A);
+            j.B = p.depth > FA_ROW_LG ? (double*)(d + o.B) : nullptr;
+        }
+        if (q.kind == FA_REQ_MPN) {
+            fa_bmpn& m = mrec[(size_t)rec_of[(size_t)r]];
+            m.Q = (const double*)(d + o.A);
+            m.z = dz;
+            m.flags = (uint8_t*)(d + o.flags);
+            m.agg = (uint8_t*)(d + o.agg);
+        } else {
+            fa_bpoly& b = prec[(size_t)rec_of[(size_t)r]];
+            b.Q = p.np ? (const double*)(d + o.A) : nullptr;
+            b.C = p.np ? (const void*)(d + pconsts[{p.np, p.offset}]) : nullptr;
+            b.z = dz;
+        }
+    }
+
+    // upload records, constants and work lists in one image
+    std::vector<char> img(L.size, 0);
+    auto put = [&](size_t off, const void* src, size_t bytes) {
+        if (bytes) memcpy(img.data() + off, src, bytes);
+    };
+    put(B->jobs, jobs.data(), jobs.size() * sizeof(fa_bjob));
+    put(B->mpn_rec, mrec.data(), mrec.size() * sizeof(fa_bmpn));
+    put(B->poly_rec, prec.data(), prec.size() * sizeof(fa_bpoly));
+    {
+        size_t k = 0;
+        for (auto& e : small_bins) { put(B->small[k].second.first, e.second.data(), e.second.size() * 8); k++; }
+        k = 0;
+        for (auto& e : trans_bins) {
+            put(B->trans[k].fwd, e.second.fwd.data(), e.second.fwd.size() * 8);
+            put(B->trans[k].row, e.second.row.data(), e.second.row.size() * 8);
+            put(B->trans[k].inv, e.second.inv.data(), e.second.inv.size() * 8);
+            k++;
+        }
+        k = 0;
+        for (auto& e : crt_bins) { put(B->crt_mpn[k].second.first, e.second.data(), e.second.size() * 8); k++; }
+    }
+    put(B->carry_jobs, carry.data(), carry.size() * 4);
+    put(B->nmod_work, nmod_w.data(), nmod_w.size() * 8);
+    put(B->fmpz_work, fmpz_w.data(), fmpz_w.size() * 8);
+    for (auto& e : pconsts) {
+        fa_crt_consts C;
+        fa_crt_consts_init(&C, e.first.first, e.first.second);
+        put(e.second, &C, sizeof(C));
+    }
+    for (int k = 2; k <= FA_NPRIMES; k++) {
+        fa_crt_consts C;
+        fa_crt_consts_init(&C, k, 0);
+        fa_crt_fp_consts F;
+        fa_crt_fp_consts_init(&F, C);
+        put(fpc[k], &F, sizeof(F));
+    }
+    // (the residue rows and staging regions of the image are don't-care)
+    if (hipMemcpy(d, img.data(), L.size, hipMemcpyHostToDevice) != hipSuccess) {
+        fa_batch_destroy(B);
+        return fa_set_error(FA_E_HIP, "fa_batch_create: upload");
+    }
+
+    if (flags & FA_BATCH_GRAPH) {
+        if (hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            fa_batch_destroy(B);
+            return fa_set_error(FA_E_HIP, "fa_batch_create: begin capture");
+        }
+        const int prof = fa_prof_on();
+        if (prof) fa_prof_enable(0);
+        int rc = batch_record(B, B->stream);
+        if (prof) fa_prof_enable(1);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(B->stream, &g);
+        if (rc || e != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            fa_batch_destroy(B);
+            return rc ? rc : fa_set_error(FA_E_HIP, "fa_batch_create: end capture");
+        }
+        B->graph = g;
+        if (hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            fa_batch_destroy(B);
+            return fa_set_error(FA_E_HIP, "fa_batch_create: instantiate");
+        }
+    }
+    *out = B;
+    return FA_OK;
+}
+
+extern "C" int fa_batch_launch(fa_batch_s* B, void* stream)
+{
+    if (!B) return fa_set_error(FA_E_ARG, "fa_batch_launch: null batch");
+    hipStream_t s = stream ? (hipStream_t)stream : B->stream;
+    for (auto& c : B->h2d)
+        if (hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, s) != hipSuccess)
+            return fa_set_error(FA_E_HIP, "fa_batch_launch: upload");
+    int rc = FA_OK;
+    if (B->exec) {
+        if (hipGraphLaunch(B->exec, s) != hipSuccess)
+            return fa_set_error(FA_E_HIP, "fa_batch_launch: graph launch");
+    } else {
+        rc = batch_record(B, s);
+        if (rc) return rc;
+    }
+    for (auto& c : B->d2h)
+        if (hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, s) != hipSuccess)
+            return fa_set_error(FA_E_HIP, "fa_batch_launch: download");
+    if (!(B->flags & FA_BATCH_DEVICE) || !stream) {
+        if (hipStreamSynchronize(s) != hipSuccess)
+            return fa_set_error(FA_E_HIP, "fa_batch_launch: %s",
+                                hipGetErrorString(hipGetLastError()));
+    }
+    return FA_OK;
+}
+
+extern "C" int fa_batch_mul(fa_request* R, int64_t nreq, int flags)
+{
+    fa_batch_s* B = nullptr;
+    int rc = fa_batch_create(&B, R, nreq, flags & ~FA_BATCH_DEVICE);
+    if (rc) return rc;
+    rc = fa_batch_launch(B, nullptr);
+    fa_batch_destroy(B);
+    return rc;
+}

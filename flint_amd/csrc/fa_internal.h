@@ -135,7 +135,37 @@ struct fa_bjob {
     ((uint64_t)(job) | ((uint64_t)(prime) << 32) | ((uint64_t)(op) << 36) |              \
      ((uint64_t)(tile) << 40))
 
+// per-product records of the batched CRT stages (device)
+struct fa_bmpn {               // integer product
+    const double* Q;           // residue rows (np x stride)
+    uint64_t stride;
+    uint64_t zlen;             // coefficients of the chunked product
+    uint64_t* z;               // output limbs
+    uint64_t zn;               // an + bn
+    uint64_t ngroups, nblk;    // CRT groups of 8 coefficients, CRT workgroups
+    uint8_t* flags;            // per group / per workgroup carry flags
+    uint8_t* agg;
+    int bits, np;
+};
+struct fa_bpoly {              // nmod or fmpz polynomial product
+    const double* Q;           // residue rows; null: the zero product (n = 1)
+    uint64_t stride;
+    const void* C;             // fa_crt_consts of the prime set (device)
+    uint64_t zl, cnt, zn;      // window [zl, zl + cnt) of a product of zn coefficients
+    uint64_t n, dinv;          // nmod modulus and its preinverse
+    int norm, zw;              // nmod shift, fmpz output limbs per coefficient
+    uint64_t* z;
+};
+
 extern "C++" {
+int fa_batch_crt_mpn(int np, const fa_bmpn* M, const uint64_t* work, unsigned nwork,
+                     const void* F, hipStream_t s);
+int fa_batch_carry(const fa_bmpn* M, const unsigned* jobs, unsigned njobs, hipStream_t s);
+int fa_nmod_primes(const fa_tables* T, uint64_t len, uint64_t n, int* np, int* offset);
+int fa_fmpz_primes(uint64_t bn, int64_t b1, int64_t b2);
+int64_t fa_vec_max_bits(const uint64_t* v, uint64_t w, int64_t n);
+int fa_batch_crt_poly(int fmpz, const fa_bpoly* P, const uint64_t* work, unsigned nwork,
+                      hipStream_t s);
 // launches of one bin; return FA_OK or an error code
 int fa_batch_small_bin(int depth, int kind, const fa_bjob* J, const uint64_t* work,
                        unsigned nwork, const fa_prime_set& S, hipStream_t s);

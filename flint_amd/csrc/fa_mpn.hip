@@ -653,10 +653,9 @@ __global__ void __launch_bounds__(64) fa_k_carry_apply16(const uint8_t* __restri
 // scanned in LDS, and each thread then walks its range: a workgroup that
 // receives no carry and generates none inside (bit 2 of its aggregate) is
 // skipped, any other is resolved group by group from the group flags.
-__global__ void __launch_bounds__(1024) fa_k_carry_fused(const uint8_t* __restrict__ flags,
-                                                         const uint8_t* __restrict__ agg,
-                                                         uint64_t* __restrict__ z, uint64_t zn,
-                                                         int own, uint64_t ngroups, uint64_t nblk)
+FA_DEV void fa_carry_resolve(const uint8_t* __restrict__ flags, const uint8_t* __restrict__ agg,
+                             uint64_t* __restrict__ z, uint64_t zn, int own, uint64_t ngroups,
+                             uint64_t nblk)
 {
     __shared__ uint8_t sg[1024], sp[1024];
     const int t = threadIdx.x;
@@ -728,6 +727,14 @@ __global__ void __launch_bounds__(1024) fa_k_carry_fused(const uint8_t* __restri
     }
 }
 
+__global__ void __launch_bounds__(1024) fa_k_carry_fused(const uint8_t* __restrict__ flags,
+                                                         const uint8_t* __restrict__ agg,
+                                                         uint64_t* __restrict__ z, uint64_t zn,
+                                                         int own, uint64_t ngroups, uint64_t nblk)
+{
+    fa_carry_resolve(flags, agg, z, zn, own, ngroups, nblk);
+}
+
 // ------------------------------------------------------------------------
 // CRT on the FP64 lanes (fa_crt_fp.h), 31 groups of 8 coefficients per
 // workgroup of 256 threads plus one halo group in front (its overhang
@@ -754,11 +761,10 @@ FA_DEV uint32_t fa_seg_ballot(bool pred)
 }
 
 template <int NP>
-__global__ void __launch_bounds__(256) fa_k_crt_fp(const double* __restrict__ Q, uint64_t stride,
-                                                   fa_crt_fp_consts F, uint64_t zlen, int bits,
-                                                   uint64_t* __restrict__ z, uint64_t zn,
-                                                   uint64_t ngroups, uint8_t* __restrict__ flags,
-                                                   uint8_t* __restrict__ agg)
+FA_DEV void fa_crt_fp_block(const double* __restrict__ Q, uint64_t stride,
+                            const fa_crt_fp_consts& F, uint64_t zlen, int bits,
+                            uint64_t* __restrict__ z, uint64_t zn, uint64_t ngroups,
+                            uint8_t* __restrict__ flags, uint8_t* __restrict__ agg, uint64_t blk)
 {
     constexpr int NL = fa_crt_fp_dims<NP>::NL;
     constexpr int NLS = NL + 1;
@@ -769,7 +775,7 @@ __global__ void __launch_bounds__(256) fa_k_crt_fp(const double* __restrict__ Q,
     const int own = bits >> 3;
     const int TL = ((7 * bits) >> 6) + NLS;
     const int ovh = TL - own;
-    const int64_t g0 = (int64_t)blockIdx.x * (FA_CRT16_GROUPS - 1) - 1;
+    const int64_t g0 = (int64_t)blk * (FA_CRT16_GROUPS - 1) - 1;
 
     {   // 1. the coefficient of thread t, shifted to its bit offset in the group
         const int64_t j = g0 * 8 + t;
@@ -873,8 +879,74 @@ __global__ void __launch_bounds__(256) fa_k_crt_fp(const double* __restrict__ Q,
             pp = (uint8_t)(pp & ((ft >> 1) & 1));
             any = (uint8_t)(any | (ft & 1));
         }
-        agg[blockIdx.x] = (uint8_t)(gg | (pp << 1) | (any << 2));
+        agg[blk] = (uint8_t)(gg | (pp << 1) | (any << 2));
     }
+}
+
+template <int NP>
+__global__ void __launch_bounds__(256) fa_k_crt_fp(const double* __restrict__ Q, uint64_t stride,
+                                                   fa_crt_fp_consts F, uint64_t zlen, int bits,
+                                                   uint64_t* __restrict__ z, uint64_t zn,
+                                                   uint64_t ngroups, uint8_t* __restrict__ flags,
+                                                   uint8_t* __restrict__ agg)
+{
+    fa_crt_fp_block<NP>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, blockIdx.x);
+}
+
+// batched CRT of integer products: work item = job | block << 32
+FA_DEV unsigned fa_xcd_lin()
+{
+    const unsigned nb = gridDim.x, bid = blockIdx.x;
+    return (bid & 7) * (nb >> 3) + (bid >> 3);
+}
+
+template <int NP>
+__global__ void __launch_bounds__(256) fa_kb_crt_fp(const fa_bmpn* __restrict__ M,
+                                                    const uint64_t* __restrict__ work,
+                                                    unsigned nwork,
+                                                    const fa_crt_fp_consts* __restrict__ F)
+{
+    const unsigned L = fa_xcd_lin();
+    if (L >= nwork) return;
+    const uint64_t w = work[L];
+    const fa_bmpn& m = M[(unsigned)(w & 0xffffffffu)];
+    fa_crt_fp_block<NP>(m.Q, m.stride, *F, m.zlen, m.bits, m.z, m.zn, m.ngroups, m.flags, m.agg,
+                        w >> 32);
+}
+
+__global__ void __launch_bounds__(1024) fa_kb_carry(const fa_bmpn* __restrict__ M,
+                                                    const unsigned* __restrict__ jobs,
+                                                    unsigned njobs)
+{
+    if (blockIdx.x >= njobs) return;
+    const fa_bmpn& m = M[jobs[blockIdx.x]];
+    fa_carry_resolve(m.flags, m.agg, m.z, m.zn, m.bits / 8, m.ngroups, m.nblk);
+}
+
+int fa_batch_crt_mpn(int np, const fa_bmpn* M, const uint64_t* work, unsigned nwork,
+                     const void* Fv, hipStream_t s)
+{
+    if (!nwork) return FA_OK;
+    const fa_crt_fp_consts* F = (const fa_crt_fp_consts*)Fv;
+    const unsigned grid = (nwork + 7) & ~7u;
+    switch (np) {
+    case 2: FA_LAUNCH("fa_kb_crt_fp", fa_kb_crt_fp<2>, dim3(grid), dim3(256), 0, s, M, work, nwork, F); break;
+    case 3: FA_LAUNCH("fa_kb_crt_fp", fa_kb_crt_fp<3>, dim3(grid), dim3(256), 0, s, M, work, nwork, F); break;
+    case 4: FA_LAUNCH("fa_kb_crt_fp", fa_kb_crt_fp<4>, dim3(grid), dim3(256), 0, s, M, work, nwork, F); break;
+    case 5: FA_LAUNCH("fa_kb_crt_fp", fa_kb_crt_fp<5>, dim3(grid), dim3(256), 0, s, M, work, nwork, F); break;
+    case 6: FA_LAUNCH("fa_kb_crt_fp", fa_kb_crt_fp<6>, dim3(grid), dim3(256), 0, s, M, work, nwork, F); break;
+    case 7: FA_LAUNCH("fa_kb_crt_fp", fa_kb_crt_fp<7>, dim3(grid), dim3(256), 0, s, M, work, nwork, F); break;
+    case 8: FA_LAUNCH("fa_kb_crt_fp", fa_kb_crt_fp<8>, dim3(grid), dim3(256), 0, s, M, work, nwork, F); break;
+    default: return fa_set_error(FA_E_ARG, "batch crt: np %d", np);
+    }
+    return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "batch crt launch");
+}
+
+int fa_batch_carry(const fa_bmpn* M, const unsigned* jobs, unsigned njobs, hipStream_t s)
+{
+    if (!njobs) return FA_OK;
+    FA_LAUNCH("fa_kb_carry", fa_kb_carry, dim3(njobs), dim3(1024), 0, s, M, jobs, njobs);
+    return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "batch carry launch");
 }
 
 int fa_crt_fp_enabled = 1;

@@ -165,6 +165,88 @@ __global__ void __launch_bounds__(256) fa_k_crt_fmpz(const double* __restrict__ 
     }
 }
 
+// batched nmod / fmpz CRT: work item = job | chunk << 32, 256 outputs per
+// chunk; outputs at or beyond the product length are zero
+FA_DEV void fa_emit_fmpz(uint64_t* s, const fa_crt_consts& C, uint64_t* o, uint64_t zw)
+{
+    int gt = 0;
+    for (int l = FA_CRT_MAXL - 1; l >= 0; l--) {
+        uint64_t h = (C.M[l] >> 1) | ((l + 1 < FA_CRT_MAXL) ? (C.M[l + 1] << 63) : 0);
+        if (s[l] != h) { gt = s[l] > h; break; }
+    }
+    uint64_t ext = 0;
+    if (gt) {
+        uint64_t br = 0;
+#pragma unroll
+        for (int l = 0; l < FA_CRT_MAXL; l++) {
+            uint64_t t1 = s[l] - C.M[l];
+            uint64_t b1 = (s[l] < C.M[l]);
+            uint64_t t2 = t1 - br;
+            uint64_t b2 = (t1 < br);
+            s[l] = t2;
+            br = b1 | b2;
+        }
+        ext = ~UINT64_C(0);
+    }
+    for (uint64_t l = 0; l < zw; l++) {
+        uint64_t v = ext;
+#pragma unroll
+        for (int k = 0; k < FA_CRT_MAXL; k++)
+            if ((uint64_t)k == l) v = s[k];
+        o[l] = v;
+    }
+}
+
+template <int FMPZ>
+__global__ void __launch_bounds__(256) fa_kb_crt_poly(const fa_bpoly* __restrict__ P,
+                                                      const uint64_t* __restrict__ work,
+                                                      unsigned nwork)
+{
+    const unsigned nb = gridDim.x, bid = blockIdx.x;
+    const unsigned L = (bid & 7) * (nb >> 3) + (bid >> 3);
+    if (L >= nwork) return;
+    const uint64_t w = work[L];
+    const fa_bpoly& j = P[(unsigned)(w & 0xffffffffu)];
+    const uint64_t t = (w >> 32) * 256 + threadIdx.x;
+    if (t >= j.cnt) return;
+    const uint64_t idx = j.zl + t;
+    if (FMPZ) {
+        uint64_t* o = j.z + t * (uint64_t)j.zw;
+        if (idx >= j.zn || !j.Q) {
+            for (int l = 0; l < j.zw; l++) o[l] = 0;
+            return;
+        }
+        const fa_crt_consts& C = *(const fa_crt_consts*)j.C;
+        uint64_t s[FA_CRT_MAXL];
+        fa_crt_coeff_poly(j.Q, j.stride, idx, C, s);
+        fa_emit_fmpz(s, C, o, (uint64_t)j.zw);
+    } else {
+        if (idx >= j.zn || !j.Q) {
+            j.z[t] = 0;
+            return;
+        }
+        const fa_crt_consts& C = *(const fa_crt_consts*)j.C;
+        uint64_t s[FA_CRT_MAXL];
+        fa_crt_coeff_poly(j.Q, j.stride, idx, C, s);
+        uint64_t r = 0;
+#pragma unroll
+        for (int l = FA_CRT_MAXL - 1; l >= 0; l--) r = fa_ll_mod_preinv(r, s[l], j.n, j.dinv, j.norm);
+        j.z[t] = r;
+    }
+}
+
+int fa_batch_crt_poly(int fmpz, const fa_bpoly* P, const uint64_t* work, unsigned nwork,
+                      hipStream_t s)
+{
+    if (!nwork) return FA_OK;
+    const unsigned grid = (nwork + 7) & ~7u;
+    if (fmpz)
+        FA_LAUNCH("fa_kb_crt_fmpz", fa_kb_crt_poly<1>, dim3(grid), dim3(256), 0, s, P, work, nwork);
+    else
+        FA_LAUNCH("fa_kb_crt_nmod", fa_kb_crt_poly<0>, dim3(grid), dim3(256), 0, s, P, work, nwork);
+    return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "batch crt launch");
+}
+
 // ------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------
@@ -241,24 +323,10 @@ extern "C" int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh, const u
     fa_thread_state* st = fa_state();
     if (!st) return FA_E_NODEV;
 
-    // prime set (plan.c:356-398): the modulus itself when it is one of the
-    // FFT primes, otherwise the fewest primes with prod >= len * 2^(2 modbits)
     int modbits = nbits_u64(n);
     poly_job J;
-    J.np = 0;
-    J.offset = 0;
-    if (modbits == 50)
-        for (int i = 0; i < FA_NPRIMES; i++)
-            if (T->pr[i].n == n) { J.np = 1; J.offset = i; }
-    if (!J.np) {
-        uint64_t len = (uint64_t)(an < bn ? an : bn);
-        for (int np = 1; np <= FA_NPRIMES && !J.np; np++) {
-            fa_crt_consts C;
-            fa_crt_consts_init(&C, np, 0);
-            if (prod_ge(C, len, 2 * (uint64_t)modbits)) J.np = np;
-        }
-    }
-    if (!J.np) return fa_set_error(FA_E_RANGE, "no prime set for modulus");
+    int rc0 = fa_nmod_primes(T, (uint64_t)(an < bn ? an : bn), n, &J.np, &J.offset);
+    if (rc0) return rc0;
     fa_crt_consts_init(&J.C, J.np, J.offset);
     J.depth = clog2_u64((uint64_t)zn);
     if (J.depth < 4) J.depth = 4;
@@ -301,7 +369,7 @@ extern "C" void _nmod_poly_mul_mid_default_mpn_ctx(uint64_t* res, int64_t zl, in
 }
 
 // _fmpz_vec_max_bits: max bit size of |c|, negated if some c < 0
-static int64_t vec_max_bits(const uint64_t* v, uint64_t w, int64_t n)
+int64_t fa_vec_max_bits(const uint64_t* v, uint64_t w, int64_t n)
 {
     uint64_t mb = 0;
     int neg = 0;
@@ -327,6 +395,38 @@ static int64_t vec_max_bits(const uint64_t* v, uint64_t w, int64_t n)
     return neg ? -(int64_t)mb : (int64_t)mb;
 }
 
+// primes for an fmpz product: the fewest np with prod >= bn * 2^(|b1|+|b2|+1)
+// (fmpz_poly_mul.c:374-379); 0 when eight are not enough
+int fa_fmpz_primes(uint64_t bn, int64_t b1, int64_t b2)
+{
+    uint64_t modbits = (uint64_t)(b1 < 0 ? -b1 : b1) + (uint64_t)(b2 < 0 ? -b2 : b2) + 1;
+    for (int k = 1; k <= FA_NPRIMES; k++) {
+        fa_crt_consts C;
+        fa_crt_consts_init(&C, k, 0);
+        if (prod_ge(C, bn, modbits)) return k;
+    }
+    return 0;
+}
+
+// prime set for an nmod product (plan.c:356-398): the modulus itself when it
+// is one of the FFT primes, otherwise the fewest primes with
+// prod >= len * 2^(2 modbits)
+int fa_nmod_primes(const fa_tables* T, uint64_t len, uint64_t n, int* np, int* offset)
+{
+    const int modbits = nbits_u64(n);
+    *np = 0;
+    *offset = 0;
+    if (modbits == 50)
+        for (int i = 0; i < FA_NPRIMES; i++)
+            if (T->pr[i].n == n) { *np = 1; *offset = i; }
+    for (int k = 1; k <= FA_NPRIMES && !*np; k++) {
+        fa_crt_consts C;
+        fa_crt_consts_init(&C, k, 0);
+        if (prod_ge(C, len, 2 * (uint64_t)modbits)) *np = k;
+    }
+    return *np ? FA_OK : fa_set_error(FA_E_RANGE, "no prime set for modulus");
+}
+
 extern "C" int fa_fmpz_poly_mul_mid(uint64_t* z, uint64_t zw, int64_t zl, int64_t zh,
                                     const uint64_t* a, uint64_t aw, int64_t an, const uint64_t* b,
                                     uint64_t bw, int64_t bn)
@@ -342,16 +442,9 @@ extern "C" int fa_fmpz_poly_mul_mid(uint64_t* z, uint64_t zw, int64_t zl, int64_
         if (zl >= zn) return 1;
         zh = zn;
     }
-    int64_t b1 = vec_max_bits(a, aw, an);
-    int64_t b2 = vec_max_bits(b, bw, bn);
-    uint64_t modbits = (uint64_t)(b1 < 0 ? -b1 : b1) + (uint64_t)(b2 < 0 ? -b2 : b2) + 1;
-    // need prod_of_primes >= bn * 2^modbits (fmpz_poly_mul.c:374-379)
-    int np = 0;
-    for (int k = 1; k <= FA_NPRIMES && !np; k++) {
-        fa_crt_consts C;
-        fa_crt_consts_init(&C, k, 0);
-        if (prod_ge(C, (uint64_t)bn, modbits)) np = k;
-    }
+    int64_t b1 = fa_vec_max_bits(a, aw, an);
+    int64_t b2 = fa_vec_max_bits(b, bw, bn);
+    int np = fa_fmpz_primes((uint64_t)bn, b1, b2);
     if (!np) return 0;
     int depth = clog2_u64((uint64_t)zn);
     if (depth < 4) depth = 4;

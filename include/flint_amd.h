@@ -163,6 +163,52 @@ FA_API int fa_copy_device(void* dst, const void* src, uint64_t nbytes, void* str
    pattern (tiles of 2^lgC columns, 16 rows per thread) and no arithmetic */
 FA_API int fa_probe_colcopy(const void* src, void* dst, int d1, int d2, int lgC, void* stream);
 
+/* ------------------------------------------------------------------------
+   Batches of heterogeneous products (no counterpart in the reference: the
+   reference multiplies one product per call).  Every request is planned as
+   its single-product entry point would plan it (mpn_mul_default_mpn_ctx,
+   _nmod_poly_mul_mid_default_mpn_ctx, _fmpz_poly_mul_mid_default_mpn_ctx)
+   and gives the identical output; the batch runs as one recorded sequence of
+   binned launches, optionally captured into a hipGraph and replayed.
+   ------------------------------------------------------------------------ */
+enum { FA_REQ_MPN = 0, FA_REQ_NMOD = 1, FA_REQ_FMPZ = 2 };
+enum {
+    FA_BATCH_DEVICE = 1,   /* a, b, z are device addresses; results stay in HBM */
+    FA_BATCH_GRAPH = 2     /* capture the launch sequence into a hipGraph */
+};
+
+typedef struct {
+    int32_t kind;          /* FA_REQ_* */
+    int32_t status;        /* out: 1 done, 0 declined (fmpz needing more than 8
+                              primes, as the reference), < 0 an FA_E_* error */
+    const uint64_t* a;
+    const uint64_t* b;
+    int64_t an, bn;        /* limbs (mpn) or coefficients (nmod, fmpz) */
+    int64_t aw, bw;        /* fmpz: limbs per two's complement coefficient */
+    uint64_t* z;           /* mpn: an + bn limbs; nmod: zh - zl values;
+                              fmpz: (zh - zl) * zw limbs */
+    int64_t zl, zh, zw;    /* nmod / fmpz output window and width */
+    fa_nmod_t mod;         /* nmod modulus (nmod_init) */
+    int64_t abits, bbits;  /* fmpz: _fmpz_vec_max_bits of a and b; 0, 0 = measure
+                              (host pointers only) */
+    int32_t np, bits;      /* mpn plan override (0 = choose); out: plan used */
+    int32_t depth, pad;    /* out: transform depth */
+} fa_request;
+
+typedef struct fa_batch_s* fa_batch;
+
+/* plans and lays out a batch; with FA_BATCH_GRAPH records the launches.
+   Statuses are written into the requests.  The request array and, for
+   FA_BATCH_DEVICE, the buffers it names must outlive the batch. */
+FA_API int fa_batch_create(fa_batch* out, fa_request* reqs, int64_t n, int flags);
+/* runs the batch on `stream` (NULL: the batch's own stream).  Host-pointer
+   batches copy in, run and copy out, and return when the outputs are in host
+   memory; device batches given a stream return without waiting. */
+FA_API int fa_batch_launch(fa_batch b, void* stream);
+FA_API void fa_batch_destroy(fa_batch b);
+/* create + launch + destroy for host-pointer requests */
+FA_API int fa_batch_mul(fa_request* reqs, int64_t n, int flags);
+
 #ifdef __cplusplus
 }
 #endif
