@@ -91,6 +91,20 @@ SIGNATURES = {
     "fa_batch_launch": (ctypes.c_int, [_VP, _VP]),
     "fa_batch_destroy": (None, [_VP]),
     "fa_batch_mul": (ctypes.c_int, [_VP, _s64, ctypes.c_int]),
+    "fa_mpn_candidates": (ctypes.c_int, [_s64, _s64, _VP]),
+    "fa_plan_features": (ctypes.c_int, [_s64, _s64, _VP]),
+    "fa_autotune_mpn": (ctypes.c_int, [_s64, _s64, ctypes.c_int, _VP]),
+    "fa_mlp_create": (ctypes.c_int, [ctypes.POINTER(_VP), _u64]),
+    "fa_mlp_destroy": (None, [_VP]),
+    "fa_mlp_param_count": (ctypes.c_int, []),
+    "fa_mlp_get_params": (ctypes.c_int, [_VP, _VP]),
+    "fa_mlp_set_params": (ctypes.c_int, [_VP, _VP]),
+    "fa_mlp_forward": (ctypes.c_int, [_VP, _VP, ctypes.c_int, _VP]),
+    "fa_mlp_loss_grad": (ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int, _VP, _VP]),
+    "fa_mlp_train": (ctypes.c_int, [_VP, _VP, _VP, _VP, _s64, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_float, _VP]),
+    "fa_mlp_plan_mpn": (ctypes.c_int, [_VP, _VP, ctypes.c_int, _VP]),
+    "fa_fmpz_vec_max_bits": (_s64, [_VP, _u64, _s64]),
 }
 
 _lib = None
@@ -248,6 +262,12 @@ def fmpz_poly_mul_mid(a, aw, b, bw, zl, zh, zw):
     return z[: (zh - zl) * zw] if rc == 1 else None
 
 
+def fmpz_vec_max_bits(coeffs, w):
+    """_fmpz_vec_max_bits of w-limb two's complement coefficients"""
+    c = _arr(coeffs)
+    return int(lib().fa_fmpz_vec_max_bits(c.ctypes.data, w, len(c) // w))
+
+
 def batch_mul(requests):
     """Runs a list of host-side products as one batch (fa_batch_mul).
 
@@ -323,6 +343,82 @@ class DeviceBatch:
     def close(self):
         if self.h:
             lib().fa_batch_destroy(self.h)
+            self.h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mpn_candidates(an, bn):
+    """the 64 candidate plans (np, bits) of an x bn limbs; (0, 0) = invalid"""
+    out = np.zeros((64, 2), dtype=np.int32)
+    _check(lib().fa_mpn_candidates(an, bn, out.ctypes.data))
+    return out
+
+
+def plan_features(an, bn):
+    f = np.zeros(16, dtype=np.float32)
+    _check(lib().fa_plan_features(an, bn, f.ctypes.data))
+    return f
+
+
+def autotune_mpn(an, bn, reps=3):
+    """device time (ms) of every candidate plan; -1 where invalid"""
+    ms = np.zeros(64, dtype=np.float32)
+    _check(lib().fa_autotune_mpn(an, bn, reps, ms.ctypes.data))
+    return ms
+
+
+class MLP:
+    """The planner network (fa_mlp_*): 16 -> 64 -> 64 -> 64, FP32 on the GPU."""
+
+    def __init__(self, seed=1):
+        self.h = _VP()
+        _check(lib().fa_mlp_create(ctypes.byref(self.h), seed))
+        self.nparam = lib().fa_mlp_param_count()
+
+    def params(self):
+        p = np.zeros(self.nparam, dtype=np.float32)
+        _check(lib().fa_mlp_get_params(self.h, p.ctypes.data))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().fa_mlp_set_params(self.h, p.ctypes.data))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((len(X), 64), dtype=np.float32)
+        _check(lib().fa_mlp_forward(self.h, X.ctypes.data, len(X), Y.ctypes.data))
+        return Y
+
+    def loss_grad(self, X, T, M):
+        X, T, M = (np.ascontiguousarray(v, dtype=np.float32) for v in (X, T, M))
+        loss = np.zeros(1, dtype=np.float32)
+        g = np.zeros(self.nparam, dtype=np.float32)
+        _check(lib().fa_mlp_loss_grad(self.h, X.ctypes.data, T.ctypes.data, M.ctypes.data,
+                                      len(X), loss.ctypes.data, g.ctypes.data))
+        return float(loss[0]), g
+
+    def train(self, X, T, M, batch=256, epochs=10, lr=1e-3):
+        X, T, M = (np.ascontiguousarray(v, dtype=np.float32) for v in (X, T, M))
+        losses = np.zeros(epochs, dtype=np.float32)
+        _check(lib().fa_mlp_train(self.h, X.ctypes.data, T.ctypes.data, M.ctypes.data, len(X),
+                                  batch, epochs, ctypes.c_float(lr), losses.ctypes.data))
+        return losses
+
+    def plan_mpn(self, shapes):
+        shapes = np.ascontiguousarray(shapes, dtype=np.int64)
+        out = np.zeros((len(shapes), 2), dtype=np.int32)
+        _check(lib().fa_mlp_plan_mpn(self.h, shapes.ctypes.data, len(shapes), out.ctypes.data))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().fa_mlp_destroy(self.h)
             self.h = _VP()
 
     def __del__(self):

@@ -14,6 +14,7 @@
 // The launch sequence is captured once into a hipGraph (FA_BATCH_GRAPH) and
 // replayed by fa_batch_launch.  No memset node is recorded: every output
 // word is written by a kernel (outputs past the product are written as 0).
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -63,8 +64,51 @@ int nbits_u64(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
 
 }  // namespace
 
+// per-thread grow-only buffers lent to one-shot host batches (fa_batch_mul):
+// device memory and pinned staging survive across calls
+namespace {
+struct lent_buffers {
+    char* dmem = nullptr;
+    size_t dsize = 0;
+    char* hin = nullptr;
+    size_t hin_size = 0;
+    char* hout = nullptr;
+    size_t hout_size = 0;
+};
+thread_local lent_buffers lent;
+
+int lend_device(size_t bytes, char** out)
+{
+    if (bytes > lent.dsize) {
+        if (lent.dmem) (void)hipFree(lent.dmem);
+        lent.dmem = nullptr;
+        lent.dsize = 0;
+        if (hipMalloc((void**)&lent.dmem, bytes) != hipSuccess) return FA_E_NOMEM;
+        lent.dsize = bytes;
+    }
+    *out = lent.dmem;
+    return FA_OK;
+}
+
+int lend_pinned(char** buf, size_t* have, size_t bytes, char** out)
+{
+    if (bytes > *have) {
+        if (*buf) (void)hipHostFree(*buf);
+        *buf = nullptr;
+        *have = 0;
+        if (hipHostMalloc((void**)buf, bytes, 0) != hipSuccess) return FA_E_NOMEM;
+        *have = bytes;
+    }
+    *out = *buf;
+    return FA_OK;
+}
+}  // namespace
+
+#define FA_BATCH_LENT 0x100   // internal: borrow the per-thread buffers
+
 struct fa_batch_s {
     int flags;
+    int lent;                          // buffers borrowed from the thread
     int64_t nreq;
     fa_request* reqs;
     hipStream_t stream;
@@ -72,7 +116,12 @@ struct fa_batch_s {
     size_t dsize;
     hipGraph_t graph;
     hipGraphExec_t exec;
-    std::vector<copy_op> h2d, d2h;
+    std::vector<copy_op> h2d, d2h;     // host-pointer batches: (offset, host) pairs
+    char* din;                         // device staging of operands / outputs
+    char* dout;
+    size_t nin, nout;
+    char* hin;                         // pinned host staging
+    char* hout;
     // recorded launch parameters (device addresses inside dmem)
     fa_prime_set S;
     std::vector<std::pair<bin_key, std::pair<size_t, unsigned>>> small;   // key -> (work, n)
@@ -127,7 +176,11 @@ extern "C" void fa_batch_destroy(fa_batch_s* B)
     if (!B) return;
     if (B->exec) (void)hipGraphExecDestroy(B->exec);
     if (B->graph) (void)hipGraphDestroy(B->graph);
-    if (B->dmem) (void)hipFree(B->dmem);
+    if (!B->lent) {
+        if (B->dmem) (void)hipFree(B->dmem);
+        if (B->hin) (void)hipHostFree(B->hin);
+        if (B->hout) (void)hipHostFree(B->hout);
+    }
     if (B->stream) (void)hipStreamDestroy(B->stream);
     delete B;
 }
@@ -230,7 +283,9 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
     fa_prime_set_init(&B->S, T, FA_NPRIMES, maxdepth, 0, 0);
 
     // ---- layout
-    layout L;
+    // two regions: the records, constants and work lists (uploaded), then the
+    // residue rows, staging and carry flags (no initial contents)
+    layout L, Lb, Lin, Lout;
     std::vector<fa_bjob> jobs;
     std::vector<fa_bmpn> mrec;
     std::vector<fa_bpoly> prec;
@@ -245,21 +300,21 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
         offs& o = O[(size_t)r];
         const uint64_t stride = UINT64_C(1) << p.depth;
         if (p.np) {
-            o.A = L.take((size_t)p.np * stride * sizeof(double));
-            if (p.depth > FA_ROW_LG) o.B = L.take((size_t)p.np * stride * sizeof(double));
+            o.A = Lb.take((size_t)p.np * stride * sizeof(double));
+            if (p.depth > FA_ROW_LG) o.B = Lb.take((size_t)p.np * stride * sizeof(double));
         }
         if (!dev) {
             const size_t aw = q.kind == FA_REQ_FMPZ ? (size_t)q.aw : 1;
             const size_t bw = q.kind == FA_REQ_FMPZ ? (size_t)q.bw : 1;
-            o.a = L.take((size_t)q.an * aw * 8);
-            o.b = L.take((size_t)q.bn * bw * 8);
+            o.a = Lin.take((size_t)q.an * aw * 8);
+            o.b = Lin.take((size_t)q.bn * bw * 8);
         }
         size_t zwords = q.kind == FA_REQ_MPN ? (size_t)p.zn
                         : q.kind == FA_REQ_NMOD ? (size_t)p.cnt : (size_t)p.cnt * (size_t)q.zw;
-        if (!dev) o.z = L.take(zwords * 8);
+        if (!dev) o.z = Lout.take(zwords * 8);
         if (q.kind == FA_REQ_MPN) {
-            o.flags = L.take((size_t)p.ngroups + 8);
-            o.agg = L.take((size_t)p.nblk + 8);
+            o.flags = Lb.take((size_t)p.ngroups + 8);
+            o.agg = Lb.take((size_t)p.nblk + 8);
         }
         if (q.kind != FA_REQ_MPN && p.np && !pconsts.count({p.np, p.offset}))
             pconsts[{p.np, p.offset}] = L.take(sizeof(fa_crt_consts));
@@ -269,6 +324,7 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
 
     // job records and work lists (host images first; offsets resolved after
     // the allocation)
+    std::map<int, std::vector<double>> norms;
     std::map<bin_key, std::vector<uint64_t>> small_bins;
     std::map<bin_key, tbin> trans_bins;
     std::map<int, std::vector<uint64_t>> crt_bins;
@@ -293,9 +349,16 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
             j.np = p.np;
             j.offset = p.offset;
             j.depth = p.depth;
-            fa_prime_set Sj;
-            fa_prime_set_init(&Sj, T, p.np, p.depth, p.offset, 1);
-            for (int i = 0; i < p.np; i++) j.norm[i] = Sj.norm[i];
+            // normalizers depend on (np, depth, offset) only
+            const int nkey = (p.np * 32 + p.depth) * 16 + p.offset;
+            auto it = norms.find(nkey);
+            if (it == norms.end()) {
+                fa_prime_set Sj;
+                fa_prime_set_init(&Sj, T, p.np, p.depth, p.offset, 1);
+                std::vector<double> v(Sj.norm, Sj.norm + FA_NPRIMES);
+                it = norms.emplace(nkey, v).first;
+            }
+            for (int i = 0; i < p.np; i++) j.norm[i] = it->second[(size_t)i];
             jobs.push_back(j);
             const bin_key key{p.depth, p.kind};
             if (p.depth <= FA_ROW_LG) {
@@ -381,12 +444,19 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
     B->nfmpz = (unsigned)fmpz_w.size();
     for (int k = 0; k <= FA_NPRIMES; k++) B->fp_consts[k] = fpc[k];
 
-    B->dsize = L.size + 256;
-    if (hipMalloc((void**)&B->dmem, B->dsize) != hipSuccess) {
+    const size_t big = (L.size + 255) & ~(size_t)255;
+    const size_t inb = big + Lb.size, outb = inb + Lin.size;
+    B->dsize = outb + Lout.size + 256;
+    B->lent = (flags & FA_BATCH_LENT) != 0;
+    if (B->lent ? lend_device(B->dsize, &B->dmem) != FA_OK
+                : hipMalloc((void**)&B->dmem, B->dsize) != hipSuccess) {
         fa_batch_destroy(B);
-        return fa_set_error(FA_E_NOMEM, "fa_batch_create: %zu bytes of device memory", L.size);
+        return fa_set_error(FA_E_NOMEM, "fa_batch_create: %zu bytes of device memory", B->dsize);
     }
     char* d = B->dmem;
+    char* db0 = d + big;   // residue rows and carry flags
+    char* din = d + inb;   // staged operands (host-pointer batches)
+    char* dout = d + outb; // staged outputs
 
     // resolve device addresses into the records
     for (int64_t r = 0; r < nreq; r++) {
@@ -394,34 +464,35 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
         plan& p = P[(size_t)r];
         if (!p.live || q.status != 1) continue;
         offs& o = O[(size_t)r];
-        const uint64_t* da = dev ? q.a : (const uint64_t*)(d + o.a);
-        const uint64_t* db = dev ? q.b : (const uint64_t*)(d + o.b);
-        uint64_t* dz = dev ? q.z : (uint64_t*)(d + o.z);
+        const uint64_t* da = dev ? q.a : (const uint64_t*)(din + o.a);
+        const uint64_t* db = dev ? q.b : (const uint64_t*)(din + o.b);
+        uint64_t* dz = dev ? q.z : (uint64_t*)(dout + o.z);
         if (!dev) {
+            // host side of the staging: offsets into the pinned buffers
             const size_t aw = q.kind == FA_REQ_FMPZ ? (size_t)q.aw : 1;
             const size_t bw = q.kind == FA_REQ_FMPZ ? (size_t)q.bw : 1;
-            B->h2d.push_back({(void*)da, q.a, (size_t)q.an * aw * 8});
-            B->h2d.push_back({(void*)db, q.b, (size_t)q.bn * bw * 8});
+            B->h2d.push_back({(void*)(uintptr_t)o.a, q.a, (size_t)q.an * aw * 8});
+            B->h2d.push_back({(void*)(uintptr_t)o.b, q.b, (size_t)q.bn * bw * 8});
             size_t zwords = q.kind == FA_REQ_MPN ? (size_t)p.zn
                             : q.kind == FA_REQ_NMOD ? (size_t)p.cnt : (size_t)p.cnt * (size_t)q.zw;
-            B->d2h.push_back({q.z, dz, zwords * 8});
+            B->d2h.push_back({q.z, (const void*)(uintptr_t)o.z, zwords * 8});
         }
         if (job_of[(size_t)r] >= 0) {
             fa_bjob& j = jobs[(size_t)job_of[(size_t)r]];
             j.a = da;
             j.b = db;
-            j.A = (double*)(d + o.A);
-            j.B = p.depth > FA_ROW_LG ? (double*)(d + o.B) : nullptr;
+            j.A = (double*)(db0 + o.A);
+            j.B = p.depth > FA_ROW_LG ? (double*)(db0 + o.B) : nullptr;
         }
         if (q.kind == FA_REQ_MPN) {
             fa_bmpn& m = mrec[(size_t)rec_of[(size_t)r]];
-            m.Q = (const double*)(d + o.A);
+            m.Q = (const double*)(db0 + o.A);
             m.z = dz;
-            m.flags = (uint8_t*)(d + o.flags);
-            m.agg = (uint8_t*)(d + o.agg);
+            m.flags = (uint8_t*)(db0 + o.flags);
+            m.agg = (uint8_t*)(db0 + o.agg);
         } else {
             fa_bpoly& b = prec[(size_t)rec_of[(size_t)r]];
-            b.Q = p.np ? (const double*)(d + o.A) : nullptr;
+            b.Q = p.np ? (const double*)(db0 + o.A) : nullptr;
             b.C = p.np ? (const void*)(d + pconsts[{p.np, p.offset}]) : nullptr;
             b.z = dz;
         }
@@ -463,10 +534,24 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
         fa_crt_fp_consts_init(&F, C);
         put(fpc[k], &F, sizeof(F));
     }
-    // (the residue rows and staging regions of the image are don't-care)
     if (hipMemcpy(d, img.data(), L.size, hipMemcpyHostToDevice) != hipSuccess) {
         fa_batch_destroy(B);
         return fa_set_error(FA_E_HIP, "fa_batch_create: upload");
+    }
+    B->din = din;
+    B->dout = dout;
+    B->nin = Lin.size;
+    B->nout = Lout.size;
+    if (!dev && (Lin.size || Lout.size)) {
+        const bool ok =
+            B->lent ? (lend_pinned(&lent.hin, &lent.hin_size, Lin.size + 256, &B->hin) == FA_OK &&
+                       lend_pinned(&lent.hout, &lent.hout_size, Lout.size + 256, &B->hout) == FA_OK)
+                    : (hipHostMalloc((void**)&B->hin, Lin.size + 256, 0) == hipSuccess &&
+                       hipHostMalloc((void**)&B->hout, Lout.size + 256, 0) == hipSuccess);
+        if (!ok) {
+            fa_batch_destroy(B);
+            return fa_set_error(FA_E_NOMEM, "fa_batch_create: pinned staging");
+        }
     }
 
     if (flags & FA_BATCH_GRAPH) {
@@ -499,9 +584,12 @@ extern "C" int fa_batch_launch(fa_batch_s* B, void* stream)
 {
     if (!B) return fa_set_error(FA_E_ARG, "fa_batch_launch: null batch");
     hipStream_t s = stream ? (hipStream_t)stream : B->stream;
-    for (auto& c : B->h2d)
-        if (hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, s) != hipSuccess)
+    if (!B->h2d.empty()) {
+        // operands gathered into pinned memory, one copy up
+        for (auto& c : B->h2d) memcpy(B->hin + (uintptr_t)c.dst, c.src, c.bytes);
+        if (hipMemcpyAsync(B->din, B->hin, B->nin, hipMemcpyHostToDevice, s) != hipSuccess)
             return fa_set_error(FA_E_HIP, "fa_batch_launch: upload");
+    }
     int rc = FA_OK;
     if (B->exec) {
         if (hipGraphLaunch(B->exec, s) != hipSuccess)
@@ -510,21 +598,22 @@ extern "C" int fa_batch_launch(fa_batch_s* B, void* stream)
         rc = batch_record(B, s);
         if (rc) return rc;
     }
-    for (auto& c : B->d2h)
-        if (hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, s) != hipSuccess)
-            return fa_set_error(FA_E_HIP, "fa_batch_launch: download");
+    if (!B->d2h.empty() &&
+        hipMemcpyAsync(B->hout, B->dout, B->nout, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return fa_set_error(FA_E_HIP, "fa_batch_launch: download");
     if (!(B->flags & FA_BATCH_DEVICE) || !stream) {
         if (hipStreamSynchronize(s) != hipSuccess)
             return fa_set_error(FA_E_HIP, "fa_batch_launch: %s",
                                 hipGetErrorString(hipGetLastError()));
     }
+    for (auto& c : B->d2h) memcpy(c.dst, B->hout + (uintptr_t)c.src, c.bytes);
     return FA_OK;
 }
 
 extern "C" int fa_batch_mul(fa_request* R, int64_t nreq, int flags)
 {
     fa_batch_s* B = nullptr;
-    int rc = fa_batch_create(&B, R, nreq, flags & ~FA_BATCH_DEVICE);
+    int rc = fa_batch_create(&B, R, nreq, (flags & ~FA_BATCH_DEVICE & ~FA_BATCH_GRAPH) | FA_BATCH_LENT);
     if (rc) return rc;
     rc = fa_batch_launch(B, nullptr);
     fa_batch_destroy(B);

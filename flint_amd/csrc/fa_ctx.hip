@@ -578,6 +578,25 @@ int fa_mpn_choose_plan(uint64_t an, uint64_t bn, fa_mpn_plan_t* P)
     return FA_OK;
 }
 
+// largest chunk size (a multiple of 8 up to 200) that the plan constraints
+// allow for np primes, or 0
+int fa_mpn_max_bits(uint64_t an, uint64_t bn, int np)
+{
+    if (an == 0 || bn == 0 || np < 2 || np > FA_NPRIMES) return 0;
+    fa_crt_consts C;
+    fa_crt_consts_init(&C, np, 0);
+    for (int bits = 200; bits >= 8; bits -= 8) {
+        if (9 * bits < 64 * C.nl + 64) break;
+        uint64_t alen = cdiv_u64(64 * an, bits), blen = cdiv_u64(64 * bn, bits);
+        uint64_t mlen = alen < blen ? alen : blen;
+        if (!ml_ge_ui_2exp(C.M, mlen, 2 * (uint64_t)bits)) continue;
+        int depth = clog2_u64(alen + blen - 1);
+        if (depth > FA_MAX_DEPTH) return 0;
+        return bits;
+    }
+    return 0;
+}
+
 // a caller-chosen (np, bits) checked against the same exactness and
 // geometry constraints as the automatic choice
 int fa_mpn_fixed_plan(uint64_t an, uint64_t bn, int np, int bits, fa_mpn_plan_t* P)

@@ -85,6 +85,7 @@ void fa_crt_consts_init(fa_crt_consts* C, int np, int offset);
 uint64_t fa_crt_cop(int np, int i);         // (M/p_i) mod p_i
 int fa_mpn_choose_plan(uint64_t an, uint64_t bn, fa_mpn_plan_t* P);
 int fa_mpn_fixed_plan(uint64_t an, uint64_t bn, int np, int bits, fa_mpn_plan_t* P);
+int fa_mpn_max_bits(uint64_t an, uint64_t bn, int np);
 void fa_prime_set_init(fa_prime_set* S, const fa_tables* T, int np, int depth, int offset,
                        int with_norm);
 uint64_t fa_mulmod_u64(uint64_t a, uint64_t b, uint64_t n);

@@ -395,6 +395,12 @@ int64_t fa_vec_max_bits(const uint64_t* v, uint64_t w, int64_t n)
     return neg ? -(int64_t)mb : (int64_t)mb;
 }
 
+// _fmpz_vec_max_bits over w-limb two's complement coefficients
+extern "C" int64_t fa_fmpz_vec_max_bits(const uint64_t* v, uint64_t w, int64_t n)
+{
+    return (v && w && n > 0) ? fa_vec_max_bits(v, w, n) : 0;
+}
+
 // primes for an fmpz product: the fewest np with prod >= bn * 2^(|b1|+|b2|+1)
 // (fmpz_poly_mul.c:374-379); 0 when eight are not enough
 int fa_fmpz_primes(uint64_t bn, int64_t b1, int64_t b2)

@@ -1,0 +1,158 @@
+// fa_tune.hip -- the plan space of integer products, plan features for the
+// learned planner, and the exhaustive autotuner that times every plan.
+//
+// Candidate j < 64 of a product of an x bn limbs is (np, bits) with
+// np = j / 8 + 1 and bits = bmax(np) - 8 (j mod 8), where bmax(np) is the
+// largest chunk size np primes can carry exactly for this shape
+// (fa_mpn_max_bits); np = 1 and sizes the constraints reject are invalid.
+// The automatic plan (fa_mpn_choose_plan) is always candidate
+// (np - 1) * 8 for its np.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "fa_internal.h"
+#include "../../include/flint_amd.h"
+
+#define FA_NCAND 64
+
+extern "C" int fa_mpn_candidates(int64_t an, int64_t bn, int32_t* out)
+{
+    if (an <= 0 || bn <= 0 || !out) return fa_set_error(FA_E_ARG, "fa_mpn_candidates: bad arguments");
+    for (int j = 0; j < FA_NCAND; j++) {
+        const int np = j / 8 + 1;
+        int bits = 0;
+        if (np >= 2) {
+            const int bmax = fa_mpn_max_bits((uint64_t)an, (uint64_t)bn, np);
+            bits = bmax ? bmax - 8 * (j % 8) : 0;
+            if (bits > 0) {
+                fa_mpn_plan_t P;
+                // silent validity check: the same constraints as fa_mpn_fixed_plan
+                fa_crt_consts C;
+                fa_crt_consts_init(&C, np, 0);
+                if (9 * bits < 64 * C.nl + 64) bits = 0;
+                else if (fa_mpn_fixed_plan((uint64_t)an, (uint64_t)bn, np, bits, &P) != FA_OK) bits = 0;
+            } else {
+                bits = 0;
+            }
+        }
+        out[2 * j] = bits ? np : 0;
+        out[2 * j + 1] = bits;
+    }
+    return FA_OK;
+}
+
+// 16 features of a product shape for the planner
+extern "C" int fa_plan_features(int64_t an, int64_t bn, float* f)
+{
+    if (an <= 0 || bn <= 0 || !f) return fa_set_error(FA_E_ARG, "fa_plan_features: bad arguments");
+    const double la = log2((double)an), lb = log2((double)bn);
+    fa_mpn_plan_t P;
+    memset(&P, 0, sizeof(P));
+    if (fa_mpn_choose_plan((uint64_t)an, (uint64_t)bn, &P) != FA_OK) memset(&P, 0, sizeof(P));
+    f[0] = (float)(la / 24);
+    f[1] = (float)(lb / 24);
+    f[2] = (float)(fmin(la, lb) / 24);
+    f[3] = (float)(fmax(la, lb) / 24);
+    f[4] = (float)(log2((double)(an + bn)) / 25);
+    f[5] = (float)((la + lb) / 48);
+    f[6] = (float)(P.np / 8.0);
+    f[7] = (float)(P.bits / 200.0);
+    f[8] = (float)(P.depth / 24.0);
+    f[9] = P.depth ? (float)((double)P.zlen / (double)(UINT64_C(1) << P.depth)) : 0.f;
+    for (int np = 3; np <= 8; np++)
+        f[7 + np] = (float)(fa_mpn_max_bits((uint64_t)an, (uint64_t)bn, np) / 200.0);
+    return FA_OK;
+}
+
+// device buffers reused by the tuner (per host thread)
+static thread_local uint64_t* tune_buf = nullptr;
+static thread_local size_t tune_words = 0;
+
+// times every candidate of an x bn on the device alone: after a warm-up
+// call, K back-to-back products (K = 2^21 / (an + bn), between 1 and 64)
+// are captured into one hipGraph, and each of `reps` replays is timed with
+// HIP events; ms[j] = the mean time per product, -1 for invalid candidates.
+// Host launch overhead is outside the measurement.
+extern "C" int fa_autotune_mpn(int64_t an, int64_t bn, int reps, float* ms)
+{
+    if (an <= 0 || bn <= 0 || reps <= 0 || !ms)
+        return fa_set_error(FA_E_ARG, "fa_autotune_mpn: bad arguments");
+    int32_t cand[2 * FA_NCAND];
+    int rc = fa_mpn_candidates(an, bn, cand);
+    if (rc) return rc;
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    const size_t words = 2 * (size_t)(an + bn);
+    if (words > tune_words) {
+        if (tune_buf) (void)hipFree(tune_buf);
+        tune_buf = nullptr;
+        tune_words = 0;
+        if (hipMalloc((void**)&tune_buf, words * 8) != hipSuccess)
+            return fa_set_error(FA_E_NOMEM, "fa_autotune_mpn: %zu words", words);
+        tune_words = words;
+        (void)hipMemsetAsync(tune_buf, 0xA5, words * 8, st->stream);
+    }
+    uint64_t* a = tune_buf;
+    uint64_t* b = a + an;
+    uint64_t* z = b + bn;
+    int64_t K = (INT64_C(1) << 21) / (an + bn);
+    if (K < 1) K = 1;
+    if (K > 64) K = 64;
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+        return fa_set_error(FA_E_HIP, "fa_autotune_mpn: stream");
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        (void)hipStreamDestroy(cs);
+        return fa_set_error(FA_E_HIP, "fa_autotune_mpn: events");
+    }
+    for (int j = 0; j < FA_NCAND && !rc; j++) {
+        ms[j] = -1.f;
+        if (!cand[2 * j]) continue;
+        const int np = cand[2 * j], bits = cand[2 * j + 1];
+        // warm-up outside the capture: twiddle tables and scratch reach size
+        rc = fa_mpn_mul_device_plan(z, a, an, b, bn, np, bits, cs);
+        if (rc || hipStreamSynchronize(cs) != hipSuccess) {
+            rc = rc ? rc : fa_set_error(FA_E_HIP, "tune warm-up");
+            break;
+        }
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            rc = fa_set_error(FA_E_HIP, "tune capture");
+            break;
+        }
+        for (int64_t k = 0; k < K && !rc; k++)
+            rc = fa_mpn_mul_device_plan(z, a, an, b, bn, np, bits, cs);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(cs, &g);
+        hipGraphExec_t ex = nullptr;
+        if (!rc && (ce != hipSuccess || !g ||
+                    hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess))
+            rc = fa_set_error(FA_E_HIP, "tune graph");
+        float tot = 0.f;
+        if (!rc && hipGraphLaunch(ex, cs) != hipSuccess) rc = fa_set_error(FA_E_HIP, "tune launch");
+        for (int r = 0; r < reps && !rc; r++) {
+            (void)hipEventRecord(e0, cs);
+            if (hipGraphLaunch(ex, cs) != hipSuccess) {
+                rc = fa_set_error(FA_E_HIP, "tune launch");
+                break;
+            }
+            (void)hipEventRecord(e1, cs);
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                rc = fa_set_error(FA_E_HIP, "tune sync");
+                break;
+            }
+            float t = 0;
+            (void)hipEventElapsedTime(&t, e0, e1);
+            tot += t;
+        }
+        if (ex) (void)hipGraphExecDestroy(ex);
+        if (g) (void)hipGraphDestroy(g);
+        if (!rc) ms[j] = tot / (float)(reps * K);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(cs);
+    (void)hipStreamDestroy(cs);
+    return rc;
+}

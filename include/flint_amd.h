@@ -195,6 +195,10 @@ typedef struct {
     int32_t depth, pad;    /* out: transform depth */
 } fa_request;
 
+/* _fmpz_vec_max_bits (fmpz_vec.h) of n w-limb two's complement
+   coefficients: the largest bit size of |c|, negated if some c < 0 */
+FA_API int64_t fa_fmpz_vec_max_bits(const uint64_t* v, uint64_t w, int64_t n);
+
 typedef struct fa_batch_s* fa_batch;
 
 /* plans and lays out a batch; with FA_BATCH_GRAPH records the launches.
@@ -208,6 +212,38 @@ FA_API int fa_batch_launch(fa_batch b, void* stream);
 FA_API void fa_batch_destroy(fa_batch b);
 /* create + launch + destroy for host-pointer requests */
 FA_API int fa_batch_mul(fa_request* reqs, int64_t n, int flags);
+
+/* ------------------------------------------------------------------------
+   Plan space, autotuner and learned planner for integer products (no
+   counterpart in the reference, whose fixed rule is mpn_ctx_best_profile,
+   mpn_mul.c:1248-1340).  Candidate j < 64 of an x bn limbs is
+   (np, bits) = (j / 8 + 1, bmax(np) - 8 (j mod 8)), bmax(np) the largest
+   chunk size np primes carry exactly; invalid candidates are (0, 0).
+   ------------------------------------------------------------------------ */
+FA_API int fa_mpn_candidates(int64_t an, int64_t bn, int32_t* out /* 64 x {np, bits} */);
+/* 16 planner features of a shape (log sizes, the automatic plan, bmax) */
+FA_API int fa_plan_features(int64_t an, int64_t bn, float* f16);
+/* exhaustive autotuner: device time (ms) per product of every candidate,
+   K back-to-back products captured in one hipGraph and replayed `reps`
+   times (host launch overhead excluded); -1 for invalid candidates */
+FA_API int fa_autotune_mpn(int64_t an, int64_t bn, int reps, float* ms64);
+
+/* FP32 MLP 16 -> 64 -> 64 -> 64 predicting log(ms) of the 64 candidates */
+typedef struct fa_mlp_s* fa_mlp;
+FA_API int fa_mlp_create(fa_mlp* out, uint64_t seed);
+FA_API void fa_mlp_destroy(fa_mlp m);
+FA_API int fa_mlp_param_count(void);
+FA_API int fa_mlp_get_params(fa_mlp m, float* host);
+FA_API int fa_mlp_set_params(fa_mlp m, const float* host);
+FA_API int fa_mlp_forward(fa_mlp m, const float* X, int B, float* Y);
+/* loss (masked mean square error) and its gradient at the current parameters */
+FA_API int fa_mlp_loss_grad(fa_mlp m, const float* X, const float* T, const float* M, int B,
+                            float* loss, float* grad);
+/* minibatch Adam on the device; loss_out[e] = mean loss of epoch e */
+FA_API int fa_mlp_train(fa_mlp m, const float* X, const float* T, const float* M, int64_t N,
+                        int batch, int epochs, float lr, float* loss_out);
+/* learned plans for n shapes (an, bn): out = n x {np, bits} */
+FA_API int fa_mlp_plan_mpn(fa_mlp m, const int64_t* shapes, int n, int32_t* out);
 
 #ifdef __cplusplus
 }
