@@ -255,7 +255,7 @@ def main():
     ap.add_argument("--no-planner", action="store_true",
                     help="batched workloads: automatic plans instead of the learned planner")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-log2", type=int, default=22)
+    ap.add_argument("--cpu-sample-log2", type=int, default=24)
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_mpn_2p24.json"))
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

@@ -368,13 +368,19 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
                 auto& tb = trans_bins[key];
                 const int ng = fa_batch_col_groups(p.depth);
                 const int nrow = 1 << (p.depth - FA_ROW_LG);
+                // the two column groups of a 128-byte line adjacent, then
+                // the primes (the order of the single-product kernels)
                 for (int z = 0; z < 2; z++)
-                    for (int g = 0; g < ng; g++)
-                        for (int i = 0; i < p.np; i++) tb.fwd.push_back(FA_WORK(jid, i, z, g));
+                    for (int h = 0; h < ng / 2; h++)
+                        for (int i = 0; i < p.np; i++)
+                            for (int lo = 0; lo < 2; lo++)
+                                tb.fwd.push_back(FA_WORK(jid, i, z, 2 * h + lo));
                 for (int i = 0; i < p.np; i++)
                     for (int row = 0; row < nrow; row++) tb.row.push_back(FA_WORK(jid, i, 0, row));
-                for (int g = 0; g < ng; g++)
-                    for (int i = 0; i < p.np; i++) tb.inv.push_back(FA_WORK(jid, i, 0, g));
+                for (int h = 0; h < ng / 2; h++)
+                    for (int i = 0; i < p.np; i++)
+                        for (int lo = 0; lo < 2; lo++)
+                            tb.inv.push_back(FA_WORK(jid, i, 0, 2 * h + lo));
             }
         }
         if (q.kind == FA_REQ_MPN) {

@@ -415,6 +415,22 @@ FA_DEV void fa_col_inv_tile(double* __restrict__ dst, int d2, int g, double n, d
     for (int k = 0; k < G::NEL; k++) dst[((uint64_t)G::pos(0, tc, k) << d2) + col] = x[k];
 }
 
+// work index L -> (prime i, column group g, operand z) with the two column
+// groups of one 128-byte line (g = 2h, 2h + 1) adjacent, then the primes,
+// then the line pairs: concurrent workgroups of an XCD share every line they
+// read or write, and the primes of one group follow each other (the limb
+// reads of the import are shared too).  ngroups is even.
+FA_DEV void fa_col_split(unsigned L, int np, int ngroups, int* i, int* g, int* z)
+{
+    const int lo = L & 1;
+    const unsigned L2 = L >> 1;
+    *i = (int)(L2 % (unsigned)np);
+    const unsigned rest = L2 / (unsigned)np;
+    const int half = ngroups >> 1;
+    *g = 2 * (int)(rest % (unsigned)half) + lo;
+    *z = (int)(rest / (unsigned)half);
+}
+
 template <int D1, int KIND>
 __global__ void __launch_bounds__(1024) fa_k_col_fwd16(fa_operand a, fa_operand b, int d2,
                                                        double* __restrict__ A,
@@ -423,11 +439,8 @@ __global__ void __launch_bounds__(1024) fa_k_col_fwd16(fa_operand a, fa_operand 
                                                        int nops)
 {
     __shared__ double lds[col_geom<D1>::LDS];
-    const unsigned L = fa_xcd_linear();
-    const int i = L % np;
-    const unsigned rest = L / np;
-    const int g = rest % ngroups;
-    const int z = rest / ngroups;
+    int i, g, z;
+    fa_col_split(fa_xcd_linear(), np, ngroups, &i, &g, &z);
     if (z >= nops) return;
     fa_col_fwd_tile<D1, KIND>(z ? b : a, (z ? B : A) + (uint64_t)i * stride, d2, g, S.p[i],
                               S.pinv[i], S.w2[i], S.pow2[i], lds);
@@ -439,10 +452,9 @@ __global__ void __launch_bounds__(1024) fa_k_col_inv16(double* __restrict__ A, i
                                                        int ngroups)
 {
     __shared__ double lds[col_geom<D1>::LDS];
-    const unsigned L = fa_xcd_linear();
-    const int i = L % np;
-    const int g = L / np;
-    if (g >= ngroups) return;
+    int i, g, z;
+    fa_col_split(fa_xcd_linear(), np, ngroups, &i, &g, &z);
+    if (z >= 1) return;
     fa_col_inv_tile<D1>(A + (uint64_t)i * stride, d2, g, S.p[i], S.pinv[i], S.w2inv[i], lds);
 }
 
@@ -609,11 +621,8 @@ __global__ void __launch_bounds__(1024) fa_k_col_fwd16w(fa_operand a, fa_operand
                                                         int nops)
 {
     __shared__ double lds[FA_C8_LDS];
-    const unsigned L = fa_xcd_linear();
-    const int i = L % np;
-    const unsigned rest = L / np;
-    const int g = rest % ngroups;
-    const int z = rest / ngroups;
+    int i, g, z;
+    fa_col_split(fa_xcd_linear(), np, ngroups, &i, &g, &z);
     if (z >= nops) return;
     fa_col_fwd8_tile<KIND>(z ? b : a, (z ? B : A) + (uint64_t)i * stride, d2, g, S.p[i], S.pinv[i],
                            S.w2[i], S.pow2[i], lds);
@@ -624,10 +633,9 @@ __global__ void __launch_bounds__(1024) fa_k_col_inv16w(double* __restrict__ A, 
                                                         int ngroups)
 {
     __shared__ double lds[FA_C8_LDS];
-    const unsigned L = fa_xcd_linear();
-    const int i = L % np;
-    const int g = L / np;
-    if (g >= ngroups) return;
+    int i, g, z;
+    fa_col_split(fa_xcd_linear(), np, ngroups, &i, &g, &z);
+    if (z >= 1) return;
     fa_col_inv8_tile(A + (uint64_t)i * stride, d2, g, S.p[i], S.pinv[i], S.w2inv[i], lds);
 }
 
