@@ -32,6 +32,26 @@ FA_DEV void fa_fwd16(double* x, const double* t, double n, double ninv)
                 t[5 + 3 * j], n, ninv);
 }
 
+// fa_fwd16 for inputs whose upper half x[8..15] is zero: the first radix-4
+// layer needs only the products by x[k+4]
+FA_DEV void fa_fwd16_hz(double* x, const double* t, double n, double ninv)
+{
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double a0 = fa_reduce_pm1n(x[k], n, ninv);
+        const double v1 = fa_mulmod(x[k + 4], t[1], n, ninv);
+        const double v3 = fa_mulmod(x[k + 4], t[2], n, ninv);
+        x[k] = a0 + v1;
+        x[k + 4] = a0 - v1;
+        x[k + 8] = a0 + v3;
+        x[k + 12] = a0 - v3;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        fa_fwd4(x[4 * j], x[4 * j + 1], x[4 * j + 2], x[4 * j + 3], t[3 + 3 * j], t[4 + 3 * j],
+                t[5 + 3 * j], n, ninv);
+}
+
 FA_DEV void fa_inv16(double* x, const double* t, double n, double ninv)
 {
 #pragma unroll
@@ -490,16 +510,34 @@ FA_DEV void fa_col_fwd8_tile(const fa_operand& op, double* __restrict__ dst, int
     const int c = threadIdx.x & 7, u = threadIdx.x >> 3;
     const uint64_t col = (uint64_t)g * 8 + c;
     double x0[16], x1[16], tw[15];
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        x0[k] = fa_operand_value_k<KIND>(op, ((uint64_t)(u + 256 * k) << d2) + col, pow2, n, ninv);
-        x1[k] = fa_operand_value_k<KIND>(op, ((uint64_t)(u + 128 + 256 * k) << d2) + col, pow2, n,
-                                         ninv);
-    }
-    // stage 0: levels 0..3, block 0
+    // rows >= 2048 (elements >= 2^(d2+11)) all zero: the usual case of a
+    // product's operands, each at most half the transform length
+    const bool hz = fa_operand_len(op) <= (UINT64_C(1) << (d2 + 11));
     fa_tw16(w2, 0, tw);
-    fa_fwd16(x0, tw, n, ninv);
-    fa_fwd16(x1, tw, n, ninv);
+    if (hz) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            x0[k] = fa_operand_value_k<KIND>(op, ((uint64_t)(u + 256 * k) << d2) + col, pow2, n,
+                                             ninv);
+            x1[k] = fa_operand_value_k<KIND>(op, ((uint64_t)(u + 128 + 256 * k) << d2) + col,
+                                             pow2, n, ninv);
+        }
+#pragma unroll
+        for (int k = 8; k < 16; k++) x0[k] = x1[k] = 0.0;
+        // stage 0: levels 0..3, block 0
+        fa_fwd16_hz(x0, tw, n, ninv);
+        fa_fwd16_hz(x1, tw, n, ninv);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            x0[k] = fa_operand_value_k<KIND>(op, ((uint64_t)(u + 256 * k) << d2) + col, pow2, n,
+                                             ninv);
+            x1[k] = fa_operand_value_k<KIND>(op, ((uint64_t)(u + 128 + 256 * k) << d2) + col,
+                                             pow2, n, ninv);
+        }
+        fa_fwd16(x0, tw, n, ninv);
+        fa_fwd16(x1, tw, n, ninv);
+    }
     // exchange 0 -> 1 in two halves
     double y0[16], y1[16];
 #pragma unroll

@@ -167,6 +167,13 @@ FA_DEV double fa_operand_value(const fa_operand& op, uint64_t x,
     return fa_coeff_residue_s(op.src + x * (uint64_t)op.width, op.width, pow2, n, ninv);
 }
 
+// number of elements of an operand (chunks of a limb array, coefficients)
+FA_DEV uint64_t fa_operand_len(const fa_operand& op)
+{
+    if (op.kind == FA_PACK_MPN) return (64 * op.n + (uint64_t)op.bits - 1) / (uint64_t)op.bits;
+    return op.n;
+}
+
 // import specialized at compile time (KIND = FA_PACK_*), FA_PACK_ANY
 // dispatching on op.kind
 template <int KIND>

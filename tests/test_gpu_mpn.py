@@ -105,7 +105,8 @@ def test_config_checksums(name):
 @pytest.mark.parametrize("an,bn,np_,bits", [(65536, 65536, 6, 136), (60000, 50000, 6, 136),
                                            (1 << 20, 1 << 20, 6, 136), (1000000, 3, 4, 88),
                                            (700000, 650000, 8, 184), (3 << 20, 2 << 20, 6, 136),
-                                           (30000, 30000, 3, 64), (5000, 4000, 2, 40)])
+                                           (30000, 30000, 3, 64), (5000, 4000, 2, 40),
+                                           (20000000, 100, 6, 136)])
 def test_fast_kernels_match_generic(an, bn, np_, bits):
     """the register radix-16 kernels and the generic LDS kernels give the
     same product (and both the exact one)"""
