@@ -105,15 +105,25 @@ def cpu_baseline(W, sample_log2):
     }
 
 
+def _mpn_ws(an, bn, fa):
+    p = fa.mpn_plan(an, bn)
+    return 16 * p["np"] * (1 << p["depth"]) + 16 * (an + bn)
+
+
 def _plan_ws(q, fa):
-    """device workspace of a request (residue rows), for chunking a stream"""
+    """device workspace of a request (residue rows, KS scratch), for chunking
+    a stream"""
     if q["kind"] == "mpn":
-        p = fa.mpn_plan(len(q["a"]), len(q["b"]))
-        return 16 * p["np"] * (1 << p["depth"]) + 16 * (len(q["a"]) + len(q["b"]))
+        return _mpn_ws(len(q["a"]), len(q["b"]), fa)
     an = len(q["a"]) // q.get("aw", 1)
     bn = len(q["b"]) // q.get("bw", 1)
     depth = max(4, int(np.ceil(np.log2(an + bn))))
-    return 16 * 8 * (1 << depth) + 8 * (an + bn) * q.get("zw", 1)
+    ws = 16 * 8 * (1 << depth) + 8 * (an + bn) * q.get("zw", 1)
+    if q["kind"] == "fmpz":
+        sb = abs(q["abits"]) + abs(q["bbits"]) + int(min(an, bn)).bit_length() + 1
+        la, lb = -(-an * sb // 64), -(-bn * sb // 64)
+        ws += 3 * _mpn_ws(la, lb, fa) + 8 * 6 * (la + lb)
+    return ws
 
 
 def batched_main(args, torch, fa):
@@ -140,8 +150,11 @@ def batched_main(args, torch, fa):
         elif q["kind"] == "nmod":
             plain_out.append(fa.nmod_poly_mul_mid(q["a"], q["b"], q["zl"], q["zh"], q["n"]))
         else:
-            plain_out.append(fa.fmpz_poly_mul_mid(q["a"], q["aw"], q["b"], q["bw"], q["zl"],
-                                                  q["zh"], q["zw"]))
+            z = fa.fmpz_poly_mul_mid(q["a"], q["aw"], q["b"], q["bw"], q["zl"], q["zh"], q["zw"])
+            if z is None:   # beyond eight primes: Kronecker substitution, as _fmpz_poly_mul
+                z = fa.fmpz_poly_mul_ks(q["a"], q["aw"], q["b"], q["bw"], q["zl"], q["zh"],
+                                        q["zw"])
+            plain_out.append(z)
     t_plain = time.perf_counter() - tp0
     done = [k for k, z in enumerate(plain_out) if z is not None]
     lp_done = sum(reqs[k]["lp"] for k in done)
@@ -162,11 +175,11 @@ def batched_main(args, torch, fa):
     # host-pointer batches: one untimed pass sizes the per-thread device and
     # pinned buffers the one-shot batch borrows, then the timed pass
     for ch in chunks[:1]:
-        fa.batch_mul([reqs_p[k] for k in ch])
+        fa.batch_mul([reqs_p[k] for k in ch], ks=True)
     th0 = time.perf_counter()
     batch_out = {}
     for ch in chunks:
-        res = fa.batch_mul([reqs_p[k] for k in ch])
+        res = fa.batch_mul([reqs_p[k] for k in ch], ks=True)
         for k, (st, z) in zip(ch, res):
             batch_out[k] = (st, z)
     t_host = time.perf_counter() - th0

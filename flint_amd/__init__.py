@@ -44,7 +44,7 @@ class Request(ctypes.Structure):
 
 
 REQ_MPN, REQ_NMOD, REQ_FMPZ = 0, 1, 2
-BATCH_DEVICE, BATCH_GRAPH = 1, 2
+BATCH_DEVICE, BATCH_GRAPH, BATCH_FMPZ_KS = 1, 2, 4
 
 
 class SdFftCtx(ctypes.Structure):
@@ -105,6 +105,8 @@ SIGNATURES = {
                                     ctypes.c_float, _VP]),
     "fa_mlp_plan_mpn": (ctypes.c_int, [_VP, _VP, ctypes.c_int, _VP]),
     "fa_fmpz_vec_max_bits": (_s64, [_VP, _u64, _s64]),
+    "fa_fmpz_poly_mul_ks": (ctypes.c_int, [_P64, _u64, _s64, _s64, _P64, _u64, _s64, _P64, _u64,
+                                           _s64]),
 }
 
 _lib = None
@@ -262,20 +264,32 @@ def fmpz_poly_mul_mid(a, aw, b, bw, zl, zh, zw):
     return z[: (zh - zl) * zw] if rc == 1 else None
 
 
+def fmpz_poly_mul_ks(a, aw, b, bw, zl, zh, zw):
+    """fmpz product by Kronecker substitution onto the integer product: the
+    route taken when fmpz_poly_mul_mid declines; always computes"""
+    a = _arr(a)
+    b = _arr(b)
+    z = np.zeros(max((zh - zl) * zw, 1), dtype=np.uint64)
+    _check(lib().fa_fmpz_poly_mul_ks(_p(z), zw, zl, zh, _p(a), aw, len(a) // aw, _p(b), bw,
+                                     len(b) // bw))
+    return z[: (zh - zl) * zw]
+
+
 def fmpz_vec_max_bits(coeffs, w):
     """_fmpz_vec_max_bits of w-limb two's complement coefficients"""
     c = _arr(coeffs)
     return int(lib().fa_fmpz_vec_max_bits(c.ctypes.data, w, len(c) // w))
 
 
-def batch_mul(requests):
+def batch_mul(requests, ks=False):
     """Runs a list of host-side products as one batch (fa_batch_mul).
 
     Each request is a dict: {"kind": "mpn", "a": limbs, "b": limbs},
     {"kind": "nmod", "a", "b", "n", "zl", "zh"} or
     {"kind": "fmpz", "a", "aw", "b", "bw", "zl", "zh", "zw"} (coefficients as
     flat w-limb two's complement arrays).  Returns a list of (status, output)
-    with output None unless status == 1."""
+    with output None unless status == 1.  ks=True computes the fmpz products
+    the eight-prime rule declines by Kronecker substitution."""
     n = len(requests)
     R = (Request * max(n, 1))()
     keep = []
@@ -311,7 +325,7 @@ def batch_mul(requests):
         r.b = b.ctypes.data
         r.z = z.ctypes.data
         outs.append(z)
-    _check(lib().fa_batch_mul(ctypes.cast(R, _VP), n, 0))
+    _check(lib().fa_batch_mul(ctypes.cast(R, _VP), n, BATCH_FMPZ_KS if ks else 0))
     res = []
     for k, q in enumerate(requests):
         st = R[k].status
@@ -330,10 +344,10 @@ class DeviceBatch:
     `reqs` is a ctypes array of Request whose a, b, z are device addresses;
     it must stay alive as long as the batch."""
 
-    def __init__(self, reqs, graph=True):
+    def __init__(self, reqs, graph=True, ks=False):
         self.reqs = reqs
         self.h = _VP()
-        flags = BATCH_DEVICE | (BATCH_GRAPH if graph else 0)
+        flags = BATCH_DEVICE | (BATCH_GRAPH if graph else 0) | (BATCH_FMPZ_KS if ks else 0)
         _check(lib().fa_batch_create(ctypes.byref(self.h), ctypes.cast(reqs, _VP), len(reqs),
                                      flags))
 

@@ -139,6 +139,16 @@ struct fa_batch_s {
     size_t poly_rec;
     size_t nmod_work, fmpz_work;
     unsigned nnmod, nfmpz;
+    // fmpz products routed through Kronecker substitution: packing before
+    // the transforms, recombination after the integer CRT (device addresses)
+    struct ksrec {
+        const uint64_t* a;
+        const uint64_t* b;
+        uint64_t* z;
+        uint64_t* scratch;
+        uint64_t aw, an, bw, bn, sb, zw, zl, cnt;
+    };
+    std::vector<ksrec> ks;
 };
 
 static int batch_record(fa_batch_s* B, hipStream_t s)
@@ -146,6 +156,10 @@ static int batch_record(fa_batch_s* B, hipStream_t s)
     char* d = B->dmem;
     const fa_bjob* J = (const fa_bjob*)(d + B->jobs);
     int rc;
+    for (auto& k : B->ks) {
+        rc = fa_ks_pre(k.a, k.aw, k.an, k.b, k.bw, k.bn, k.sb, k.scratch, s);
+        if (rc) return rc;
+    }
     for (auto& e : B->small) {
         rc = fa_batch_small_bin(e.first.depth, e.first.kind, J, (const uint64_t*)(d + e.second.first),
                                 e.second.second, B->S, s);
@@ -168,7 +182,13 @@ static int batch_record(fa_batch_s* B, hipStream_t s)
     const fa_bpoly* P = (const fa_bpoly*)(d + B->poly_rec);
     rc = fa_batch_crt_poly(0, P, (const uint64_t*)(d + B->nmod_work), B->nnmod, s);
     if (rc) return rc;
-    return fa_batch_crt_poly(1, P, (const uint64_t*)(d + B->fmpz_work), B->nfmpz, s);
+    rc = fa_batch_crt_poly(1, P, (const uint64_t*)(d + B->fmpz_work), B->nfmpz, s);
+    if (rc) return rc;
+    for (auto& k : B->ks) {
+        rc = fa_ks_post(k.sb, k.an, k.bn, k.scratch, k.z, k.zw, k.zl, k.cnt, s);
+        if (rc) return rc;
+    }
+    return FA_OK;
 }
 
 extern "C" void fa_batch_destroy(fa_batch_s* B)
@@ -200,19 +220,36 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
         return fa_set_error(FA_E_HIP, "fa_batch_create: stream");
     }
 
-    // ---- plan every request
+    // ---- plan every request; fmpz products declined by the eight-prime rule
+    // become, under FA_BATCH_FMPZ_KS, three internal integer products
+    // appended after the caller's requests
     struct plan {
         int live;           // has device work
         int np, offset, depth, bits, kind;
         uint64_t zn, zlen, cnt, ngroups, nblk;
         int64_t b1, b2;
+        int ks;             // caller's fmpz request routed through KS: index into ksv
+        int virt;           // internal product k of KS record virt_ks
+        int virt_ks;
     };
+    struct ksplan {
+        int64_t r;
+        uint64_t sb, La, Lb;
+    };
+    std::vector<ksplan> ksv;
+    std::vector<fa_request> V;          // internal integer products
     std::vector<plan> P((size_t)nreq);
     int maxdepth = 4;
-    for (int64_t r = 0; r < nreq; r++) {
-        fa_request& q = R[r];
+    auto req = [&](int64_t r) -> fa_request& { return r < nreq ? R[r] : V[(size_t)(r - nreq)]; };
+    for (int64_t r = 0; r < nreq + (int64_t)V.size(); r++) {
+        if (r >= (int64_t)P.size()) P.resize((size_t)r + 1);
+        fa_request& q = req(r);
         plan& p = P[(size_t)r];
+        const int virt = r >= nreq ? p.virt : 0, virt_ks = r >= nreq ? p.virt_ks : 0;
         memset(&p, 0, sizeof(p));
+        p.virt = virt;
+        p.virt_ks = virt_ks;
+        p.ks = -1;
         q.status = 1;
         if (q.an <= 0 || q.bn <= 0 || !q.a || !q.b || !q.z) {
             q.status = FA_E_ARG;
@@ -260,6 +297,32 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
                     p.b2 = fa_vec_max_bits(q.b, (uint64_t)q.bw, q.bn);
                 }
                 p.np = fa_fmpz_primes((uint64_t)q.bn, p.b1, p.b2);
+                if (!p.np && (flags & FA_BATCH_FMPZ_KS)) {
+                    // Kronecker substitution: three integer products of La x Lb limbs
+                    ksplan K;
+                    K.r = r;
+                    K.sb = fa_ks_slot_bits(p.b1, p.b2, (uint64_t)q.an, (uint64_t)q.bn);
+                    K.La = (q.an * K.sb + 63) / 64;
+                    K.Lb = (q.bn * K.sb + 63) / 64;
+                    p.ks = (int)ksv.size();
+                    ksv.push_back(K);
+                    for (int k = 0; k < 3; k++) {
+                        fa_request v;
+                        memset(&v, 0, sizeof(v));
+                        v.kind = FA_REQ_MPN;
+                        v.an = (int64_t)K.La;
+                        v.bn = (int64_t)K.Lb;
+                        v.a = v.b = (const uint64_t*)&V;     // resolved to the KS scratch
+                        v.z = (uint64_t*)&V;
+                        V.push_back(v);
+                        plan pv;
+                        memset(&pv, 0, sizeof(pv));
+                        pv.virt = k + 1;
+                        pv.virt_ks = p.ks;
+                        P.push_back(pv);
+                    }
+                    continue;                            // status stays 1
+                }
                 if (!p.np) { q.status = 0; continue; }   // declined, as the reference does
             }
             p.live = 1;
@@ -278,6 +341,14 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
         q.depth = p.depth;
         if (p.depth > maxdepth) maxdepth = p.depth;
     }
+    const int64_t ntot = nreq + (int64_t)V.size();
+    for (auto& K : ksv) {
+        // a failed internal product fails its fmpz request
+        for (int k = 0; k < 3; k++) {
+            const int64_t v = nreq + 3 * (int64_t)(&K - ksv.data()) + k;
+            if (req(v).status != 1) R[K.r].status = req(v).status;
+        }
+    }
     fa_tables* T = fa_get_tables(maxdepth < 20 ? 20 : maxdepth);
     if (!T) { fa_batch_destroy(B); return FA_E_NODEV; }
     fa_prime_set_init(&B->S, T, FA_NPRIMES, maxdepth, 0, 0);
@@ -289,12 +360,24 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
     std::vector<fa_bjob> jobs;
     std::vector<fa_bmpn> mrec;
     std::vector<fa_bpoly> prec;
-    std::vector<int> job_of((size_t)nreq, -1), rec_of((size_t)nreq, -1);
+    std::vector<int> job_of((size_t)ntot, -1), rec_of((size_t)ntot, -1);
     struct offs { size_t A, B, a, b, z, flags, agg; };
-    std::vector<offs> O((size_t)nreq);
+    std::vector<offs> O((size_t)ntot);
+    std::vector<size_t> ks_scratch(ksv.size());
+    for (size_t k = 0; k < ksv.size(); k++) {
+        const fa_request& q = R[ksv[k].r];
+        const plan& p = P[(size_t)ksv[k].r];
+        ks_scratch[k] = Lb.take(fa_ks_scratch_words(p.b1, p.b2, (uint64_t)q.an, (uint64_t)q.bn) * 8);
+        if (!dev) {
+            offs& o = O[(size_t)ksv[k].r];
+            o.a = Lin.take((size_t)q.an * (size_t)q.aw * 8);
+            o.b = Lin.take((size_t)q.bn * (size_t)q.bw * 8);
+            o.z = Lout.take((size_t)(q.zh - q.zl) * (size_t)q.zw * 8);
+        }
+    }
     std::map<std::pair<int, int>, size_t> pconsts;   // (np, offset) -> device offset
-    for (int64_t r = 0; r < nreq; r++) {
-        fa_request& q = R[r];
+    for (int64_t r = 0; r < ntot; r++) {
+        fa_request& q = req(r);
         plan& p = P[(size_t)r];
         if (!p.live || q.status != 1) continue;
         offs& o = O[(size_t)r];
@@ -303,7 +386,7 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
             o.A = Lb.take((size_t)p.np * stride * sizeof(double));
             if (p.depth > FA_ROW_LG) o.B = Lb.take((size_t)p.np * stride * sizeof(double));
         }
-        if (!dev) {
+        if (!dev && !p.virt) {
             const size_t aw = q.kind == FA_REQ_FMPZ ? (size_t)q.aw : 1;
             const size_t bw = q.kind == FA_REQ_FMPZ ? (size_t)q.bw : 1;
             o.a = Lin.take((size_t)q.an * aw * 8);
@@ -311,7 +394,7 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
         }
         size_t zwords = q.kind == FA_REQ_MPN ? (size_t)p.zn
                         : q.kind == FA_REQ_NMOD ? (size_t)p.cnt : (size_t)p.cnt * (size_t)q.zw;
-        if (!dev) o.z = Lout.take(zwords * 8);
+        if (!dev && !p.virt) o.z = Lout.take(zwords * 8);
         if (q.kind == FA_REQ_MPN) {
             o.flags = Lb.take((size_t)p.ngroups + 8);
             o.agg = Lb.take((size_t)p.nblk + 8);
@@ -330,8 +413,8 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
     std::map<int, std::vector<uint64_t>> crt_bins;
     std::vector<unsigned> carry;
     std::vector<uint64_t> nmod_w, fmpz_w;
-    for (int64_t r = 0; r < nreq; r++) {
-        fa_request& q = R[r];
+    for (int64_t r = 0; r < ntot; r++) {
+        fa_request& q = req(r);
         plan& p = P[(size_t)r];
         if (!p.live || q.status != 1) continue;
         if (p.np) {
@@ -464,16 +547,54 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
     char* din = d + inb;   // staged operands (host-pointer batches)
     char* dout = d + outb; // staged outputs
 
+    // KS records: the caller's operands and output, the scratch
+    for (size_t k = 0; k < ksv.size(); k++) {
+        const fa_request& q = R[ksv[k].r];
+        if (q.status != 1) continue;
+        const offs& o = O[(size_t)ksv[k].r];
+        fa_batch_s::ksrec K;
+        K.a = dev ? q.a : (const uint64_t*)(din + o.a);
+        K.b = dev ? q.b : (const uint64_t*)(din + o.b);
+        K.z = dev ? q.z : (uint64_t*)(dout + o.z);
+        K.scratch = (uint64_t*)(db0 + ks_scratch[k]);
+        K.aw = (uint64_t)q.aw;
+        K.an = (uint64_t)q.an;
+        K.bw = (uint64_t)q.bw;
+        K.bn = (uint64_t)q.bn;
+        K.sb = ksv[k].sb;
+        K.zw = (uint64_t)q.zw;
+        K.zl = (uint64_t)q.zl;
+        K.cnt = (uint64_t)(q.zh - q.zl);
+        B->ks.push_back(K);
+        if (!dev) {
+            B->h2d.push_back({(void*)(uintptr_t)o.a, q.a, (size_t)(q.an * q.aw) * 8});
+            B->h2d.push_back({(void*)(uintptr_t)o.b, q.b, (size_t)(q.bn * q.bw) * 8});
+            B->d2h.push_back({q.z, (const void*)(uintptr_t)o.z, (size_t)(K.cnt * K.zw) * 8});
+        }
+    }
+
     // resolve device addresses into the records
-    for (int64_t r = 0; r < nreq; r++) {
-        fa_request& q = R[r];
+    for (int64_t r = 0; r < ntot; r++) {
+        fa_request& q = req(r);
         plan& p = P[(size_t)r];
         if (!p.live || q.status != 1) continue;
         offs& o = O[(size_t)r];
         const uint64_t* da = dev ? q.a : (const uint64_t*)(din + o.a);
         const uint64_t* db = dev ? q.b : (const uint64_t*)(din + o.b);
         uint64_t* dz = dev ? q.z : (uint64_t*)(dout + o.z);
-        if (!dev) {
+        if (p.virt) {
+            // internal product k of a KS record: operands and product in its scratch
+            const size_t kk = (size_t)p.virt_ks;
+            uint64_t* ptrs[9];
+            uint64_t lens[2];
+            const fa_request& qs = R[ksv[kk].r];
+            fa_ks_products(ksv[kk].sb, (uint64_t)qs.an, (uint64_t)qs.bn,
+                           (uint64_t*)(db0 + ks_scratch[kk]), ptrs, lens);
+            da = ptrs[3 * (p.virt - 1)];
+            db = ptrs[3 * (p.virt - 1) + 1];
+            dz = ptrs[3 * (p.virt - 1) + 2];
+        }
+        if (!dev && !p.virt) {
             // host side of the staging: offsets into the pinned buffers
             const size_t aw = q.kind == FA_REQ_FMPZ ? (size_t)q.aw : 1;
             const size_t bw = q.kind == FA_REQ_FMPZ ? (size_t)q.bw : 1;

@@ -174,7 +174,10 @@ FA_API int fa_probe_colcopy(const void* src, void* dst, int d1, int d2, int lgC,
 enum { FA_REQ_MPN = 0, FA_REQ_NMOD = 1, FA_REQ_FMPZ = 2 };
 enum {
     FA_BATCH_DEVICE = 1,   /* a, b, z are device addresses; results stay in HBM */
-    FA_BATCH_GRAPH = 2     /* capture the launch sequence into a hipGraph */
+    FA_BATCH_GRAPH = 2,    /* capture the launch sequence into a hipGraph */
+    FA_BATCH_FMPZ_KS = 4   /* fmpz products the eight-prime rule declines are computed by
+                              Kronecker substitution (status 1), as the reference's
+                              callers do (fmpz_poly/mul.c:160), instead of returning 0 */
 };
 
 typedef struct {
@@ -194,6 +197,14 @@ typedef struct {
     int32_t np, bits;      /* mpn plan override (0 = choose); out: plan used */
     int32_t depth, pad;    /* out: transform depth */
 } fa_request;
+
+/* the fmpz product the reference's callers compute when the eight-prime
+   path declines, by Kronecker substitution onto the integer product
+   (fmpz_poly/mul.c:160, _fmpz_poly_mul_KS); arguments and output as
+   fa_fmpz_poly_mul_mid; always computes (returns 1) */
+FA_API int fa_fmpz_poly_mul_ks(uint64_t* z, uint64_t zw, int64_t zl, int64_t zh,
+                               const uint64_t* a, uint64_t aw, int64_t an, const uint64_t* b,
+                               uint64_t bw, int64_t bn);
 
 /* _fmpz_vec_max_bits (fmpz_vec.h) of n w-limb two's complement
    coefficients: the largest bit size of |c|, negated if some c < 0 */

@@ -197,3 +197,70 @@ def test_batch_device_graph_replays():
                 assert np.array_equal(got, w), (rep, k)
     finally:
         B.close()
+
+
+def test_batch_fmpz_kronecker_route():
+    """fmpz products beyond eight primes inside a batch (FA_BATCH_FMPZ_KS):
+    the same outputs as the single-call Kronecker route, and exact on a
+    small case; host and graph-captured device forms"""
+    import ctypes
+    import torch
+    import flint_amd as fa
+    from tests.golden.make_fixtures import poly_ks
+    rng = np.random.default_rng(0x4B5)
+    reqs = []
+    for bits, an, bn in [(300, 40, 30), (700, 500, 400), (150, 800, 900), (2000, 64, 100)]:
+        w = bits // 64 + 1
+        zw = (2 * bits + 16 + 64) // 64 + 1
+        a = signed_coeffs(rng, an, bits, w)
+        b = signed_coeffs(rng, bn, bits, w)
+        reqs.append({"kind": "fmpz", "a": a, "aw": w, "b": b, "bw": w, "zl": 0,
+                     "zh": an + bn + 1, "zw": zw})
+    got = fa.batch_mul(reqs, ks=True)
+    for k, q in enumerate(reqs):
+        st, z = got[k]
+        assert st == 1, k
+        want = fa.fmpz_poly_mul_ks(q["a"], q["aw"], q["b"], q["bw"], q["zl"], q["zh"], q["zw"])
+        assert np.array_equal(z, want), k
+    # exact integers for the first one
+    q = reqs[0]
+
+    def dec(v, w):
+        out = []
+        for i in range(len(v) // w):
+            x = int.from_bytes(np.ascontiguousarray(v[i * w:(i + 1) * w]).tobytes(), "little")
+            out.append(x - (1 << (64 * w)) if x >> (64 * w - 1) else x)
+        return out
+    exact = poly_ks(dec(q["a"], q["aw"]), dec(q["b"], q["bw"]), signed=True)
+    assert dec(got[0][1], q["zw"])[:len(exact)] == exact
+    # device form with a graph, two replays
+    dev = torch.device("cuda:0")
+    R = (fa.Request * len(reqs))()
+    keep, outs = [], []
+    for k, q in enumerate(reqs):
+        ta = torch.from_numpy(q["a"].view(np.int64)).to(dev)
+        tb = torch.from_numpy(q["b"].view(np.int64)).to(dev)
+        tz = torch.zeros((q["zh"] - q["zl"]) * q["zw"], dtype=torch.int64, device=dev)
+        keep += [ta, tb, tz]
+        outs.append(tz)
+        r = R[k]
+        r.kind = fa.REQ_FMPZ
+        r.a, r.b, r.z = ta.data_ptr(), tb.data_ptr(), tz.data_ptr()
+        r.aw, r.bw, r.zw = q["aw"], q["bw"], q["zw"]
+        r.an, r.bn = len(q["a"]) // q["aw"], len(q["b"]) // q["bw"]
+        r.zl, r.zh = q["zl"], q["zh"]
+        r.abits = fa.fmpz_vec_max_bits(q["a"], q["aw"])
+        r.bbits = fa.fmpz_vec_max_bits(q["b"], q["bw"])
+    B = fa.DeviceBatch(R, graph=True, ks=True)
+    stream = torch.cuda.Stream()
+    try:
+        for rep in range(2):
+            for tz in outs:
+                tz.zero_()
+            torch.cuda.synchronize()
+            B.launch(ctypes.c_void_p(stream.cuda_stream))
+            stream.synchronize()
+            for k in range(len(reqs)):
+                assert np.array_equal(outs[k].cpu().numpy().view(np.uint64), got[k][1]), (rep, k)
+    finally:
+        B.close()
