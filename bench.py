@@ -218,7 +218,10 @@ def batched_main(args, torch, fa):
             elif q["kind"] == "nmod":
                 O.nmod_poly_mul_mid(q["a"], q["b"], q["zl"], q["zh"], q["n"])
             else:
-                O.fmpz_poly_mul_mid(q["a"], q["aw"], q["b"], q["bw"], q["zl"], q["zh"], q["zw"])
+                ok, _ = O.fmpz_poly_mul_mid(q["a"], q["aw"], q["b"], q["bw"], q["zl"], q["zh"],
+                                            q["zw"])
+                if not ok:      # the oracle has no Kronecker route: not part of the sample
+                    continue
             lp_s += q["lp"]
             n_s += 1
             if time.perf_counter() - tc0 > 15.0:
