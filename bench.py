@@ -105,9 +105,15 @@ def cpu_baseline(W, sample_log2):
     }
 
 
+def _res_words(depth):
+    """doubles of one prime's residue array (rows of 4096 at a pitch of 4224
+    above depth 12)"""
+    return (4224 << (depth - 12)) if depth > 12 else (1 << depth)
+
+
 def _mpn_ws(an, bn, fa):
     p = fa.mpn_plan(an, bn)
-    return 16 * p["np"] * (1 << p["depth"]) + 16 * (an + bn)
+    return 16 * p["np"] * _res_words(p["depth"]) + 16 * (an + bn)
 
 
 def _plan_ws(q, fa):
@@ -118,7 +124,7 @@ def _plan_ws(q, fa):
     an = len(q["a"]) // q.get("aw", 1)
     bn = len(q["b"]) // q.get("bw", 1)
     depth = max(4, int(np.ceil(np.log2(an + bn))))
-    ws = 16 * 8 * (1 << depth) + 8 * (an + bn) * q.get("zw", 1)
+    ws = 16 * 8 * _res_words(depth) + 8 * (an + bn) * q.get("zw", 1)
     if q["kind"] == "fmpz":
         sb = abs(q["abits"]) + abs(q["bbits"]) + int(min(an, bn)).bit_length() + 1
         la, lb = -(-an * sb // 64), -(-bn * sb // 64)

@@ -85,6 +85,7 @@ SIGNATURES = {
     "fa_copy_device": (ctypes.c_int, [_VP, _VP, _u64, _VP]),
     "fa_set_fast_kernels": (None, [ctypes.c_int]),
     "fa_set_col_variant": (None, [ctypes.c_int]),
+    "fa_set_row_variant": (None, [ctypes.c_int]),
     "fa_set_crt_variant": (None, [ctypes.c_int]),
     "fa_probe_colcopy": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP]),
     "fa_batch_create": (ctypes.c_int, [ctypes.POINTER(_VP), _VP, _s64, ctypes.c_int]),
@@ -186,6 +187,12 @@ def prof_read(max_entries=64):
 
 def set_fast_kernels(on=True):
     lib().fa_set_fast_kernels(1 if on else 0)
+
+
+def set_row_variant(v):
+    """row pass of the two-level transforms: 1 = three workgroups per CU,
+    last-stage twiddles formed per layer (default); 0 = two, all held"""
+    lib().fa_set_row_variant(int(v))
 
 
 def set_col_variant(v):

@@ -381,7 +381,7 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
         plan& p = P[(size_t)r];
         if (!p.live || q.status != 1) continue;
         offs& o = O[(size_t)r];
-        const uint64_t stride = UINT64_C(1) << p.depth;
+        const uint64_t stride = fa_res_stride(p.depth);
         if (p.np) {
             o.A = Lb.take((size_t)p.np * stride * sizeof(double));
             if (p.depth > FA_ROW_LG) o.B = Lb.take((size_t)p.np * stride * sizeof(double));
@@ -428,7 +428,7 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
             j.bw = q.kind == FA_REQ_FMPZ ? (int)q.bw : 1;
             j.kind = p.kind == FA_PACK_MPN8 ? FA_PACK_MPN : p.kind;
             j.bits = p.bits;
-            j.stride = UINT64_C(1) << p.depth;
+            j.stride = fa_res_stride(p.depth);
             j.np = p.np;
             j.offset = p.offset;
             j.depth = p.depth;
@@ -471,7 +471,7 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
             rec_of[(size_t)r] = (int)mid;
             fa_bmpn m;
             memset(&m, 0, sizeof(m));
-            m.stride = UINT64_C(1) << p.depth;
+            m.stride = fa_res_stride(p.depth);
             m.zlen = p.zlen;
             m.zn = p.zn;
             m.ngroups = p.ngroups;
@@ -487,7 +487,7 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
             rec_of[(size_t)r] = (int)pid;
             fa_bpoly b;
             memset(&b, 0, sizeof(b));
-            b.stride = UINT64_C(1) << p.depth;
+            b.stride = fa_res_stride(p.depth);
             b.zl = (uint64_t)q.zl;
             b.cnt = p.cnt;
             b.zn = p.zn;

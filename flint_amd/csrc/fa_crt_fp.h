@@ -67,7 +67,7 @@ FA_DEV void fa_crt_fp_value(const double* __restrict__ Q, uint64_t stride, uint6
     for (int i = 0; i < NP; i++) {
         if (i == 6) fa_crt_fp_carry<ND>(col);   // keep every column below 2^53
         const double n = F.p[i];
-        double x = fa_reduce_pm1n(Q[(uint64_t)i * stride + j], n, F.pinv[i]);
+        double x = fa_reduce_pm1n(Q[(uint64_t)i * stride + fa_res_pos(j, stride)], n, F.pinv[i]);
         x = (x < 0.0) ? x + n : x;               // [0, n)
         qe = __builtin_fma(x, F.rp[i], qe);
         const double v = __builtin_floor(x * Binv);

@@ -143,6 +143,87 @@ FA_DEV void fa_row_tw_last(const double* rho, const double* __restrict__ w, int 
     }
 }
 
+// radix-16 stage j < 2 of block bq with each radix-4 layer's twiddles read
+// from the LDS table just before the layer
+FA_DEV void fa_fwd16_rowtab(double* x, const double* tab, int j, int bq, double n, double ninv)
+{
+    const int m = 4 * j;
+    {
+        const double T0 = tab[(1 << m) - 1 + bq];
+        const double T1 = tab[(2 << m) - 1 + 2 * bq];
+        const double T2 = tab[(2 << m) - 1 + 2 * bq + 1];
+#pragma unroll
+        for (int k = 0; k < 4; k++) fa_fwd4(x[k], x[k + 4], x[k + 8], x[k + 12], T0, T1, T2, n, ninv);
+    }
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+        const int Qp = 4 * bq + jj;
+        const double T0 = tab[(4 << m) - 1 + Qp];
+        const double T1 = tab[(8 << m) - 1 + 2 * Qp];
+        const double T2 = tab[(8 << m) - 1 + 2 * Qp + 1];
+        fa_fwd4(x[4 * jj], x[4 * jj + 1], x[4 * jj + 2], x[4 * jj + 3], T0, T1, T2, n, ninv);
+    }
+}
+
+FA_DEV void fa_inv16_rowtab(double* x, const double* tab, int j, int bq, double n, double ninv)
+{
+    const int m = 4 * j;
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+        const int Qp = 4 * bq + jj;
+        const double T0 = tab[(4 << m) - 1 + Qp];
+        const double T1 = tab[(8 << m) - 1 + 2 * Qp];
+        const double T2 = tab[(8 << m) - 1 + 2 * Qp + 1];
+        fa_inv4(x[4 * jj], x[4 * jj + 1], x[4 * jj + 2], x[4 * jj + 3], T0, T1, T2, n, ninv);
+    }
+    const double T0 = tab[(1 << m) - 1 + bq];
+    const double T1 = tab[(2 << m) - 1 + 2 * bq];
+    const double T2 = tab[(2 << m) - 1 + 2 * bq + 1];
+#pragma unroll
+    for (int k = 0; k < 4; k++) fa_inv4(x[k], x[k + 4], x[k + 8], x[k + 12], T0, T1, T2, n, ninv);
+}
+
+// the last radix-16 stage (levels 8..11 of block t) with its twiddles formed
+// just before each radix-4 layer: fewer registers live across the row
+// transforms than with all 15 held (three workgroups share a CU)
+FA_DEV void fa_fwd16_rowlast(double* x, const double* rho, const double* __restrict__ w, int t,
+                             double n, double ninv)
+{
+    {
+        const double T0 = fa_row_entry(rho[8], w, t, n, ninv);
+        const double T1 = fa_row_entry(rho[9], w, 2 * t, n, ninv);
+        const double T2 = fa_row_entry(rho[9], w, 2 * t + 1, n, ninv);
+#pragma unroll
+        for (int k = 0; k < 4; k++) fa_fwd4(x[k], x[k + 4], x[k + 8], x[k + 12], T0, T1, T2, n, ninv);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int Qp = 4 * t + j;
+        const double T0 = fa_row_entry(rho[10], w, Qp, n, ninv);
+        const double T1 = fa_row_entry(rho[11], w, 2 * Qp, n, ninv);
+        const double T2 = fa_row_entry(rho[11], w, 2 * Qp + 1, n, ninv);
+        fa_fwd4(x[4 * j], x[4 * j + 1], x[4 * j + 2], x[4 * j + 3], T0, T1, T2, n, ninv);
+    }
+}
+
+FA_DEV void fa_inv16_rowlast(double* x, const double* rho, const double* __restrict__ w, int t,
+                             double n, double ninv)
+{
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int Qp = 4 * t + j;
+        const double T0 = fa_row_entry(rho[10], w, Qp, n, ninv);
+        const double T1 = fa_row_entry(rho[11], w, 2 * Qp, n, ninv);
+        const double T2 = fa_row_entry(rho[11], w, 2 * Qp + 1, n, ninv);
+        fa_inv4(x[4 * j], x[4 * j + 1], x[4 * j + 2], x[4 * j + 3], T0, T1, T2, n, ninv);
+    }
+    const double T0 = fa_row_entry(rho[8], w, t, n, ninv);
+    const double T1 = fa_row_entry(rho[9], w, 2 * t, n, ninv);
+    const double T2 = fa_row_entry(rho[9], w, 2 * t + 1, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 4; k++) fa_inv4(x[k], x[k + 4], x[k + 8], x[k + 12], T0, T1, T2, n, ninv);
+}
+
 // 4096-point forward transform of one row, 256 threads, input in the
 // stage-0 layout (positions t + 256k), output in the stage-2 layout
 // (positions 16t + k); twl = the thread's last-stage twiddles
@@ -220,6 +301,79 @@ FA_DEV void fa_row_conv_tile(double* __restrict__ arow, const double* __restrict
     for (int k = 0; k < 16; k++) arow[t + 256 * k] = x[k];
 }
 
+// fa_row_conv_tile with the last-stage twiddles formed on the fly
+FA_DEV void fa_row_conv_tile_lr(double* __restrict__ arow, const double* __restrict__ brow,
+                                uint64_t r, double n, double ninv, double m,
+                                const double* __restrict__ w2, const double* __restrict__ w2inv,
+                                double* xch, double* tab, double* rho)
+{
+    const int t = threadIdx.x;
+    const int b = t >> 4, i0 = t & 15;
+    double x[16], y[16];
+    fa_build_row_table(tab, w2, r, rho, n, ninv);
+#pragma unroll
+    for (int z = 0; z < 2; z++) {
+        double* v = z ? x : y;
+        const double* src = z ? arow : brow;
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[k] = src[t + 256 * k];
+        fa_fwd16_rowtab(v, tab, 0, 0, n, ninv);
+#pragma unroll
+        for (int k = 0; k < 16; k++) xch[PADX(t + 256 * k)] = v[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[k] = xch[PADX(256 * b + i0 + 16 * k)];
+        fa_fwd16_rowtab(v, tab, 1, b, n, ninv);
+#pragma unroll
+        for (int k = 0; k < 16; k++) xch[PADX(256 * b + i0 + 16 * k)] = v[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[k] = xch[PADX(16 * t + k)];
+        __builtin_amdgcn_sched_barrier(0);
+        fa_fwd16_rowlast(v, rho, w2, t, n, ninv);
+        // the finished row is materialized here, not next to its use
+#pragma unroll
+        for (int k = 0; k < 16; k++) asm volatile("" : "+v"(v[k]));
+        __builtin_amdgcn_sched_barrier(0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = fa_mulmod(fa_mulmod(x[k], m, n, ninv), y[k], n, ninv);
+    fa_build_row_table(tab, w2inv, r, rho, n, ninv);
+    __builtin_amdgcn_sched_barrier(0);
+    fa_inv16_rowlast(x, rho, w2inv, t, n, ninv);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < 16; k++) xch[PADX(16 * t + k)] = x[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = xch[PADX(256 * b + i0 + 16 * k)];
+    fa_inv16_rowtab(x, tab, 1, b, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 16; k++) xch[PADX(256 * b + i0 + 16 * k)] = x[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = xch[PADX(t + 256 * k)];
+    fa_inv16_rowtab(x, tab, 0, 0, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 16; k++) arow[t + 256 * k] = x[k];
+}
+
+__global__ void __launch_bounds__(256, 3) fa_k_row_conv16lr(double* __restrict__ A,
+                                                             const double* __restrict__ B,
+                                                             uint64_t stride, fa_prime_set S)
+{
+    __shared__ double xch[4352];
+    __shared__ double tab[256];
+    __shared__ double rho[16];
+    const int i = blockIdx.y;
+    const uint64_t r = blockIdx.x;
+    fa_row_conv_tile_lr(A + (uint64_t)i * stride + r * FA_ROW_PITCH,
+                        B + (uint64_t)i * stride + r * FA_ROW_PITCH,
+                        r, S.p[i], S.pinv[i], S.norm[i], S.w2[i], S.w2inv[i], xch, tab, rho);
+}
+
 // blockIdx.x = row, blockIdx.y = prime
 __global__ void __launch_bounds__(256) fa_k_row_conv16(double* __restrict__ A,
                                                           const double* __restrict__ B,
@@ -230,7 +384,8 @@ __global__ void __launch_bounds__(256) fa_k_row_conv16(double* __restrict__ A,
     __shared__ double rho[16];
     const int i = blockIdx.y;
     const uint64_t r = blockIdx.x;
-    fa_row_conv_tile(A + (uint64_t)i * stride + (r << 12), B + (uint64_t)i * stride + (r << 12), r,
+    fa_row_conv_tile(A + (uint64_t)i * stride + r * FA_ROW_PITCH,
+                     B + (uint64_t)i * stride + r * FA_ROW_PITCH, r,
                      S.p[i], S.pinv[i], S.norm[i], S.w2[i], S.w2inv[i], xch, tab, rho);
 }
 
@@ -396,7 +551,7 @@ FA_DEV void fa_col_fwd_tile(const fa_operand& op, double* __restrict__ dst, int 
     }
     const int jl = (D1 < 4) ? 0 : G::NSTAGE - 1;
 #pragma unroll
-    for (int k = 0; k < G::NEL; k++) dst[((uint64_t)G::pos(jl, tc, k) << d2) + col] = x[k];
+    for (int k = 0; k < G::NEL; k++) dst[(uint64_t)G::pos(jl, tc, k) * FA_ROW_PITCH + col] = x[k];
 }
 
 template <int D1>
@@ -410,7 +565,7 @@ FA_DEV void fa_col_inv_tile(double* __restrict__ dst, int d2, int g, double n, d
     double x[16], tw[15];
     const int jl = (D1 < 4) ? 0 : G::NSTAGE - 1;
 #pragma unroll
-    for (int k = 0; k < G::NEL; k++) x[k] = dst[((uint64_t)G::pos(jl, tc, k) << d2) + col];
+    for (int k = 0; k < G::NEL; k++) x[k] = dst[(uint64_t)G::pos(jl, tc, k) * FA_ROW_PITCH + col];
     if (D1 < 4) {
         fa_inv_small_col<(D1 < 4 ? D1 : 1)>(x, wi, n, ninv);
     } else {
@@ -432,7 +587,7 @@ FA_DEV void fa_col_inv_tile(double* __restrict__ dst, int d2, int g, double n, d
         }
     }
 #pragma unroll
-    for (int k = 0; k < G::NEL; k++) dst[((uint64_t)G::pos(0, tc, k) << d2) + col] = x[k];
+    for (int k = 0; k < G::NEL; k++) dst[(uint64_t)G::pos(0, tc, k) * FA_ROW_PITCH + col] = x[k];
 }
 
 // work index L -> (prime i, column group g, operand z) with the two column
@@ -479,36 +634,41 @@ __global__ void __launch_bounds__(1024) fa_k_col_inv16(double* __restrict__ A, i
 }
 
 // ---------------------------------------------------------------------------
-// 8-column tiles for D1 = 12.  Lane l of a workgroup takes column
-// c = l mod 8 and u = l / 8 < 128, and at every stage holds two radix-16
-// groups, tc = u and tc = u + 128 (x0, x1), so one wave instruction covers
-// 8 rows x 64 contiguous bytes.  A stage exchange runs in two halves of the
-// column (rows [0, 2048) and [2048, 4096)), each 8 columns x 2048 rows in
-// LDS:
+// C-column tiles for D1 = 12 (C = 8: 1024 threads, C = 4: 512 threads).
+// Lane l of a workgroup takes column c = l mod C and u = l / C < 128, and at
+// every stage holds two radix-16 groups, tc = u and tc = u + 128 (x0, x1),
+// so one wave instruction covers 64/C rows x 8C contiguous bytes.  A stage
+// exchange runs in two halves of the column (rows [0, 2048) and
+// [2048, 4096)), each C columns x 2048 rows in LDS:
 //  - stage 0 -> 1: a stage-0 group (rows tc + 256k) has k < 8 in the first
 //    half and k >= 8 in the second; the stage-1 group read back in half H is
 //    tc = 128H + u (blocks 8H .. 8H+7 of level 4);
 //  - stage 1 -> 2: stage-1 group u + 128H and stage-2 group u + 128H both
 //    lie in half H.
-// LDS rows interleave the 8 columns (element (c, r) at 8 PADX(r) + c), so
-// the lanes of a wave touch consecutive doubles when their rows are
-// consecutive, and the pad of one row per 16 separates the 16-row strides
-// of the last stage.
+// LDS rows interleave the C columns (element (c, r) at C PADX(r) + c), so the
+// lanes of a wave touch consecutive doubles when their rows are consecutive,
+// and the pad of one row per 16 separates the 16-row strides of the last
+// stage: a half-wave of 64-bit accesses then covers 64 distinct banks for
+// C = 8 (4 rows x 8 columns) and C = 4 (8 rows x 4 columns).  The C = 4 tile
+// needs 69.6 KB of LDS, so two workgroups share a CU and one's loads and
+// stores overlap the other's arithmetic.
 // ---------------------------------------------------------------------------
-#define FA_C8_LDS (8 * 2176)
+#define FA_CW_LDS(C) ((C) * 2176)
+#define FA_C8_LDS FA_CW_LDS(8)
 
-FA_DEV int fa_c8_addr(int c, int row) { return 8 * PADX(row & 2047) + c; }
+template <int C>
+FA_DEV int fa_cw_addr(int c, int row) { return C * PADX(row & 2047) + c; }
 
 // stage-1 row of group tc, element k
 FA_DEV int fa_c8_row1(int tc, int k) { return ((tc >> 4) << 8) + (tc & 15) + 16 * k; }
 
-template <int KIND>
-FA_DEV void fa_col_fwd8_tile(const fa_operand& op, double* __restrict__ dst, int d2, int g,
+template <int C, int KIND>
+FA_DEV void fa_col_fwdw_tile(const fa_operand& op, double* __restrict__ dst, int d2, int g,
                              double n, double ninv, const double* __restrict__ w2,
                              const double* __restrict__ pow2, double* lds)
 {
-    const int c = threadIdx.x & 7, u = threadIdx.x >> 3;
-    const uint64_t col = (uint64_t)g * 8 + c;
+    const int c = threadIdx.x & (C - 1), u = threadIdx.x / C;
+    const uint64_t col = (uint64_t)g * C + c;
     double x0[16], x1[16], tw[15];
     // rows >= 2048 (elements >= 2^(d2+11)) all zero: the usual case of a
     // product's operands, each at most half the transform length
@@ -544,14 +704,14 @@ FA_DEV void fa_col_fwd8_tile(const fa_operand& op, double* __restrict__ dst, int
     for (int H = 0; H < 2; H++) {
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            lds[fa_c8_addr(c, u + 256 * (8 * H + k))] = x0[8 * H + k];
-            lds[fa_c8_addr(c, u + 128 + 256 * (8 * H + k))] = x1[8 * H + k];
+            lds[fa_cw_addr<C>(c, u + 256 * (8 * H + k))] = x0[8 * H + k];
+            lds[fa_cw_addr<C>(c, u + 128 + 256 * (8 * H + k))] = x1[8 * H + k];
         }
         __syncthreads();
         const int tc = 128 * H + u;
 #pragma unroll
         for (int k = 0; k < 16; k++) {
-            const double v = lds[fa_c8_addr(c, fa_c8_row1(tc, k))];
+            const double v = lds[fa_cw_addr<C>(c, fa_c8_row1(tc, k))];
             if (H == 0) y0[k] = v; else y1[k] = v;
         }
         __syncthreads();
@@ -567,11 +727,11 @@ FA_DEV void fa_col_fwd8_tile(const fa_operand& op, double* __restrict__ dst, int
         const int tc = 128 * H + u;
 #pragma unroll
         for (int k = 0; k < 16; k++)
-            lds[fa_c8_addr(c, fa_c8_row1(tc, k))] = H == 0 ? y0[k] : y1[k];
+            lds[fa_cw_addr<C>(c, fa_c8_row1(tc, k))] = H == 0 ? y0[k] : y1[k];
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < 16; k++) {
-            const double v = lds[fa_c8_addr(c, 16 * tc + k)];
+            const double v = lds[fa_cw_addr<C>(c, 16 * tc + k)];
             if (H == 0) x0[k] = v; else x1[k] = v;
         }
         if (H == 0) __syncthreads();
@@ -583,21 +743,22 @@ FA_DEV void fa_col_fwd8_tile(const fa_operand& op, double* __restrict__ dst, int
     fa_fwd16(x1, tw, n, ninv);
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-        dst[((uint64_t)(16 * u + k) << d2) + col] = x0[k];
-        dst[((uint64_t)(16 * (u + 128) + k) << d2) + col] = x1[k];
+        dst[(uint64_t)(16 * u + k) * FA_ROW_PITCH + col] = x0[k];
+        dst[(uint64_t)(16 * (u + 128) + k) * FA_ROW_PITCH + col] = x1[k];
     }
 }
 
-FA_DEV void fa_col_inv8_tile(double* __restrict__ dst, int d2, int g, double n, double ninv,
+template <int C>
+FA_DEV void fa_col_invw_tile(double* __restrict__ dst, int d2, int g, double n, double ninv,
                              const double* __restrict__ wi, double* lds)
 {
-    const int c = threadIdx.x & 7, u = threadIdx.x >> 3;
-    const uint64_t col = (uint64_t)g * 8 + c;
+    const int c = threadIdx.x & (C - 1), u = threadIdx.x / C;
+    const uint64_t col = (uint64_t)g * C + c;
     double x0[16], x1[16], tw[15];
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-        x0[k] = dst[((uint64_t)(16 * u + k) << d2) + col];
-        x1[k] = dst[((uint64_t)(16 * (u + 128) + k) << d2) + col];
+        x0[k] = dst[(uint64_t)(16 * u + k) * FA_ROW_PITCH + col];
+        x1[k] = dst[(uint64_t)(16 * (u + 128) + k) * FA_ROW_PITCH + col];
     }
     // stage 2 (levels 11..8), block tc
     fa_tw16(wi, (uint64_t)u, tw);
@@ -610,11 +771,11 @@ FA_DEV void fa_col_inv8_tile(double* __restrict__ dst, int d2, int g, double n, 
     for (int H = 0; H < 2; H++) {
         const int tc = 128 * H + u;
 #pragma unroll
-        for (int k = 0; k < 16; k++) lds[fa_c8_addr(c, 16 * tc + k)] = H == 0 ? x0[k] : x1[k];
+        for (int k = 0; k < 16; k++) lds[fa_cw_addr<C>(c, 16 * tc + k)] = H == 0 ? x0[k] : x1[k];
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < 16; k++) {
-            const double v = lds[fa_c8_addr(c, fa_c8_row1(tc, k))];
+            const double v = lds[fa_cw_addr<C>(c, fa_c8_row1(tc, k))];
             if (H == 0) y0[k] = v; else y1[k] = v;
         }
         __syncthreads();
@@ -631,12 +792,12 @@ FA_DEV void fa_col_inv8_tile(double* __restrict__ dst, int d2, int g, double n, 
         const int tc = 128 * H + u;
 #pragma unroll
         for (int k = 0; k < 16; k++)
-            lds[fa_c8_addr(c, fa_c8_row1(tc, k))] = H == 0 ? y0[k] : y1[k];
+            lds[fa_cw_addr<C>(c, fa_c8_row1(tc, k))] = H == 0 ? y0[k] : y1[k];
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            x0[8 * H + k] = lds[fa_c8_addr(c, u + 256 * (8 * H + k))];
-            x1[8 * H + k] = lds[fa_c8_addr(c, u + 128 + 256 * (8 * H + k))];
+            x0[8 * H + k] = lds[fa_cw_addr<C>(c, u + 256 * (8 * H + k))];
+            x1[8 * H + k] = lds[fa_cw_addr<C>(c, u + 128 + 256 * (8 * H + k))];
         }
         if (H == 0) __syncthreads();
     }
@@ -646,35 +807,65 @@ FA_DEV void fa_col_inv8_tile(double* __restrict__ dst, int d2, int g, double n, 
     fa_inv16(x1, tw, n, ninv);
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-        dst[((uint64_t)(u + 256 * k) << d2) + col] = x0[k];
-        dst[((uint64_t)(u + 128 + 256 * k) << d2) + col] = x1[k];
+        dst[(uint64_t)(u + 256 * k) * FA_ROW_PITCH + col] = x0[k];
+        dst[(uint64_t)(u + 128 + 256 * k) * FA_ROW_PITCH + col] = x1[k];
     }
 }
 
 template <int KIND>
-__global__ void __launch_bounds__(1024) fa_k_col_fwd16w(fa_operand a, fa_operand b, int d2,
-                                                        double* __restrict__ A,
-                                                        double* __restrict__ B, uint64_t stride,
-                                                        fa_prime_set S, int np, int ngroups,
-                                                        int nops)
+FA_DEV void fa_col_fwd8_tile(const fa_operand& op, double* __restrict__ dst, int d2, int g,
+                             double n, double ninv, const double* __restrict__ w2,
+                             const double* __restrict__ pow2, double* lds)
 {
-    __shared__ double lds[FA_C8_LDS];
-    int i, g, z;
-    fa_col_split(fa_xcd_linear(), np, ngroups, &i, &g, &z);
-    if (z >= nops) return;
-    fa_col_fwd8_tile<KIND>(z ? b : a, (z ? B : A) + (uint64_t)i * stride, d2, g, S.p[i], S.pinv[i],
-                           S.w2[i], S.pow2[i], lds);
+    fa_col_fwdw_tile<8, KIND>(op, dst, d2, g, n, ninv, w2, pow2, lds);
 }
 
-__global__ void __launch_bounds__(1024) fa_k_col_inv16w(double* __restrict__ A, int d2,
-                                                        uint64_t stride, fa_prime_set S, int np,
-                                                        int ngroups)
+FA_DEV void fa_col_inv8_tile(double* __restrict__ dst, int d2, int g, double n, double ninv,
+                             const double* __restrict__ wi, double* lds)
 {
-    __shared__ double lds[FA_C8_LDS];
+    fa_col_invw_tile<8>(dst, d2, g, n, ninv, wi, lds);
+}
+
+// work index L -> (prime i, column group g, operand z) with the 16/C column
+// groups of one 128-byte line adjacent, as fa_col_split does for two
+template <int C>
+FA_DEV void fa_colw_split(unsigned L, int np, int ngroups, int* i, int* g, int* z)
+{
+    constexpr int LG = (C == 8) ? 1 : 2;   // log2(16 / C)
+    const int lo = L & ((1 << LG) - 1);
+    const unsigned L2 = L >> LG;
+    *i = (int)(L2 % (unsigned)np);
+    const unsigned rest = L2 / (unsigned)np;
+    const int lines = ngroups >> LG;
+    *g = ((int)(rest % (unsigned)lines) << LG) + lo;
+    *z = (int)(rest / (unsigned)lines);
+}
+
+template <int C, int KIND>
+__global__ void __launch_bounds__(128 * C) fa_k_col_fwd16w(fa_operand a, fa_operand b, int d2,
+                                                           double* __restrict__ A,
+                                                           double* __restrict__ B,
+                                                           uint64_t stride, fa_prime_set S,
+                                                           int np, int ngroups, int nops)
+{
+    __shared__ double lds[FA_CW_LDS(C)];
     int i, g, z;
-    fa_col_split(fa_xcd_linear(), np, ngroups, &i, &g, &z);
+    fa_colw_split<C>(fa_xcd_linear(), np, ngroups, &i, &g, &z);
+    if (z >= nops) return;
+    fa_col_fwdw_tile<C, KIND>(z ? b : a, (z ? B : A) + (uint64_t)i * stride, d2, g, S.p[i],
+                              S.pinv[i], S.w2[i], S.pow2[i], lds);
+}
+
+template <int C>
+__global__ void __launch_bounds__(128 * C) fa_k_col_inv16w(double* __restrict__ A, int d2,
+                                                           uint64_t stride, fa_prime_set S,
+                                                           int np, int ngroups)
+{
+    __shared__ double lds[FA_CW_LDS(C)];
+    int i, g, z;
+    fa_colw_split<C>(fa_xcd_linear(), np, ngroups, &i, &g, &z);
     if (z >= 1) return;
-    fa_col_inv8_tile(A + (uint64_t)i * stride, d2, g, S.p[i], S.pinv[i], S.w2inv[i], lds);
+    fa_col_invw_tile<C>(A + (uint64_t)i * stride, d2, g, S.p[i], S.pinv[i], S.w2inv[i], lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -758,7 +949,7 @@ __global__ void __launch_bounds__(256, 3) fa_kb_row_conv(const fa_bjob* __restri
     const fa_bjob& j = J[w.job];
     const int pi = j.offset + w.prime;
     const uint64_t r = (uint64_t)w.tile;
-    const uint64_t off = (uint64_t)w.prime * j.stride + (r << 12);
+    const uint64_t off = (uint64_t)w.prime * j.stride + r * FA_ROW_PITCH;
     fa_row_conv_tile(j.A + off, j.B + off, r, S.p[pi], S.pinv[pi], j.norm[w.prime], S.w2[pi],
                      S.w2inv[pi], xch, tab, rho);
 }
@@ -883,8 +1074,14 @@ int fa_batch_transform_bin(int depth, int kind, const fa_bjob* J, const uint64_t
     }
 }
 
-int fa_col_variant = 1;   // 1: 8-column tiles at d1 = 12, 0: 4-column tiles
+// column tiles at d1 = 12: 1 = 8 columns, 1024 threads; 2 = 4 columns, 512
+// threads, half-column LDS; 0 = 4 columns, 1024 threads, whole-column LDS
+int fa_col_variant = 1;
 extern "C" FA_API void fa_set_col_variant(int v) { fa_col_variant = v; }
+// row pass: 1 = three workgroups per CU (twiddles of the last stage formed
+// per layer, 168 registers), 0 = two workgroups per CU (all 15 held)
+int fa_row_variant = 1;
+extern "C" FA_API void fa_set_row_variant(int v) { fa_row_variant = v; }
 
 // ---------------------------------------------------------------------------
 // host launcher: transform lengths 2^13 .. 2^24 (row 2^12, column 2^1 ..
@@ -892,6 +1089,7 @@ extern "C" FA_API void fa_set_col_variant(int v) { fa_col_variant = v; }
 // ---------------------------------------------------------------------------
 
 extern int fa_col_variant;
+extern int fa_row_variant;
 
 template <int D1, int KIND>
 static int launch_fast(const fa_operand& oa, const fa_operand& ob, int nops, int np, double* A,
@@ -899,23 +1097,34 @@ static int launch_fast(const fa_operand& oa, const fa_operand& ob, int nops, int
 {
     typedef col_geom<D1> G;
     const int d2 = FA_ROW_LG;
-    const bool wide = (D1 == 12) && fa_col_variant == 1;
-    const int ngroups = (1 << d2) / (wide ? 8 : G::CPW);
+    const int wide = (D1 == 12 && (fa_col_variant == 1 || fa_col_variant == 2))
+                         ? (fa_col_variant == 1 ? 8 : 4) : 0;
+    const int ngroups = (1 << d2) / (wide ? wide : G::CPW);
     unsigned total = (unsigned)(np * ngroups * nops);
     unsigned grid = (total + 7) & ~7u;
-    if (wide)
-        FA_LAUNCH("fa_k_col_fwd16w", fa_k_col_fwd16w<KIND>, dim3(grid), dim3(1024), 0, s, oa, ob,
-                  d2, A, B, stride, S, np, ngroups, nops);
+    if (wide == 8)
+        FA_LAUNCH("fa_k_col_fwd16w", (fa_k_col_fwd16w<8, KIND>), dim3(grid), dim3(1024), 0, s, oa,
+                  ob, d2, A, B, stride, S, np, ngroups, nops);
+    else if (wide == 4)
+        FA_LAUNCH("fa_k_col_fwd16w", (fa_k_col_fwd16w<4, KIND>), dim3(grid), dim3(512), 0, s, oa,
+                  ob, d2, A, B, stride, S, np, ngroups, nops);
     else
         FA_LAUNCH("fa_k_col_fwd16", (fa_k_col_fwd16<D1, KIND>), dim3(grid), dim3(1024), 0, s, oa, ob,
                   d2, A, B, stride, S, np, ngroups, nops);
-    FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16, dim3(1u << D1, np), dim3(256), 0, s, A, B, stride,
-              S);
+    if (fa_row_variant == 1)
+        FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16lr, dim3(1u << D1, np), dim3(256), 0, s, A, B,
+                  stride, S);
+    else
+        FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16, dim3(1u << D1, np), dim3(256), 0, s, A, B,
+                  stride, S);
     unsigned total2 = (unsigned)(np * ngroups);
     unsigned grid2 = (total2 + 7) & ~7u;
-    if (wide)
-        FA_LAUNCH("fa_k_col_inv16w", fa_k_col_inv16w, dim3(grid2), dim3(1024), 0, s, A, d2, stride,
-                  S, np, ngroups);
+    if (wide == 8)
+        FA_LAUNCH("fa_k_col_inv16w", fa_k_col_inv16w<8>, dim3(grid2), dim3(1024), 0, s, A, d2,
+                  stride, S, np, ngroups);
+    else if (wide == 4)
+        FA_LAUNCH("fa_k_col_inv16w", fa_k_col_inv16w<4>, dim3(grid2), dim3(512), 0, s, A, d2,
+                  stride, S, np, ngroups);
     else
         FA_LAUNCH("fa_k_col_inv16", fa_k_col_inv16<D1>, dim3(grid2), dim3(1024), 0, s, A, d2, stride,
                   S, np, ngroups);

@@ -10,6 +10,27 @@
 #define FA_NPRIMES 8
 #define FA_MAX_DEPTH 24          // longest transform: 2^24 points
 #define FA_ROW_LG 12             // row length of the two-level decomposition
+
+// Residue arrays of a two-level transform (depth > 12) hold 2^(depth-12) rows
+// of 4096 points at a pitch of FA_ROW_PITCH doubles.  The column passes read
+// and write eight rows per wave instruction, one pitch apart; at a 32 KB
+// pitch those accesses fall on the same few HBM channels, and the 1 KB pad
+// spreads them (a column-pattern copy runs 1.5x faster with it).
+// Coefficient j of such an array sits at (j >> 12) * FA_ROW_PITCH + (j & 4095);
+// a one-level array (depth <= 12) is contiguous.  A two-level array's stride
+// exceeds 4096 doubles, which is how fa_res_pos tells the two apart.
+#define FA_ROW_PITCH 4224
+__host__ __device__ inline uint64_t fa_res_stride(int depth)
+{
+    return depth > FA_ROW_LG ? ((uint64_t)FA_ROW_PITCH << (depth - FA_ROW_LG))
+                             : ((uint64_t)1 << depth);
+}
+__host__ __device__ inline uint64_t fa_res_pos(uint64_t j, uint64_t stride)
+{
+    return stride > ((uint64_t)1 << FA_ROW_LG)
+               ? (j >> FA_ROW_LG) * FA_ROW_PITCH + (j & (((uint64_t)1 << FA_ROW_LG) - 1))
+               : j;
+}
 #define FA_LDS_COL_DOUBLES 8192  // LDS budget of one column tile (64 KiB)
 #define FA_CRT_MAXL 8            // limbs of M = prod p_i (np <= 8: < 400 bits)
 

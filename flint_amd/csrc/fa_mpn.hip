@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(256) fa_k_col_fwd(fa_operand a, fa_operand b, 
     fa_lds_fwd(lds, d1, lgC, 0, S.w2[i], n, ninv);
     for (int u = threadIdx.x; u < total; u += blockDim.x) {
         uint64_t row = (uint64_t)(u >> lgC), c = (uint64_t)(u & (C - 1));
-        dst[(row << d2) + col0 + c] = lds[u];
+        dst[row * FA_ROW_PITCH + col0 + c] = lds[u];
     }
 }
 
@@ -91,8 +91,8 @@ __global__ void __launch_bounds__(256) fa_k_row_conv(double* __restrict__ A,
     const double n = S.p[i], ninv = S.pinv[i];
     const int N2 = 1 << d2;
     const uint64_t r = blockIdx.x;
-    double* arow = A + (uint64_t)i * stride + (r << d2);
-    const double* brow = B + (uint64_t)i * stride + (r << d2);
+    double* arow = A + (uint64_t)i * stride + r * FA_ROW_PITCH;
+    const double* brow = B + (uint64_t)i * stride + r * FA_ROW_PITCH;
     double* sa = lds;
     double* sb = lds + N2;
     for (int x = threadIdx.x; x < N2; x += blockDim.x) {
@@ -123,13 +123,13 @@ __global__ void __launch_bounds__(256) fa_k_col_inv(double* __restrict__ A, int 
     const int total = (1 << d1) << lgC;
     for (int u = threadIdx.x; u < total; u += blockDim.x) {
         uint64_t row = (uint64_t)(u >> lgC), c = (uint64_t)(u & (C - 1));
-        lds[u] = dst[(row << d2) + col0 + c];
+        lds[u] = dst[row * FA_ROW_PITCH + col0 + c];
     }
     __syncthreads();
     fa_lds_inv(lds, d1, lgC, 0, S.w2inv[i], n, ninv);
     for (int u = threadIdx.x; u < total; u += blockDim.x) {
         uint64_t row = (uint64_t)(u >> lgC), c = (uint64_t)(u & (C - 1));
-        dst[(row << d2) + col0 + c] = lds[u];
+        dst[row * FA_ROW_PITCH + col0 + c] = lds[u];
     }
 }
 
@@ -151,7 +151,8 @@ FA_DEV void fa_crt_coeff(const double* __restrict__ Q, uint64_t stride, uint64_t
     for (int l = 0; l < FA_CRT_MAXL; l++) s[l] = 0;
     double qe = 0.0;
     for (int i = 0; i < C.np; i++) {
-        uint64_t x = fa_reduce_0n_u64(Q[(uint64_t)i * stride + j], C.p[i], C.pinv[i]);
+        uint64_t x = fa_reduce_0n_u64(Q[(uint64_t)i * stride + fa_res_pos(j, stride)], C.p[i],
+                                      C.pinv[i]);
         qe += (double)x * C.rp[i];
         uint64_t carry = 0;
 #pragma unroll
@@ -424,7 +425,8 @@ FA_DEV void fa_crt_value(const double* __restrict__ Q, uint64_t stride, uint64_t
     double qe = 0.0;
 #pragma unroll
     for (int i = 0; i < NP; i++) {
-        uint64_t x = fa_reduce_0n_u64(Q[(uint64_t)i * stride + j], C.p[i], C.pinv[i]);
+        uint64_t x = fa_reduce_0n_u64(Q[(uint64_t)i * stride + fa_res_pos(j, stride)], C.p[i],
+                                      C.pinv[i]);
         qe += (double)x * C.rp[i];
         uint64_t carry = 0;
 #pragma unroll
@@ -1051,7 +1053,7 @@ int fa_convolve(fa_thread_state* st, const fa_tables* T, int np, int offset, int
     fa_prime_set S;
     fa_prime_set_init(&S, T, np, depth, offset, 1);
     fa_operand oa = to_operand(a), ob = to_operand(b);
-    const uint64_t stride = UINT64_C(1) << depth;
+    const uint64_t stride = fa_res_stride(depth);
     if (depth <= FA_ROW_LG) {
         size_t lds = 2 * stride * sizeof(double);
         FA_LAUNCH("fa_k_small_conv", fa_k_small_conv, dim3(np), dim3(256), lds, stream, oa, ob, depth, A,
@@ -1095,7 +1097,7 @@ static size_t mpn_scratch_bytes(const fa_mpn_plan_t& P, uint64_t zn, uint64_t* n
     *ngroups = (zn + own - 1) / own;
     *nblk = (*ngroups + (FA_CRT_WG - 2)) / (FA_CRT_WG - 1);
     uint64_t nblk16 = (*ngroups + 29) / 30;   // covers the 31-group workgroups
-    size_t tr = ((size_t)P.np << P.depth) * sizeof(double);
+    size_t tr = (size_t)P.np * fa_res_stride(P.depth) * sizeof(double);
     size_t bytes = 2 * tr + *ngroups + 2 * nblk16 + 4096;
     return bytes;
 }
@@ -1117,7 +1119,7 @@ static int mpn_mul_device(uint64_t* z, const uint64_t* a, uint64_t an, const uin
     void* buf;
     rc = fa_scratch_reserve(st, bytes, &buf);
     if (rc) return rc;
-    size_t tr = ((size_t)P.np << P.depth) * sizeof(double);
+    size_t tr = (size_t)P.np * fa_res_stride(P.depth) * sizeof(double);
     mpn_scratch W;
     W.A = (double*)buf;
     W.B = (double*)((char*)buf + tr);
@@ -1134,7 +1136,7 @@ static int mpn_mul_device(uint64_t* z, const uint64_t* a, uint64_t an, const uin
 
     fa_crt_consts C;
     fa_crt_consts_init(&C, P.np, 0);
-    const uint64_t stride = UINT64_C(1) << P.depth;
+    const uint64_t stride = fa_res_stride(P.depth);
     if (fa_fast_enabled) {
         uint64_t nblk16 = (ngroups + (FA_CRT16_GROUPS - 2)) / (FA_CRT16_GROUPS - 1);
         rc = crt16(P.np, W.A, stride, C, P.zlen, P.bits, z, zn, ngroups, W.flags, W.agg, W.cin,

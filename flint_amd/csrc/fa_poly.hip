@@ -28,7 +28,8 @@ FA_DEV void fa_crt_coeff_poly(const double* __restrict__ Q, uint64_t stride, uin
     for (int l = 0; l < FA_CRT_MAXL; l++) s[l] = 0;
     double qe = 0.0;
     for (int i = 0; i < C.np; i++) {
-        uint64_t x = fa_reduce_0n_u64(Q[(uint64_t)i * stride + j], C.p[i], C.pinv[i]);
+        uint64_t x = fa_reduce_0n_u64(Q[(uint64_t)i * stride + fa_res_pos(j, stride)], C.p[i],
+                                      C.pinv[i]);
         qe += (double)x * C.rp[i];
         uint64_t carry = 0;
 #pragma unroll
@@ -280,7 +281,7 @@ static int poly_buffers(fa_thread_state* st, uint64_t an_words, uint64_t bn_word
                         int depth, uint64_t zwords, uint64_t** da, uint64_t** db, double** A,
                         double** B, uint64_t** dz)
 {
-    size_t tr = ((size_t)np << depth) * sizeof(double);
+    size_t tr = (size_t)np * fa_res_stride(depth) * sizeof(double);
     size_t bytes = 2 * tr + (an_words + bn_words + zwords) * sizeof(uint64_t) + 256;
     void* buf;
     int rc = fa_scratch_reserve(st, bytes, &buf);
@@ -350,7 +351,7 @@ extern "C" int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh, const u
     uint64_t d = n << norm;
     uint64_t dinv = (uint64_t)(((u128)0 - 1) / d - ((u128)1 << 64));
     FA_LAUNCH("fa_k_crt_nmod", fa_k_crt_nmod, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
-              A, UINT64_C(1) << J.depth, J.C, (uint64_t)zl, cnt, n, dinv, norm, dz);
+              A, fa_res_stride(J.depth), J.C, (uint64_t)zl, cnt, n, dinv, norm, dz);
     HIPTRY(hipGetLastError(), "crt launch");
     HIPTRY(hipMemcpyAsync(z, dz, cnt * sizeof(uint64_t), hipMemcpyDeviceToHost, s), "download");
     HIPTRY(hipStreamSynchronize(s), "synchronize");
@@ -479,7 +480,7 @@ extern "C" int fa_fmpz_poly_mul_mid(uint64_t* z, uint64_t zw, int64_t zl, int64_
     rc = fa_convolve(st, T, np, 0, depth, &pa, &pb, A, B, s);
     if (rc) return rc;
     FA_LAUNCH("fa_k_crt_fmpz", fa_k_crt_fmpz, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
-              A, UINT64_C(1) << depth, C, (uint64_t)zl, cnt, zw, dz);
+              A, fa_res_stride(depth), C, (uint64_t)zl, cnt, zw, dz);
     HIPTRY(hipGetLastError(), "crt launch");
     HIPTRY(hipMemcpyAsync(z, dz, cnt * zw * sizeof(uint64_t), hipMemcpyDeviceToHost, s),
            "download");

@@ -153,6 +153,11 @@ FA_API int fa_prof_read(char* names, double* ms, int64_t* counts, int max);
 FA_API void fa_set_fast_kernels(int on);
 /* column-pass tiles at 2^24 points: 1 = 8 columns (default), 0 = 4 columns */
 FA_API void fa_set_col_variant(int v);
+/* row pass variant of the two-level transforms: 1 = last-stage twiddles
+   formed per radix-4 layer, three workgroups per CU (default); 0 = all 15
+   held in registers, two workgroups per CU.  Tuning switch, no counterpart
+   in the reference. */
+FA_API void fa_set_row_variant(int v);
 /* CRT of integer products: 1 = radix-2^24 digits on the FP64 lanes
    (default), 0 = 64-bit limb arithmetic */
 FA_API void fa_set_crt_variant(int v);

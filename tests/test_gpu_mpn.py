@@ -151,20 +151,42 @@ def test_fast_kernels_every_depth(depth):
 
 @pytest.mark.parametrize("an,bn", [(1 << 24, 1 << 24), (15000000, 14000000)])
 def test_column_tile_variants_agree(an, bn):
-    """8-column and 4-column tiles of the 2^24-point column passes give the
-    same product"""
+    """8-column, 4-column half-LDS and 4-column whole-LDS tiles of the
+    2^24-point column passes give the same product"""
     import flint_amd as fa
     a, b = fa.synth_mpn_pair(0xC01, an, bn)
     print("col-variants", an, bn, fa.mpn_plan(an, bn), flush=True)
+    z = {}
     try:
-        fa.set_col_variant(0)
-        z0 = fa.mpn_mul(a, b)
+        for v in (0, 2, 1):
+            fa.set_col_variant(v)
+            z[v] = fa.mpn_mul(a, b)
     finally:
         fa.set_col_variant(1)
-    z1 = fa.mpn_mul(a, b)
-    assert np.array_equal(z0, z1)
+    z1 = z[1]
+    assert np.array_equal(z[0], z1)
+    assert np.array_equal(z[2], z1)
     q = (1 << 61) - 1
     assert (to_int(a) % q) * (to_int(b) % q) % q == to_int(z1) % q
+
+
+@pytest.mark.parametrize("an,bn", [(1 << 24, 1 << 24), (3000000, 2500000)])
+def test_row_variants_agree(an, bn):
+    """the row pass with all last-stage twiddles held and with them formed
+    per radix-4 layer give the same product"""
+    import flint_amd as fa
+    a, b = fa.synth_mpn_pair(0xC02, an, bn)
+    print("row-variants", an, bn, fa.mpn_plan(an, bn), flush=True)
+    z = {}
+    try:
+        for v in (1, 0):
+            fa.set_row_variant(v)
+            z[v] = fa.mpn_mul(a, b)
+    finally:
+        fa.set_row_variant(1)
+    assert np.array_equal(z[0], z[1])
+    q = (1 << 61) - 1
+    assert (to_int(a) % q) * (to_int(b) % q) % q == to_int(z[0]) % q
 
 
 @pytest.mark.parametrize("an,bn,np_,bits", [(65536, 65536, 6, 136), (5000, 4000, 2, 40),
