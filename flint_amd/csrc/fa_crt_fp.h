@@ -4,14 +4,17 @@
 // inverse transforms are x_i = c (M/p_i)^-1 mod p_i (the normalizer of the
 // pointwise product carries (M/p_i)^-1), the explicit CRT gives
 //     c = sum_i x_i (M/p_i) - k M,   k = floor(sum_i x_i / p_i)
-// (mpn_mul.c:1120-1180 restates it with 64-bit limbs).  Here the products
-// x_i (M/p_i) are formed in radix 2^24 in doubles: x_i = u_i + v_i 2^24 with
-// u_i < 2^24, v_i < 2^26, and (M/p_i) has 24-bit digits e_(i,d); every term
-// u e < 2^48 and v e < 2^50 is an exact FMA, and a column collects at most
-// six primes' terms, 6 (2^48 + 2^50) < 2^53, before it is carried into
-// 24-bit digits.  k = rint(sum_i x_i / p_i) is k_true or k_true + 1 (the
-// sum's rounding error is far below 1/2), so c' = S - k M lies in [-M, M)
-// and one conditional + M fixes the sign.
+// (mpn_mul.c:1120-1180 restates it with 64-bit limbs).  Any representative
+// of x_i serves: S = sum_i x_i (M/p_i) is c + k M for k = floor(S / M).  The
+// kernel takes the one in [-p_i/2, p_i/2].  The products x_i (M/p_i) are
+// formed in radix 2^24 in doubles: x_i = u_i + v_i 2^24 with 0 <= u_i < 2^24
+// and |v_i| <= 2^25 (v_i = floor(x_i / 2^24)), and (M/p_i) has 24-bit digits
+// e_(i,d); every term u e < 2^48 and |v e| < 2^49 is an exact FMA, and a
+// column collects at most six primes' terms, 6 (2^48 + 2^49) < 2^53, before
+// it is carried into 24-bit digits (signed floor division).
+// k = rint(sum_i x_i / p_i) is k_true or k_true + 1 (the sum's rounding
+// error is far below 1/2), so c' = S - k M lies in [-M, M) and one
+// conditional + M fixes the sign.
 #pragma once
 
 #include "fa_internal.h"
@@ -65,10 +68,15 @@ FA_DEV void fa_crt_fp_value(const double* __restrict__ Q, uint64_t stride, uint6
     double qe = 0.0;
 #pragma unroll
     for (int i = 0; i < NP; i++) {
-        if (i == 6) fa_crt_fp_carry<ND>(col);   // keep every column below 2^53
+        if (i == 6) {
+            // keep every column below 2^53; the partial sum may be negative,
+            // so its carry out of the top digit goes back into that digit
+            const double top = fa_crt_fp_carry<ND>(col);
+            col[ND - 1] = __builtin_fma(top, B, col[ND - 1]);
+        }
         const double n = F.p[i];
-        double x = fa_reduce_pm1n(Q[(uint64_t)i * stride + fa_res_pos(j, stride)], n, F.pinv[i]);
-        x = (x < 0.0) ? x + n : x;               // [0, n)
+        const double x = fa_reduce_pm1n(Q[(uint64_t)i * stride + fa_res_pos(j, stride)], n,
+                                        F.pinv[i]);   // [-n/2, n/2]
         qe = __builtin_fma(x, F.rp[i], qe);
         const double v = __builtin_floor(x * Binv);
         const double u = __builtin_fma(-v, B, x);

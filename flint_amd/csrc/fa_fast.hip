@@ -189,6 +189,10 @@ FA_DEV void fa_inv16_rowtab(double* x, const double* tab, int j, int bq, double 
 FA_DEV void fa_fwd16_rowlast(double* x, const double* rho, const double* __restrict__ w, int t,
                              double n, double ninv)
 {
+    // the table loads depend on this opaque copy of t, so they are issued
+    // here rather than hoisted to the kernel's start (where they would hold
+    // registers through both forward rows)
+    asm volatile("" : "+v"(t));
     {
         const double T0 = fa_row_entry(rho[8], w, t, n, ninv);
         const double T1 = fa_row_entry(rho[9], w, 2 * t, n, ninv);
@@ -209,6 +213,7 @@ FA_DEV void fa_fwd16_rowlast(double* x, const double* rho, const double* __restr
 FA_DEV void fa_inv16_rowlast(double* x, const double* rho, const double* __restrict__ w, int t,
                              double n, double ninv)
 {
+    asm volatile("" : "+v"(t));
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const int Qp = 4 * t + j;
@@ -950,8 +955,8 @@ __global__ void __launch_bounds__(256, 3) fa_kb_row_conv(const fa_bjob* __restri
     const int pi = j.offset + w.prime;
     const uint64_t r = (uint64_t)w.tile;
     const uint64_t off = (uint64_t)w.prime * j.stride + r * FA_ROW_PITCH;
-    fa_row_conv_tile(j.A + off, j.B + off, r, S.p[pi], S.pinv[pi], j.norm[w.prime], S.w2[pi],
-                     S.w2inv[pi], xch, tab, rho);
+    fa_row_conv_tile_lr(j.A + off, j.B + off, r, S.p[pi], S.pinv[pi], j.norm[w.prime], S.w2[pi],
+                        S.w2inv[pi], xch, tab, rho);
 }
 
 // whole product of one prime in LDS (depth <= 12): import, both forward

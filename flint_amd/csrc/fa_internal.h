@@ -19,7 +19,9 @@
 // Coefficient j of such an array sits at (j >> 12) * FA_ROW_PITCH + (j & 4095);
 // a one-level array (depth <= 12) is contiguous.  A two-level array's stride
 // exceeds 4096 doubles, which is how fa_res_pos tells the two apart.
+#ifndef FA_ROW_PITCH
 #define FA_ROW_PITCH 4224
+#endif
 __host__ __device__ inline uint64_t fa_res_stride(int depth)
 {
     return depth > FA_ROW_LG ? ((uint64_t)FA_ROW_PITCH << (depth - FA_ROW_LG))

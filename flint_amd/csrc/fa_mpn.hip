@@ -814,6 +814,9 @@ FA_DEV void fa_crt_fp_block(const double* __restrict__ Q, uint64_t stride,
         offs[e] = k * NLS + q - ((bits * k) >> 6);
     }
     const int nk = qin ? kmax - kmin + 1 : 0;
+    // no limb overlaps more than (64 NLS - 1) / bits + 1 coefficients: a
+    // bound the whole workgroup shares, so the loop below stops there
+    const int emax = (64 * NLS - 1) / bits + 1;
     __syncthreads();
 
     // 2. group sums and the carries inside each group
@@ -825,6 +828,7 @@ FA_DEV void fa_crt_fp_block(const double* __restrict__ Q, uint64_t stride,
         uint32_t hi = 0;
 #pragma unroll
         for (int e = 0; e < 8; e++) {
+            if (e >= emax) break;
             if (e < nk) {
                 const uint64_t v = base[offs[e]];
                 lo += v;

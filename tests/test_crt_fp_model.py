@@ -4,7 +4,8 @@ The GPU kernel forms sum_i x_i (M/p_i) - k M with exact FP64 FMAs on 24-bit
 digits.  This model replays the same steps on Python integers and asserts,
 at every FMA and carry step, that the value is an integer below 2^53 (so
 the double operation is exact), then checks the recombined coefficient
-against the residues' defining integer for every prime count 2..8,
+against the residues' defining integer for every prime count 2..8, with the
+residues in [0, p) and in [-p/2, p/2] (the kernel's),
 including the cases where the quotient estimate rounds up (c near M/2 and
 M - 1) and c = 0.
 """
@@ -51,7 +52,8 @@ def crt_fp_model(xs, primes):
     qe = 0.0
     for i, x in enumerate(xs):
         if i == 6:
-            carry()
+            top = carry()
+            col[ND - 1] = _exact(Fraction(col[ND - 1]) + Fraction(top) * B)
         x = float(x)
         qe = float(Fraction(x) * Fraction(rp[i]) + Fraction(qe))   # one rounding (fma)
         v = float(int(x) >> 24)
@@ -83,4 +85,7 @@ def test_crt_fp_model_exact(np_):
     cases += [M - 1 - rs.randrange(2 ** 40) for _ in range(10)]
     for c in cases:
         xs = [c * cofinv[i] % primes[i] for i in range(np_)]
+        assert crt_fp_model(xs, primes) == c
+        # the kernel's representatives in [-p/2, p/2]
+        xs = [x - p if 2 * x > p else x for x, p in zip(xs, primes)]
         assert crt_fp_model(xs, primes) == c
