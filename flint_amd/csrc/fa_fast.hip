@@ -667,42 +667,15 @@ FA_DEV int fa_cw_addr(int c, int row) { return C * PADX(row & 2047) + c; }
 // stage-1 row of group tc, element k
 FA_DEV int fa_c8_row1(int tc, int k) { return ((tc >> 4) << 8) + (tc & 15) + 16 * k; }
 
-template <int C, int KIND>
-FA_DEV void fa_col_fwdw_tile(const fa_operand& op, double* __restrict__ dst, int d2, int g,
-                             double n, double ninv, const double* __restrict__ w2,
-                             const double* __restrict__ pow2, double* lds)
+// stages 1 and 2 of a C-column tile after stage 0 (x0, x1 in the stage-0
+// layout), and the stores of the stage-2 layout
+template <int C>
+FA_DEV void fa_col_fwdw_rest(double* x0, double* x1, double* __restrict__ dst, int g, double n,
+                             double ninv, const double* __restrict__ w2, double* lds)
 {
     const int c = threadIdx.x & (C - 1), u = threadIdx.x / C;
     const uint64_t col = (uint64_t)g * C + c;
-    double x0[16], x1[16], tw[15];
-    // rows >= 2048 (elements >= 2^(d2+11)) all zero: the usual case of a
-    // product's operands, each at most half the transform length
-    const bool hz = fa_operand_len(op) <= (UINT64_C(1) << (d2 + 11));
-    fa_tw16(w2, 0, tw);
-    if (hz) {
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            x0[k] = fa_operand_value_k<KIND>(op, ((uint64_t)(u + 256 * k) << d2) + col, pow2, n,
-                                             ninv);
-            x1[k] = fa_operand_value_k<KIND>(op, ((uint64_t)(u + 128 + 256 * k) << d2) + col,
-                                             pow2, n, ninv);
-        }
-#pragma unroll
-        for (int k = 8; k < 16; k++) x0[k] = x1[k] = 0.0;
-        // stage 0: levels 0..3, block 0
-        fa_fwd16_hz(x0, tw, n, ninv);
-        fa_fwd16_hz(x1, tw, n, ninv);
-    } else {
-#pragma unroll
-        for (int k = 0; k < 16; k++) {
-            x0[k] = fa_operand_value_k<KIND>(op, ((uint64_t)(u + 256 * k) << d2) + col, pow2, n,
-                                             ninv);
-            x1[k] = fa_operand_value_k<KIND>(op, ((uint64_t)(u + 128 + 256 * k) << d2) + col,
-                                             pow2, n, ninv);
-        }
-        fa_fwd16(x0, tw, n, ninv);
-        fa_fwd16(x1, tw, n, ninv);
-    }
+    double tw[15];
     // exchange 0 -> 1 in two halves
     double y0[16], y1[16];
 #pragma unroll
@@ -751,6 +724,82 @@ FA_DEV void fa_col_fwdw_tile(const fa_operand& op, double* __restrict__ dst, int
         dst[(uint64_t)(16 * u + k) * FA_ROW_PITCH + col] = x0[k];
         dst[(uint64_t)(16 * (u + 128) + k) * FA_ROW_PITCH + col] = x1[k];
     }
+}
+
+template <int C, int KIND>
+FA_DEV void fa_col_fwdw_tile(const fa_operand& op, double* __restrict__ dst, int d2, int g,
+                             double n, double ninv, const double* __restrict__ w2,
+                             const double* __restrict__ pow2, double* lds)
+{
+    const int c = threadIdx.x & (C - 1), u = threadIdx.x / C;
+    const uint64_t col = (uint64_t)g * C + c;
+    double x0[16], x1[16], tw[15];
+    // rows >= 2048 (elements >= 2^(d2+11)) all zero: the usual case of a
+    // product's operands, each at most half the transform length
+    const bool hz = fa_operand_len(op) <= (UINT64_C(1) << (d2 + 11));
+    fa_tw16(w2, 0, tw);
+    if constexpr (KIND == FA_PACK_MPN8) {
+        // byte-aligned chunks: when the last chunk a lane reads (row u + 128
+        // + 256 (nk - 1)) has its 32-byte window inside the limbs for every
+        // lane of the wave, all of the wave's chunks do; they are then read
+        // without per-chunk checks, at offsets stepping by 256 rows.  Each
+        // import path runs the rest of the tile on its own (no join of
+        // differently allocated registers).
+        const int own = op.bits >> 3;
+        const uint64_t last = ((uint64_t)(u + 128 + 256 * (hz ? 7 : 15)) << d2) + col;
+        if (__all(((last * (uint64_t)own) & ~UINT64_C(3)) + 32 <= op.n * 8)) {
+            const char* base = (const char*)op.src;
+            const uint64_t B0 = (((uint64_t)u << d2) + col) * (uint64_t)own;
+            const uint64_t step = ((uint64_t)256 << d2) * (uint64_t)own;
+            const uint64_t half = ((uint64_t)128 << d2) * (uint64_t)own;
+            if (hz) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    x0[k] = fa_chunk_residue8_in(base, B0 + k * step, own, pow2, n, ninv);
+                    x1[k] = fa_chunk_residue8_in(base, B0 + half + k * step, own, pow2, n, ninv);
+                }
+#pragma unroll
+                for (int k = 8; k < 16; k++) x0[k] = x1[k] = 0.0;
+                fa_fwd16_hz(x0, tw, n, ninv);
+                fa_fwd16_hz(x1, tw, n, ninv);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 16; k++) {
+                    x0[k] = fa_chunk_residue8_in(base, B0 + k * step, own, pow2, n, ninv);
+                    x1[k] = fa_chunk_residue8_in(base, B0 + half + k * step, own, pow2, n, ninv);
+                }
+                fa_fwd16(x0, tw, n, ninv);
+                fa_fwd16(x1, tw, n, ninv);
+            }
+            fa_col_fwdw_rest<C>(x0, x1, dst, g, n, ninv, w2, lds);
+            return;
+        }
+    }
+    if (hz) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            x0[k] = fa_operand_value_k<KIND>(op, ((uint64_t)(u + 256 * k) << d2) + col, pow2, n,
+                                             ninv);
+            x1[k] = fa_operand_value_k<KIND>(op, ((uint64_t)(u + 128 + 256 * k) << d2) + col,
+                                             pow2, n, ninv);
+        }
+#pragma unroll
+        for (int k = 8; k < 16; k++) x0[k] = x1[k] = 0.0;
+        // stage 0: levels 0..3, block 0
+        fa_fwd16_hz(x0, tw, n, ninv);
+        fa_fwd16_hz(x1, tw, n, ninv);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            x0[k] = fa_operand_value_k<KIND>(op, ((uint64_t)(u + 256 * k) << d2) + col, pow2, n,
+                                             ninv);
+            x1[k] = fa_operand_value_k<KIND>(op, ((uint64_t)(u + 128 + 256 * k) << d2) + col,
+                                             pow2, n, ninv);
+        }
+        fa_fwd16(x0, tw, n, ninv);
+        fa_fwd16(x1, tw, n, ninv);
+    }
+    fa_col_fwdw_rest<C>(x0, x1, dst, g, n, ninv, w2, lds);
 }
 
 template <int C>

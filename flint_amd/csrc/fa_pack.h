@@ -121,6 +121,41 @@ FA_DEV double fa_chunk_residue8(const uint64_t* __restrict__ limbs, uint64_t nli
     return acc;
 }
 
+// fa_chunk_residue8 for a chunk whose 32-byte window [B & ~3, (B & ~3) + 32)
+// lies inside the limb array (no bounds checks)
+FA_DEV double fa_chunk_residue8_in(const char* __restrict__ base, uint64_t B, int own,
+                                   const double* __restrict__ pow2, double n, double ninv)
+{
+    const fa_u32x4_a4* q = (const fa_u32x4_a4*)(base + (B & ~UINT64_C(3)));
+    const fa_u32x4 lo4 = q[0], hi4 = q[1];
+    const uint32_t w[8] = {lo4.x, lo4.y, lo4.z, lo4.w, hi4.x, hi4.y, hi4.z, hi4.w};
+    const uint32_t s = (uint32_t)(B & 3);
+    uint32_t c[7];
+#pragma unroll
+    for (int m = 0; m < 7; m++) {
+        c[m] = __builtin_amdgcn_alignbyte(w[m + 1], w[m], s);
+        const int vb = own - 4 * m;
+        if (vb < 4) c[m] = (vb <= 0) ? 0u : (c[m] & ((1u << (8 * vb)) - 1u));
+    }
+    const double T16 = 65536.0, T32 = 4294967296.0;
+    double acc = __builtin_fma((double)(c[1] & 0xffffu), T32, (double)c[0]);
+    if (own > 6) {
+        const double p1 = __builtin_fma((double)c[2], T16, (double)(c[1] >> 16));
+        acc += fa_mulmod(p1, pow2[1], n, ninv);
+    }
+    if (own > 12) {
+        const double p2 = __builtin_fma((double)(c[4] & 0xffffu), T32, (double)c[3]);
+        acc += fa_mulmod(p2, pow2[2], n, ninv);
+    }
+    if (own > 18) {
+        const double p3 = __builtin_fma((double)c[5], T16, (double)(c[4] >> 16));
+        acc += fa_mulmod(p3, pow2[3], n, ninv);
+        if (own > 24) acc += fa_mulmod((double)c[6], pow2[4], n, ninv);
+        acc = fa_reduce_pm1n(acc, n, ninv);
+    }
+    return acc;
+}
+
 // ------------------------------------------------------------------------
 // operand import: the value of element x (coefficient index) of an operand
 // ------------------------------------------------------------------------
