@@ -48,7 +48,8 @@ KERNEL_PASSES = {
     "fa_k_col_fwd16": 2,
     "fa_k_row_conv16": 3,  # reads a and b rows, writes the product row
     "fa_k_col_inv16": 2,   # reads and writes the product columns
-    "fa_k_crt_mpn16": 1,   # reads the residues; writes the output limbs
+    "fa_k_crt_fp": 1,      # reads the residues; writes the output limbs
+    "fa_k_crt_mpn16": 1,
     "fa_k_col_fwd": 2,
     "fa_k_row_conv": 3,
     "fa_k_col_inv": 2,
@@ -62,7 +63,7 @@ def kernel_bytes(name, W):
     b = 8 * KERNEL_PASSES.get(name, 0) * pts
     if name.startswith("fa_k_col_fwd"):
         b += 8 * (W["an"] + W["bn"])
-    if name.startswith("fa_k_crt_mpn"):
+    if name.startswith("fa_k_crt_mpn") or name == "fa_k_crt_fp":
         b += 8 * (W["an"] + W["bn"])
     return b
 
