@@ -87,14 +87,12 @@ FA_DEV void fa_tw16(const double* __restrict__ w, uint64_t Q, double* t)
 // computed in registers.
 // ---------------------------------------------------------------------------
 
+// the mulmod lands in (-n, n); one rounding step brings it to [-n/2, n/2]
+// (up to the last bit), the range the butterflies' bounds assume of a twiddle
 FA_DEV double fa_row_entry(double rho, const double* __restrict__ w, int q, double n,
                            double ninv)
 {
-    const double hn = 0.5 * (n - 1.0);
-    double v = fa_mulmod(rho, w[q], n, ninv);
-    v = (v > hn) ? v - n : v;
-    v = (v < -hn) ? v + n : v;
-    return v;
+    return fa_reduce_pm1n(fa_mulmod(rho, w[q], n, ninv), n, ninv);
 }
 
 // rho[m] for m < 12 and the LDS entries of levels 0..7
