@@ -86,6 +86,7 @@ SIGNATURES = {
     "fa_set_fast_kernels": (None, [ctypes.c_int]),
     "fa_set_col_variant": (None, [ctypes.c_int]),
     "fa_set_row_variant": (None, [ctypes.c_int]),
+    "fa_set_prime_groups": (None, [ctypes.c_int]),
     "fa_set_crt_variant": (None, [ctypes.c_int]),
     "fa_probe_colcopy": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP]),
     "fa_batch_create": (ctypes.c_int, [ctypes.POINTER(_VP), _VP, _s64, ctypes.c_int]),
@@ -193,6 +194,12 @@ def set_row_variant(v):
     """row pass of the two-level transforms: 1 = three workgroups per CU,
     last-stage twiddles formed per layer (default); 0 = two, all held"""
     lib().fa_set_row_variant(int(v))
+
+
+def set_prime_groups(v):
+    """2^24-point products: primes in v groups on forked streams (default 2);
+    0 or 1 = one launch per pass over all primes"""
+    lib().fa_set_prime_groups(int(v))
 
 
 def set_col_variant(v):

@@ -1143,10 +1143,112 @@ extern "C" FA_API void fa_set_row_variant(int v) { fa_row_variant = v; }
 extern int fa_col_variant;
 extern int fa_row_variant;
 
+// the primes [h, np) of a prime set as a set of their own
+static fa_prime_set prime_subset(const fa_prime_set& S, int h, int np)
+{
+    fa_prime_set T = S;
+    for (int k = 0; k + h < np; k++) {
+        T.p[k] = S.p[h + k];
+        T.pinv[k] = S.pinv[h + k];
+        T.norm[k] = S.norm[h + k];
+        for (int e = 0; e < 8; e++) T.pow2[k][e] = S.pow2[h + k][e];
+        T.w2[k] = S.w2[h + k];
+        T.w2inv[k] = S.w2inv[h + k];
+    }
+    return T;
+}
+
+// further streams of the calling host thread (default priority) and the
+// events that fork them from and join them to the caller's stream
+#define FA_PIPE_MAX 3
+struct fa_aux {
+    int device = -1;
+    hipStream_t s[FA_PIPE_MAX - 1] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t fwd[FA_PIPE_MAX] = {};
+    hipEvent_t join[FA_PIPE_MAX - 1] = {};
+};
+
+static fa_aux* aux_streams()
+{
+    static thread_local fa_aux X;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    if (X.device == dev) return &X;
+    if (X.device >= 0) return nullptr;   // created on another device: none here
+    for (int k = 0; k < FA_PIPE_MAX - 1; k++)
+        if (hipStreamCreateWithFlags(&X.s[k], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&X.join[k], hipEventDisableTiming) != hipSuccess)
+            return nullptr;
+    for (int k = 0; k < FA_PIPE_MAX; k++)
+        if (hipEventCreateWithFlags(&X.fwd[k], hipEventDisableTiming) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&X.fork, hipEventDisableTiming) != hipSuccess) return nullptr;
+    X.device = dev;
+    return &X;
+}
+
+// prime groups of the 2^24-point passes on their own streams (0 or 1: one
+// launch per pass over all primes)
+int fa_pipe_groups = 2;
+extern "C" FA_API void fa_set_prime_groups(int v) { fa_pipe_groups = v; }
+
+// ng prime groups, group g on its own stream, each starting when the
+// previous group's forward column pass is done: the forward and row passes
+// of later groups (arithmetic-bound) run beside the row and inverse column
+// passes of earlier ones (the latter bound by HBM).  Same kernels, same
+// results as one launch per pass over all primes.
+template <int D1, int KIND>
+static int launch_fast_pipe(const fa_operand& oa, const fa_operand& ob, int np, int ng, double* A,
+                            double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s,
+                            fa_aux* X)
+{
+    const int d2 = FA_ROW_LG;
+    const int ngroups = (1 << d2) / 8;
+    if (hipEventRecord(X->fork, s) != hipSuccess) return fa_set_error(FA_E_HIP, "pipeline fork");
+    for (int g = 0; g < ng; g++) {
+        const int lo = np * g / ng, hi = np * (g + 1) / ng, m = hi - lo;
+        const fa_prime_set Sg = prime_subset(S, lo, np);
+        double* Ag = A + (uint64_t)lo * stride;
+        double* Bg = B + (uint64_t)lo * stride;
+        hipStream_t t = g ? X->s[g - 1] : s;
+        if (g && (hipStreamWaitEvent(t, X->fork, 0) != hipSuccess ||
+                  hipStreamWaitEvent(t, X->fwd[g - 1], 0) != hipSuccess))
+            return fa_set_error(FA_E_HIP, "pipeline fork");
+        const unsigned gf = ((unsigned)(m * ngroups * 2) + 7) & ~7u;
+        FA_LAUNCH("fa_k_col_fwd16w", (fa_k_col_fwd16w<8, KIND>), dim3(gf), dim3(1024), 0, t, oa,
+                  ob, d2, Ag, Bg, stride, Sg, m, ngroups, 2);
+        if (hipEventRecord(X->fwd[g], t) != hipSuccess)
+            return fa_set_error(FA_E_HIP, "pipeline event");
+        FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16lr, dim3(1u << D1, m), dim3(256), 0, t, Ag, Bg,
+                  stride, Sg);
+        const unsigned gi = ((unsigned)(m * ngroups) + 7) & ~7u;
+        FA_LAUNCH("fa_k_col_inv16w", fa_k_col_inv16w<8>, dim3(gi), dim3(1024), 0, t, Ag, d2, stride,
+                  Sg, m, ngroups);
+        if (g && hipEventRecord(X->join[g - 1], t) != hipSuccess)
+            return fa_set_error(FA_E_HIP, "pipeline join");
+    }
+    for (int g = 1; g < ng; g++)
+        if (hipStreamWaitEvent(s, X->join[g - 1], 0) != hipSuccess)
+            return fa_set_error(FA_E_HIP, "pipeline join");
+    return 1;
+}
+
 template <int D1, int KIND>
 static int launch_fast(const fa_operand& oa, const fa_operand& ob, int nops, int np, double* A,
                        double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s)
 {
+    if constexpr (D1 == 12) {
+        if (fa_pipe_groups >= 2 && nops == 2 && np >= 2 && fa_col_variant == 1 && fa_row_variant == 1 &&
+            !fa_prof_on()) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+                fa_aux* X = aux_streams();
+                const int ng = fa_pipe_groups < np ? fa_pipe_groups : np;
+                if (X && ng >= 2 && ng <= FA_PIPE_MAX)
+                    return launch_fast_pipe<D1, KIND>(oa, ob, np, ng, A, B, stride, S, s, X);
+            }
+        }
+    }
     typedef col_geom<D1> G;
     const int d2 = FA_ROW_LG;
     const int wide = (D1 == 12 && (fa_col_variant == 1 || fa_col_variant == 2))

@@ -158,6 +158,13 @@ FA_API void fa_set_col_variant(int v);
    held in registers, two workgroups per CU.  Tuning switch, no counterpart
    in the reference. */
 FA_API void fa_set_row_variant(int v);
+/* 2^24-point products: the primes split into v groups (2 or 3; default 2),
+   each on a stream of its own forked from and joined back to the caller's,
+   so one group's row and inverse passes run beside the next group's forward
+   passes; 0 or 1 = one launch per pass over all primes.  Not used while the
+   stream is being captured into a graph or kernel timing is on.  Tuning
+   switch, no counterpart in the reference. */
+FA_API void fa_set_prime_groups(int v);
 /* CRT of integer products: 1 = radix-2^24 digits on the FP64 lanes
    (default), 0 = 64-bit limb arithmetic */
 FA_API void fa_set_crt_variant(int v);

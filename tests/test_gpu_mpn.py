@@ -170,6 +170,41 @@ def test_column_tile_variants_agree(an, bn):
     assert (to_int(a) % q) * (to_int(b) % q) % q == to_int(z1) % q
 
 
+@pytest.mark.parametrize("an,bn", [(1 << 24, 1 << 24), (14000000, 16000000), (30000000, 2000000),
+                                   (1 << 24, 9000000)])
+def test_prime_groups_agree(an, bn):
+    """the 2^24-point passes in one launch per pass, and with the primes in two
+    or three groups on forked streams, give the same product (plans with 6
+    and 5 primes, an operand longer than half the transform); the device
+    entry point on a caller's stream included"""
+    import torch
+    import flint_amd as fa
+    a, b = fa.synth_mpn_pair(0xC03, an, bn)
+    plan = fa.mpn_plan(an, bn)
+    print("prime-groups", an, bn, plan, flush=True)
+    assert plan["depth"] == 24
+    z = {}
+    try:
+        for v in (0, 3, 2):
+            fa.set_prime_groups(v)
+            z[v] = fa.mpn_mul(a, b)
+        dev = torch.device("cuda:0")
+        ta = torch.from_numpy(a.view(np.int64)).to(dev)
+        tb = torch.from_numpy(b.view(np.int64)).to(dev)
+        tz = torch.zeros(an + bn, dtype=torch.int64, device=dev)
+        stream = torch.cuda.Stream()
+        fa.mpn_mul_device(tz.data_ptr(), ta.data_ptr(), an, tb.data_ptr(), bn, stream.cuda_stream)
+        stream.synchronize()
+        zd = tz.cpu().numpy().view(np.uint64)
+    finally:
+        fa.set_prime_groups(2)
+    assert np.array_equal(z[0], z[2])
+    assert np.array_equal(z[0], z[3])
+    assert np.array_equal(z[0], zd)
+    q = (1 << 61) - 1
+    assert (to_int(a) % q) * (to_int(b) % q) % q == to_int(z[0]) % q
+
+
 @pytest.mark.parametrize("an,bn", [(1 << 24, 1 << 24), (3000000, 2500000)])
 def test_row_variants_agree(an, bn):
     """the row pass with all last-stage twiddles held and with them formed
