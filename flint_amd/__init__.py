@@ -86,6 +86,7 @@ SIGNATURES = {
     "fa_set_fast_kernels": (None, [ctypes.c_int]),
     "fa_set_col_variant": (None, [ctypes.c_int]),
     "fa_set_row_variant": (None, [ctypes.c_int]),
+    "fa_set_ks_variant": (None, [ctypes.c_int]),
     "fa_set_prime_groups": (None, [ctypes.c_int]),
     "fa_set_crt_variant": (None, [ctypes.c_int]),
     "fa_probe_colcopy": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP]),
@@ -200,6 +201,12 @@ def set_prime_groups(v):
     """2^24-point products: primes in v groups on forked streams (default 2);
     0 or 1 = one launch per pass over all primes"""
     lib().fa_set_prime_groups(int(v))
+
+
+def set_ks_variant(v):
+    """Kronecker route for fmpz products beyond eight primes: 1 = signed slots,
+    one integer product (default); 0 = three products of non-negative parts"""
+    lib().fa_set_ks_variant(int(v))
 
 
 def set_col_variant(v):

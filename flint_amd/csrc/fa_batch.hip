@@ -147,6 +147,7 @@ struct fa_batch_s {
         uint64_t* z;
         uint64_t* scratch;
         uint64_t aw, an, bw, bn, sb, zw, zl, cnt;
+        int variant;
     };
     std::vector<ksrec> ks;
 };
@@ -157,7 +158,7 @@ static int batch_record(fa_batch_s* B, hipStream_t s)
     const fa_bjob* J = (const fa_bjob*)(d + B->jobs);
     int rc;
     for (auto& k : B->ks) {
-        rc = fa_ks_pre(k.a, k.aw, k.an, k.b, k.bw, k.bn, k.sb, k.scratch, s);
+        rc = fa_ks_pre_v(k.variant, k.a, k.aw, k.an, k.b, k.bw, k.bn, k.sb, k.scratch, s);
         if (rc) return rc;
     }
     for (auto& e : B->small) {
@@ -185,7 +186,7 @@ static int batch_record(fa_batch_s* B, hipStream_t s)
     rc = fa_batch_crt_poly(1, P, (const uint64_t*)(d + B->fmpz_work), B->nfmpz, s);
     if (rc) return rc;
     for (auto& k : B->ks) {
-        rc = fa_ks_post(k.sb, k.an, k.bn, k.scratch, k.z, k.zw, k.zl, k.cnt, s);
+        rc = fa_ks_post_v(k.variant, k.sb, k.an, k.bn, k.scratch, k.z, k.zw, k.zl, k.cnt, s);
         if (rc) return rc;
     }
     return FA_OK;
@@ -211,6 +212,9 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
         return fa_set_error(FA_E_ARG, "fa_batch_create: bad arguments");
     *out = nullptr;
     const bool dev = (flags & FA_BATCH_DEVICE) != 0;
+    // the Kronecker route the whole batch uses, fixed at creation
+    const int ks_variant = fa_ks_current_variant();
+    const int ks_nprod = fa_ks_nprod(ks_variant);
     fa_batch_s* B = new fa_batch_s();
     B->flags = flags;
     B->nreq = nreq;
@@ -298,7 +302,8 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
                 }
                 p.np = fa_fmpz_primes((uint64_t)q.bn, p.b1, p.b2);
                 if (!p.np && (flags & FA_BATCH_FMPZ_KS)) {
-                    // Kronecker substitution: three integer products of La x Lb limbs
+                    // Kronecker substitution: integer products of La x Lb limbs (one
+                    // with signed slots, three with the non-negative parts)
                     ksplan K;
                     K.r = r;
                     K.sb = fa_ks_slot_bits(p.b1, p.b2, (uint64_t)q.an, (uint64_t)q.bn);
@@ -306,7 +311,7 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
                     K.Lb = (q.bn * K.sb + 63) / 64;
                     p.ks = (int)ksv.size();
                     ksv.push_back(K);
-                    for (int k = 0; k < 3; k++) {
+                    for (int k = 0; k < ks_nprod; k++) {
                         fa_request v;
                         memset(&v, 0, sizeof(v));
                         v.kind = FA_REQ_MPN;
@@ -344,8 +349,8 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
     const int64_t ntot = nreq + (int64_t)V.size();
     for (auto& K : ksv) {
         // a failed internal product fails its fmpz request
-        for (int k = 0; k < 3; k++) {
-            const int64_t v = nreq + 3 * (int64_t)(&K - ksv.data()) + k;
+        for (int k = 0; k < ks_nprod; k++) {
+            const int64_t v = nreq + ks_nprod * (int64_t)(&K - ksv.data()) + k;
             if (req(v).status != 1) R[K.r].status = req(v).status;
         }
     }
@@ -367,7 +372,8 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
     for (size_t k = 0; k < ksv.size(); k++) {
         const fa_request& q = R[ksv[k].r];
         const plan& p = P[(size_t)ksv[k].r];
-        ks_scratch[k] = Lb.take(fa_ks_scratch_words(p.b1, p.b2, (uint64_t)q.an, (uint64_t)q.bn) * 8);
+        ks_scratch[k] = Lb.take(
+            fa_ks_scratch_words_v(ks_variant, p.b1, p.b2, (uint64_t)q.an, (uint64_t)q.bn) * 8);
         if (!dev) {
             offs& o = O[(size_t)ksv[k].r];
             o.a = Lin.take((size_t)q.an * (size_t)q.aw * 8);
@@ -565,6 +571,7 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
         K.zw = (uint64_t)q.zw;
         K.zl = (uint64_t)q.zl;
         K.cnt = (uint64_t)(q.zh - q.zl);
+        K.variant = ks_variant;
         B->ks.push_back(K);
         if (!dev) {
             B->h2d.push_back({(void*)(uintptr_t)o.a, q.a, (size_t)(q.an * q.aw) * 8});
@@ -588,8 +595,8 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
             uint64_t* ptrs[9];
             uint64_t lens[2];
             const fa_request& qs = R[ksv[kk].r];
-            fa_ks_products(ksv[kk].sb, (uint64_t)qs.an, (uint64_t)qs.bn,
-                           (uint64_t*)(db0 + ks_scratch[kk]), ptrs, lens);
+            fa_ks_products_v(ks_variant, ksv[kk].sb, (uint64_t)qs.an, (uint64_t)qs.bn,
+                             (uint64_t*)(db0 + ks_scratch[kk]), ptrs, lens);
             da = ptrs[3 * (p.virt - 1)];
             db = ptrs[3 * (p.virt - 1) + 1];
             dz = ptrs[3 * (p.virt - 1) + 2];

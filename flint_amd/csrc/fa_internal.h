@@ -187,13 +187,18 @@ int fa_batch_crt_mpn(int np, const fa_bmpn* M, const uint64_t* work, unsigned nw
 int fa_batch_carry(const fa_bmpn* M, const unsigned* jobs, unsigned njobs, hipStream_t s);
 int fa_nmod_primes(const fa_tables* T, uint64_t len, uint64_t n, int* np, int* offset);
 uint64_t fa_ks_slot_bits(int64_t b1, int64_t b2, uint64_t an, uint64_t bn);
+// Kronecker route: variant 1 = signed slots, one integer product; 0 = three
+// products of non-negative parts (fa_set_ks_variant)
+int fa_ks_current_variant();
+int fa_ks_nprod(int variant);
 uint64_t fa_ks_scratch_words(int64_t b1, int64_t b2, uint64_t an, uint64_t bn);
-void fa_ks_products(uint64_t sb, uint64_t an, uint64_t bn, uint64_t* scratch,
-                    uint64_t* ptrs[9], uint64_t lens[2]);
-int fa_ks_pre(const uint64_t* a, uint64_t aw, uint64_t an, const uint64_t* b, uint64_t bw,
-              uint64_t bn, uint64_t sb, uint64_t* scratch, hipStream_t s);
-int fa_ks_post(uint64_t sb, uint64_t an, uint64_t bn, uint64_t* scratch, uint64_t* z,
-               uint64_t zw, uint64_t zl, uint64_t cnt, hipStream_t s);
+uint64_t fa_ks_scratch_words_v(int variant, int64_t b1, int64_t b2, uint64_t an, uint64_t bn);
+int fa_ks_products_v(int variant, uint64_t sb, uint64_t an, uint64_t bn, uint64_t* scratch,
+                     uint64_t* ptrs[9], uint64_t lens[2]);
+int fa_ks_pre_v(int variant, const uint64_t* a, uint64_t aw, uint64_t an, const uint64_t* b,
+                uint64_t bw, uint64_t bn, uint64_t sb, uint64_t* scratch, hipStream_t s);
+int fa_ks_post_v(int variant, uint64_t sb, uint64_t an, uint64_t bn, uint64_t* scratch,
+                 uint64_t* z, uint64_t zw, uint64_t zl, uint64_t cnt, hipStream_t s);
 int fa_ks_device(uint64_t* z, uint64_t zw, uint64_t zl, uint64_t cnt, const uint64_t* a,
                  uint64_t aw, uint64_t an, const uint64_t* b, uint64_t bw, uint64_t bn,
                  int64_t b1, int64_t b2, uint64_t* scratch, hipStream_t s);

@@ -3,16 +3,19 @@
 // reference's callers take when _fmpz_poly_mul_mid_default_mpn_ctx declines:
 // fmpz_poly/mul.c:160, _fmpz_poly_mul_KS).
 //
-// Signed coefficients are packed without borrows: with a = a+ - a- (the
-// positive part and the negated negative part, coefficient by coefficient)
+// Two routes.  The default packs signed coefficients into s-bit slots with
+// borrows and needs one integer product (see the signed-slot kernels
+// below).  The other packs without borrows: with a = a+ - a- (the positive
+// part and the negated negative part, coefficient by coefficient)
 //     a b = 2 (a+ b+ + a- b-) - |a| |b|,
 // an identity of polynomials, so it holds coefficient by coefficient.  Each
 // of the three products has non-negative coefficients below
 // min(an, bn) 2^(|b1|+|b2|) < 2^(s-1) for the slot width
 //     s = |b1| + |b2| + bitcount(min(an, bn)) + 1,
 // so packing at s bits per coefficient puts every coefficient in its own
-// slot with no carry between slots.  Packing is a gather per output limb;
-// the combination is one thread per output coefficient.
+// slot with no carry between slots, at three times the product work.
+// Packing is a gather per output limb; the combination is one thread per
+// output coefficient.
 #include <stdio.h>
 #include <string.h>
 
@@ -129,6 +132,243 @@ __global__ void __launch_bounds__(256) fa_k_ks_combine(const uint64_t* __restric
     }
 }
 
+// ---------------------------------------------------------------------------
+// Signed slots, one product (the default).  With sigma = +1 or -1 chosen so
+// that sigma a has a positive top nonzero coefficient, the packed integer
+//     A = sum_i sigma a_i 2^(s i)
+// is positive; its base-2^s digits are d_i = (sigma a_i - br_i) mod 2^s with
+// the borrow br_0 = 0, br_(i+1) = [sigma a_i - br_i < 0], i.e. generate
+// [sigma a_i < 0] and propagate [a_i = 0]: a 1-bit scan over coefficients.
+// The product C = A B = sum_k c_k 2^(s k) with |c_k| < 2^(s-1) (the slot
+// width above) has base-2^s digits e_k, and
+//     c_k = e_k + t_(k-1) - 2^s t_k,  t_k = [e_k + t_(k-1) >= 2^(s-1)],
+// again a 1-bit scan: generate [e_k >= 2^(s-1)], propagate
+// [e_k = 2^(s-1) - 1].  So c_k is the s-bit field e_k + t_(k-1), read as a
+// signed s-bit number, times sigma_a sigma_b.
+// ---------------------------------------------------------------------------
+
+// one workgroup per operand: the lowest nonzero limb of each coefficient,
+// the sign sigma of the top nonzero coefficient (+1 for a zero operand) and
+// the borrow into each coefficient's slot.  Each thread keeps one
+// contiguous block of coefficients through both passes.
+__global__ void __launch_bounds__(1024) fa_k_ks_sflags(const uint64_t* __restrict__ a, int w,
+                                                       uint64_t n, int32_t* __restrict__ low,
+                                                       uint8_t* __restrict__ br,
+                                                       int32_t* __restrict__ sig)
+{
+    __shared__ int64_t stop[1024];
+    __shared__ uint8_t sg[1024], sp[1024];
+    const int t = threadIdx.x;
+    const uint64_t m = (n + 1023) / 1024;
+    const uint64_t i0 = (uint64_t)t * m < n ? (uint64_t)t * m : n;
+    const uint64_t i1 = i0 + m < n ? i0 + m : n;
+    int64_t top = -1;
+    for (uint64_t i = i0; i < i1; i++) {
+        const uint64_t* c = a + i * (uint64_t)w;
+        int l = 0;
+        while (l < w && c[l] == 0) l++;
+        low[i] = l;
+        if (l < w) top = (int64_t)i;
+    }
+    stop[t] = top;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if (t < h && stop[t + h] > stop[t]) stop[t] = stop[t + h];
+        __syncthreads();
+    }
+    const int64_t ti = stop[0];
+    const int negs = ti >= 0 ? (int)(a[(uint64_t)ti * w + w - 1] >> 63) : 0;
+    if (t == 0) *sig = negs ? -1 : 1;
+    // block aggregates of (generate, propagate), composed left to right
+    uint8_t G = 0, P = 1;
+    for (uint64_t i = i0; i < i1; i++) {
+        const int z = low[i] >= w;
+        const int neg = (int)(a[i * (uint64_t)w + w - 1] >> 63);
+        const uint8_t gi = (uint8_t)(!z && neg != negs), pi = (uint8_t)z;
+        G = (uint8_t)(gi | (pi & G));
+        P = (uint8_t)(P & pi);
+    }
+    sg[t] = G;
+    sp[t] = P;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        uint8_t g = sg[t], q = sp[t];
+        if (t >= d) {
+            g = (uint8_t)(g | (q & sg[t - d]));
+            q = (uint8_t)(q & sp[t - d]);
+        }
+        __syncthreads();
+        sg[t] = g;
+        sp[t] = q;
+        __syncthreads();
+    }
+    uint8_t cin = t ? sg[t - 1] : 0;
+    for (uint64_t i = i0; i < i1; i++) {
+        br[i] = cin;
+        const int z = low[i] >= w;
+        const int neg = (int)(a[i * (uint64_t)w + w - 1] >> 63);
+        cin = (uint8_t)((!z && neg != negs) | (z & cin));
+    }
+}
+
+// limb m (any m >= 0, sign-extended past w) of v = sigma c - br for a w-limb
+// two's complement coefficient c whose lowest nonzero limb is `low` (w if c
+// is zero); negs: sigma = -1
+FA_DEV uint64_t fa_ks_vlimb(const uint64_t* c, int w, int m, int negs, int br, int low)
+{
+    const bool zero = low >= w;
+    const uint64_t cs = (c[w - 1] >> 63) ? ~UINT64_C(0) : 0;
+    if (!negs) {
+        if (!br) return m < w ? c[m] : cs;
+        if (zero) return ~UINT64_C(0);                       // 0 - 1
+        return m < w ? c[m] - (m <= low ? 1u : 0u) : cs;     // c - 1, same sign as c
+    }
+    if (br) return m < w ? ~c[m] : ~cs;                      // -c - 1 = ~c
+    if (zero) return 0;
+    return m < w ? ~c[m] + (m <= low ? 1u : 0u) : ~cs;       // -c
+}
+
+// out[j] = limb j of sum_i d_i 2^(s i), d_i = (sigma c_i - br_i) mod 2^s
+__global__ void __launch_bounds__(256) fa_k_ks_spack(const uint64_t* __restrict__ a, int w,
+                                                     uint64_t n, const int32_t* __restrict__ low,
+                                                     const uint8_t* __restrict__ br,
+                                                     const int32_t* __restrict__ sig, uint64_t s,
+                                                     uint64_t* __restrict__ out, uint64_t L)
+{
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= L) return;
+    const int negs = *sig < 0;
+    const uint64_t bit0 = 64 * j;
+    const uint64_t i0 = bit0 / s;
+    uint64_t acc = 0;
+    for (uint64_t i = i0; i < n && s * i < bit0 + 64; i++) {
+        const uint64_t* c = a + i * (uint64_t)w;
+        const int lo = low[i], b = br[i];
+        const int64_t rel = (int64_t)bit0 - (int64_t)(s * i);   // slot bit at bit0
+        uint64_t win;
+        if (rel >= 0) {
+            const int q = (int)(rel >> 6), r = (int)(rel & 63);
+            win = fa_ks_vlimb(c, w, q, negs, b, lo) >> r;
+            if (r) win |= fa_ks_vlimb(c, w, q + 1, negs, b, lo) << (64 - r);
+        } else {
+            win = fa_ks_vlimb(c, w, 0, negs, b, lo) << (int)(-rel);
+        }
+        // keep window positions p with 0 <= rel + p < s
+        const int64_t plo = rel < 0 ? -rel : 0;
+        const int64_t phi = (int64_t)s - rel < 64 ? (int64_t)s - rel : 64;
+        if (phi <= plo) continue;
+        uint64_t mask = phi == 64 ? ~UINT64_C(0) : ((UINT64_C(1) << phi) - 1);
+        mask &= ~((UINT64_C(1) << plo) - 1);
+        acc |= win & mask;
+    }
+    out[j] = acc;
+}
+
+// generate and propagate of slot k of the product: the slot's top bit, and
+// whether its other s - 1 bits are all ones with the top bit clear
+FA_DEV void fa_ks_slot_gp(const uint64_t* __restrict__ C, uint64_t nP, uint64_t s, uint64_t k,
+                          uint8_t* g, uint8_t* p)
+{
+    const uint64_t tb = s * k + s - 1;
+    const uint64_t q = tb >> 6;
+    const uint8_t top = q < nP ? (uint8_t)((C[q] >> (tb & 63)) & 1) : 0;
+    *g = top;
+    uint8_t ones = !top;
+    for (uint64_t lb = 0; ones && lb < s - 1; lb += 64) {
+        const uint64_t nb = s - 1 - lb < 64 ? s - 1 - lb : 64;
+        const uint64_t want = nb == 64 ? ~UINT64_C(0) : ((UINT64_C(1) << nb) - 1);
+        const uint64_t v = fa_ks_slot(C, nP, s, k, (int)(lb >> 6));
+        if ((v & want) != want) ones = 0;
+    }
+    *p = ones;
+}
+
+// one workgroup: t_k for the slots k < K of the product (the borrow each
+// slot passes up), as bytes
+__global__ void __launch_bounds__(1024) fa_k_ks_uscan(const uint64_t* __restrict__ C,
+                                                      uint64_t nP, uint64_t s, uint64_t K,
+                                                      uint8_t* __restrict__ tk)
+{
+    __shared__ uint8_t sg[1024], sp[1024];
+    const int t = threadIdx.x;
+    const uint64_t m = (K + 1023) / 1024;
+    const uint64_t k0 = (uint64_t)t * m < K ? (uint64_t)t * m : K;
+    const uint64_t k1 = k0 + m < K ? k0 + m : K;
+    uint8_t G = 0, P = 1;
+    for (uint64_t k = k0; k < k1; k++) {
+        uint8_t g, p;
+        fa_ks_slot_gp(C, nP, s, k, &g, &p);
+        tk[k] = (uint8_t)(g | (p << 1));
+        G = (uint8_t)(g | (p & G));
+        P = (uint8_t)(P & p);
+    }
+    sg[t] = G;
+    sp[t] = P;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        uint8_t g = sg[t], q = sp[t];
+        if (t >= d) {
+            g = (uint8_t)(g | (q & sg[t - d]));
+            q = (uint8_t)(q & sp[t - d]);
+        }
+        __syncthreads();
+        sg[t] = g;
+        sp[t] = q;
+        __syncthreads();
+    }
+    uint8_t cin = t ? sg[t - 1] : 0;
+    for (uint64_t k = k0; k < k1; k++) {
+        const uint8_t f = tk[k];
+        cin = (uint8_t)((f & 1) | ((f >> 1) & cin));
+        tk[k] = cin;
+    }
+}
+
+// z[t] = coefficient zl + t: the s-bit field e_k + t_(k-1) read as signed,
+// times sigma_a sigma_b, as zw limbs; zero past the product's length zn
+__global__ void __launch_bounds__(256) fa_k_ks_unpack(const uint64_t* __restrict__ C,
+                                                      uint64_t nP, uint64_t s,
+                                                      const uint8_t* __restrict__ tk,
+                                                      const int32_t* __restrict__ sig,
+                                                      uint64_t zl, uint64_t cnt, uint64_t zn,
+                                                      int zw, uint64_t* __restrict__ z)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    uint64_t* o = z + t * (uint64_t)zw;
+    const uint64_t k = zl + t;
+    if (k >= zn) {
+        for (int l = 0; l < zw; l++) o[l] = 0;
+        return;
+    }
+    const int neg = (sig[0] < 0) != (sig[1] < 0);
+    uint64_t carry = k ? tk[k - 1] : 0;
+    uint64_t fill = 0;
+    uint64_t ncarry = 1;   // two's complement negation: ~v + 1
+    for (int l = 0; l < zw; l++) {
+        const uint64_t lb = 64 * (uint64_t)l;
+        uint64_t v;
+        if (lb < s) {
+            v = fa_ks_slot(C, nP, s, k, l) + carry;
+            carry = (v < carry) ? 1 : 0;
+            if (lb + 64 >= s) {
+                const uint64_t nb = s - lb;   // 1..64 field bits in this limb
+                const uint64_t m = nb == 64 ? ~UINT64_C(0) : ((UINT64_C(1) << nb) - 1);
+                v &= m;
+                fill = ((v >> (nb - 1)) & 1) ? ~UINT64_C(0) : 0;
+                v |= fill & ~m;
+            }
+        } else {
+            v = fill;
+        }
+        if (neg) {
+            v = ~v + ncarry;
+            ncarry = (ncarry && v == 0) ? 1 : 0;
+        }
+        o[l] = v;
+    }
+}
+
 static uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 static int bitcount(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
 
@@ -139,8 +379,16 @@ uint64_t fa_ks_slot_bits(int64_t b1, int64_t b2, uint64_t an, uint64_t bn)
     return (uint64_t)(b1 < 0 ? -b1 : b1) + (uint64_t)(b2 < 0 ? -b2 : b2) + bitcount(m) + 1;
 }
 
-// scratch layout of one KS product: the packed parts of a and b, the
-// three products, the low-limb indices
+// 1: signed slots and one integer product (default); 0: the three products
+// of non-negative parts
+int fa_ks_variant = 1;
+extern "C" FA_API void fa_set_ks_variant(int v) { fa_ks_variant = v ? 1 : 0; }
+int fa_ks_current_variant() { return fa_ks_variant; }
+int fa_ks_nprod(int variant) { return variant ? 1 : 3; }
+
+// scratch layout of one KS product: the packed operands, the products, the
+// low-limb indices; for signed slots also the borrows, the product's slot
+// borrows and the two signs
 struct ks_layout {
     uint64_t La, Lb, nP;
     uint64_t* pa[3];
@@ -148,43 +396,84 @@ struct ks_layout {
     uint64_t* pr[3];
     int32_t* lowa;
     int32_t* lowb;
+    uint8_t* bra;
+    uint8_t* brb;
+    uint8_t* tk;
+    int32_t* sig;
 };
 
-static ks_layout ks_lay(uint64_t sb, uint64_t an, uint64_t bn, uint64_t* scratch)
+static ks_layout ks_lay(int variant, uint64_t sb, uint64_t an, uint64_t bn, uint64_t* scratch)
 {
     ks_layout K;
+    memset(&K, 0, sizeof(K));
+    const int np = fa_ks_nprod(variant);
     K.La = cdiv(an * sb, 64);
     K.Lb = cdiv(bn * sb, 64);
     K.nP = K.La + K.Lb;
     uint64_t* w = scratch;
-    for (int k = 0; k < 3; k++) { K.pa[k] = w; w += K.La; }
-    for (int k = 0; k < 3; k++) { K.pb[k] = w; w += K.Lb; }
-    for (int k = 0; k < 3; k++) { K.pr[k] = w; w += K.nP; }
+    for (int k = 0; k < np; k++) { K.pa[k] = w; w += K.La; }
+    for (int k = 0; k < np; k++) { K.pb[k] = w; w += K.Lb; }
+    for (int k = 0; k < np; k++) { K.pr[k] = w; w += K.nP; }
     K.lowa = (int32_t*)w;
     K.lowb = K.lowa + an;
+    if (variant) {
+        K.sig = K.lowb + bn;                  // two int32
+        K.bra = (uint8_t*)(K.sig + 2);
+        K.brb = K.bra + an;
+        K.tk = K.brb + bn;                    // an + bn slots
+    }
     return K;
 }
 
-// the three integer products of a KS product: out[k] = {a, b, z} device
-// pointers and out_n = {La, Lb}
-void fa_ks_products(uint64_t sb, uint64_t an, uint64_t bn, uint64_t* scratch,
-                    uint64_t* ptrs[9], uint64_t lens[2])
+// scratch words of a KS product
+uint64_t fa_ks_scratch_words_v(int variant, int64_t b1, int64_t b2, uint64_t an, uint64_t bn)
 {
-    ks_layout K = ks_lay(sb, an, bn, scratch);
-    for (int k = 0; k < 3; k++) {
+    const uint64_t sb = fa_ks_slot_bits(b1, b2, an, bn);
+    const uint64_t La = cdiv(an * sb, 64), Lb = cdiv(bn * sb, 64);
+    const uint64_t np = (uint64_t)fa_ks_nprod(variant);
+    uint64_t w = np * (La + Lb) + np * (La + Lb) + cdiv(an + bn, 2) + 8;
+    if (variant) w += 1 + cdiv(2 * (an + bn), 8);
+    return w;
+}
+
+uint64_t fa_ks_scratch_words(int64_t b1, int64_t b2, uint64_t an, uint64_t bn)
+{
+    return fa_ks_scratch_words_v(fa_ks_variant, b1, b2, an, bn);
+}
+
+// the integer products of a KS product: ptrs[3k .. 3k+2] = {a, b, z} device
+// pointers of product k, lens = {La, Lb}; returns the number of products
+int fa_ks_products_v(int variant, uint64_t sb, uint64_t an, uint64_t bn, uint64_t* scratch,
+                     uint64_t* ptrs[9], uint64_t lens[2])
+{
+    ks_layout K = ks_lay(variant, sb, an, bn, scratch);
+    const int np = fa_ks_nprod(variant);
+    for (int k = 0; k < np; k++) {
         ptrs[3 * k] = K.pa[k];
         ptrs[3 * k + 1] = K.pb[k];
         ptrs[3 * k + 2] = K.pr[k];
     }
     lens[0] = K.La;
     lens[1] = K.Lb;
+    return np;
 }
 
 // packing before the products
-int fa_ks_pre(const uint64_t* a, uint64_t aw, uint64_t an, const uint64_t* b, uint64_t bw,
-              uint64_t bn, uint64_t sb, uint64_t* scratch, hipStream_t s)
+int fa_ks_pre_v(int variant, const uint64_t* a, uint64_t aw, uint64_t an, const uint64_t* b,
+                uint64_t bw, uint64_t bn, uint64_t sb, uint64_t* scratch, hipStream_t s)
 {
-    ks_layout K = ks_lay(sb, an, bn, scratch);
+    ks_layout K = ks_lay(variant, sb, an, bn, scratch);
+    if (variant) {
+        FA_LAUNCH("fa_k_ks_sflags", fa_k_ks_sflags, dim3(1), dim3(1024), 0, s, a, (int)aw, an,
+                  K.lowa, K.bra, K.sig);
+        FA_LAUNCH("fa_k_ks_sflags", fa_k_ks_sflags, dim3(1), dim3(1024), 0, s, b, (int)bw, bn,
+                  K.lowb, K.brb, K.sig + 1);
+        FA_LAUNCH("fa_k_ks_spack", fa_k_ks_spack, dim3((unsigned)cdiv(K.La, 256)), dim3(256), 0, s,
+                  a, (int)aw, an, K.lowa, K.bra, K.sig, sb, K.pa[0], K.La);
+        FA_LAUNCH("fa_k_ks_spack", fa_k_ks_spack, dim3((unsigned)cdiv(K.Lb, 256)), dim3(256), 0, s,
+                  b, (int)bw, bn, K.lowb, K.brb, K.sig + 1, sb, K.pb[0], K.Lb);
+        return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "ks pack launch");
+    }
     FA_LAUNCH("fa_k_ks_low", fa_k_ks_low, dim3((unsigned)cdiv(an, 256)), dim3(256), 0, s, a,
               (int)aw, an, K.lowa);
     FA_LAUNCH("fa_k_ks_low", fa_k_ks_low, dim3((unsigned)cdiv(bn, 256)), dim3(256), 0, s, b,
@@ -200,41 +489,44 @@ int fa_ks_pre(const uint64_t* a, uint64_t aw, uint64_t an, const uint64_t* b, ui
 }
 
 // recombination after the products
-int fa_ks_post(uint64_t sb, uint64_t an, uint64_t bn, uint64_t* scratch, uint64_t* z,
-               uint64_t zw, uint64_t zl, uint64_t cnt, hipStream_t s)
+int fa_ks_post_v(int variant, uint64_t sb, uint64_t an, uint64_t bn, uint64_t* scratch,
+                 uint64_t* z, uint64_t zw, uint64_t zl, uint64_t cnt, hipStream_t s)
 {
-    ks_layout K = ks_lay(sb, an, bn, scratch);
+    ks_layout K = ks_lay(variant, sb, an, bn, scratch);
+    const uint64_t zn = an + bn - 1;
+    if (variant) {
+        const uint64_t top = zl + cnt < zn ? zl + cnt : zn;   // slots the window reads up to
+        if (top > 0)
+            FA_LAUNCH("fa_k_ks_uscan", fa_k_ks_uscan, dim3(1), dim3(1024), 0, s, K.pr[0], K.nP, sb,
+                      top, K.tk);
+        FA_LAUNCH("fa_k_ks_unpack", fa_k_ks_unpack, dim3((unsigned)cdiv(cnt, 256)), dim3(256), 0, s,
+                  K.pr[0], K.nP, sb, K.tk, K.sig, zl, cnt, zn, (int)zw, z);
+        return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "ks unpack launch");
+    }
     FA_LAUNCH("fa_k_ks_combine", fa_k_ks_combine, dim3((unsigned)cdiv(cnt, 256)), dim3(256), 0, s,
-              K.pr[0], K.pr[1], K.pr[2], K.nP, sb, zl, cnt, an + bn - 1, (int)zw, z);
+              K.pr[0], K.pr[1], K.pr[2], K.nP, sb, zl, cnt, zn, (int)zw, z);
     return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "ks combine launch");
 }
 
-// device form: a (an x aw), b (bn x bw) resident; scratch holds the packed
-// operands, the three products and the low-limb indices; enqueued on s
+// device form: a (an x aw), b (bn x bw) resident; scratch of
+// fa_ks_scratch_words; enqueued on s
 int fa_ks_device(uint64_t* z, uint64_t zw, uint64_t zl, uint64_t cnt, const uint64_t* a,
                  uint64_t aw, uint64_t an, const uint64_t* b, uint64_t bw, uint64_t bn,
                  int64_t b1, int64_t b2, uint64_t* scratch, hipStream_t s)
 {
+    const int variant = fa_ks_variant;
     const uint64_t sb = fa_ks_slot_bits(b1, b2, an, bn);
-    int rc = fa_ks_pre(a, aw, an, b, bw, bn, sb, scratch, s);
+    int rc = fa_ks_pre_v(variant, a, aw, an, b, bw, bn, sb, scratch, s);
     if (rc) return rc;
     uint64_t* ptrs[9];
     uint64_t lens[2];
-    fa_ks_products(sb, an, bn, scratch, ptrs, lens);
-    for (int k = 0; k < 3; k++) {
+    const int np = fa_ks_products_v(variant, sb, an, bn, scratch, ptrs, lens);
+    for (int k = 0; k < np; k++) {
         rc = fa_mpn_mul_device(ptrs[3 * k + 2], ptrs[3 * k], (int64_t)lens[0], ptrs[3 * k + 1],
                                (int64_t)lens[1], s);
         if (rc) return rc;
     }
-    return fa_ks_post(sb, an, bn, scratch, z, zw, zl, cnt, s);
-}
-
-// scratch words of fa_ks_device
-uint64_t fa_ks_scratch_words(int64_t b1, int64_t b2, uint64_t an, uint64_t bn)
-{
-    const uint64_t sb = fa_ks_slot_bits(b1, b2, an, bn);
-    const uint64_t La = cdiv(an * sb, 64), Lb = cdiv(bn * sb, 64);
-    return 3 * La + 3 * Lb + 3 * (La + Lb) + cdiv(an + bn, 2) + 8;
+    return fa_ks_post_v(variant, sb, an, bn, scratch, z, zw, zl, cnt, s);
 }
 
 // host-pointer form: the product the reference's caller computes when the

@@ -214,6 +214,11 @@ typedef struct {
    path declines, by Kronecker substitution onto the integer product
    (fmpz_poly/mul.c:160, _fmpz_poly_mul_KS); arguments and output as
    fa_fmpz_poly_mul_mid; always computes (returns 1) */
+/* Kronecker route of fa_fmpz_poly_mul_ks and of FA_BATCH_FMPZ_KS batches
+   (fixed when a batch is created): 1 = signed slots, one integer product
+   (default); 0 = three integer products of the non-negative parts.
+   Tuning switch, no counterpart in the reference. */
+FA_API void fa_set_ks_variant(int v);
 FA_API int fa_fmpz_poly_mul_ks(uint64_t* z, uint64_t zw, int64_t zl, int64_t zh,
                                const uint64_t* a, uint64_t aw, int64_t an, const uint64_t* b,
                                uint64_t bw, int64_t bn);
