@@ -149,7 +149,11 @@ struct fa_batch_s {
         uint64_t aw, an, bw, bn, sb, zw, zl, cnt;
         int variant;
     };
-    std::vector<ksrec> ks;
+    std::vector<ksrec> ks;     // three-product route: launches per product
+    // signed-slot route: records and work lists of all KS products, one
+    // launch per step
+    size_t ks_rec = 0, ks_packw = 0, ks_unpackw = 0;
+    unsigned nks = 0, nks_pack = 0, nks_unpack = 0;
 };
 
 static int batch_record(fa_batch_s* B, hipStream_t s)
@@ -161,6 +165,9 @@ static int batch_record(fa_batch_s* B, hipStream_t s)
         rc = fa_ks_pre_v(k.variant, k.a, k.aw, k.an, k.b, k.bw, k.bn, k.sb, k.scratch, s);
         if (rc) return rc;
     }
+    rc = fa_batch_ks_pre((const fa_bks*)(d + B->ks_rec), B->nks,
+                         (const uint64_t*)(d + B->ks_packw), B->nks_pack, s);
+    if (rc) return rc;
     for (auto& e : B->small) {
         rc = fa_batch_small_bin(e.first.depth, e.first.kind, J, (const uint64_t*)(d + e.second.first),
                                 e.second.second, B->S, s);
@@ -189,6 +196,9 @@ static int batch_record(fa_batch_s* B, hipStream_t s)
         rc = fa_ks_post_v(k.variant, k.sb, k.an, k.bn, k.scratch, k.z, k.zw, k.zl, k.cnt, s);
         if (rc) return rc;
     }
+    rc = fa_batch_ks_post((const fa_bks*)(d + B->ks_rec), B->nks,
+                          (const uint64_t*)(d + B->ks_unpackw), B->nks_unpack, s);
+    if (rc) return rc;
     return FA_OK;
 }
 
@@ -533,6 +543,29 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
                                                              (unsigned)e.second.size()}});
     B->carry_jobs = L.take(carry.size() * 4 + 8);
     B->ncarry = (unsigned)carry.size();
+    // signed-slot KS products: record index k of ks_live, work items in
+    // blocks of 256 packed limbs (per operand) and 256 output coefficients
+    std::vector<size_t> ks_live;
+    std::vector<uint64_t> ks_pw, ks_uw;
+    if (ks_variant) {
+        for (size_t k = 0; k < ksv.size(); k++) {
+            const fa_request& q = R[ksv[k].r];
+            if (q.status != 1) continue;
+            const uint64_t idx = ks_live.size();
+            ks_live.push_back(k);
+            for (uint64_t j = 0; j < (ksv[k].La + 255) / 256; j++) ks_pw.push_back(idx | (j << 32));
+            for (uint64_t j = 0; j < (ksv[k].Lb + 255) / 256; j++)
+                ks_pw.push_back(idx | (UINT64_C(1) << 31) | (j << 32));
+            const uint64_t cnt = (uint64_t)(q.zh - q.zl);
+            for (uint64_t j = 0; j < (cnt + 255) / 256; j++) ks_uw.push_back(idx | (j << 32));
+        }
+    }
+    B->nks = (unsigned)ks_live.size();
+    B->nks_pack = (unsigned)ks_pw.size();
+    B->nks_unpack = (unsigned)ks_uw.size();
+    B->ks_rec = L.take(ks_live.size() * sizeof(fa_bks) + 8);
+    B->ks_packw = L.take(ks_pw.size() * 8 + 8);
+    B->ks_unpackw = L.take(ks_uw.size() * 8 + 8);
     B->nmod_work = L.take(nmod_w.size() * 8 + 8);
     B->fmpz_work = L.take(fmpz_w.size() * 8 + 8);
     B->nnmod = (unsigned)nmod_w.size();
@@ -572,7 +605,7 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
         K.zl = (uint64_t)q.zl;
         K.cnt = (uint64_t)(q.zh - q.zl);
         K.variant = ks_variant;
-        B->ks.push_back(K);
+        if (!ks_variant) B->ks.push_back(K);
         if (!dev) {
             B->h2d.push_back({(void*)(uintptr_t)o.a, q.a, (size_t)(q.an * q.aw) * 8});
             B->h2d.push_back({(void*)(uintptr_t)o.b, q.b, (size_t)(q.bn * q.bw) * 8});
@@ -654,6 +687,22 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
         for (auto& e : crt_bins) { put(B->crt_mpn[k].second.first, e.second.data(), e.second.size() * 8); k++; }
     }
     put(B->carry_jobs, carry.data(), carry.size() * 4);
+    {
+        std::vector<fa_bks> recs(ks_live.size());
+        for (size_t i = 0; i < ks_live.size(); i++) {
+            const size_t k = ks_live[i];
+            const fa_request& q = R[ksv[k].r];
+            const offs& o = O[(size_t)ksv[k].r];
+            fa_ks_record(ksv[k].sb, dev ? q.a : (const uint64_t*)(din + o.a), (uint64_t)q.aw,
+                         (uint64_t)q.an, dev ? q.b : (const uint64_t*)(din + o.b), (uint64_t)q.bw,
+                         (uint64_t)q.bn, (uint64_t*)(db0 + ks_scratch[k]),
+                         dev ? q.z : (uint64_t*)(dout + o.z), (uint64_t)q.zw, (uint64_t)q.zl,
+                         (uint64_t)(q.zh - q.zl), &recs[i]);
+        }
+        put(B->ks_rec, recs.data(), recs.size() * sizeof(fa_bks));
+        put(B->ks_packw, ks_pw.data(), ks_pw.size() * 8);
+        put(B->ks_unpackw, ks_uw.data(), ks_uw.size() * 8);
+    }
     put(B->nmod_work, nmod_w.data(), nmod_w.size() * 8);
     put(B->fmpz_work, fmpz_w.data(), fmpz_w.size() * 8);
     for (auto& e : pconsts) {

@@ -189,6 +189,28 @@ int fa_nmod_primes(const fa_tables* T, uint64_t len, uint64_t n, int* np, int* o
 uint64_t fa_ks_slot_bits(int64_t b1, int64_t b2, uint64_t an, uint64_t bn);
 // Kronecker route: variant 1 = signed slots, one integer product; 0 = three
 // products of non-negative parts (fa_set_ks_variant)
+struct fa_bks {                 // one signed-slot KS product of a batch
+    const uint64_t* a;
+    const uint64_t* b;
+    uint64_t* z;
+    uint64_t aw, an, bw, bn, s, La, Lb, nP, zw, zl, cnt, zn, top;
+    int32_t* lowa;
+    int32_t* lowb;
+    uint8_t* bra;
+    uint8_t* brb;
+    uint8_t* tk;
+    int32_t* sig;
+    uint64_t* pa;
+    uint64_t* pb;
+    uint64_t* pr;
+};
+void fa_ks_record(uint64_t sb, const uint64_t* a, uint64_t aw, uint64_t an, const uint64_t* b,
+                  uint64_t bw, uint64_t bn, uint64_t* scratch, uint64_t* z, uint64_t zw,
+                  uint64_t zl, uint64_t cnt, fa_bks* r);
+int fa_batch_ks_pre(const fa_bks* R, unsigned nks, const uint64_t* work, unsigned nwork,
+                    hipStream_t s);
+int fa_batch_ks_post(const fa_bks* R, unsigned nks, const uint64_t* work, unsigned nwork,
+                     hipStream_t s);
 int fa_ks_current_variant();
 int fa_ks_nprod(int variant);
 uint64_t fa_ks_scratch_words(int64_t b1, int64_t b2, uint64_t an, uint64_t bn);

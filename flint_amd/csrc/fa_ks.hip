@@ -151,13 +151,10 @@ __global__ void __launch_bounds__(256) fa_k_ks_combine(const uint64_t* __restric
 // the sign sigma of the top nonzero coefficient (+1 for a zero operand) and
 // the borrow into each coefficient's slot.  Each thread keeps one
 // contiguous block of coefficients through both passes.
-__global__ void __launch_bounds__(1024) fa_k_ks_sflags(const uint64_t* __restrict__ a, int w,
-                                                       uint64_t n, int32_t* __restrict__ low,
-                                                       uint8_t* __restrict__ br,
-                                                       int32_t* __restrict__ sig)
+FA_DEV void ks_sflags_body(const uint64_t* __restrict__ a, int w, uint64_t n,
+                           int32_t* __restrict__ low, uint8_t* __restrict__ br,
+                           int32_t* __restrict__ sig, int64_t* stop, uint8_t* sg, uint8_t* sp)
 {
-    __shared__ int64_t stop[1024];
-    __shared__ uint8_t sg[1024], sp[1024];
     const int t = threadIdx.x;
     const uint64_t m = (n + 1023) / 1024;
     const uint64_t i0 = (uint64_t)t * m < n ? (uint64_t)t * m : n;
@@ -211,6 +208,16 @@ __global__ void __launch_bounds__(1024) fa_k_ks_sflags(const uint64_t* __restric
     }
 }
 
+__global__ void __launch_bounds__(1024) fa_k_ks_sflags(const uint64_t* __restrict__ a, int w,
+                                                       uint64_t n, int32_t* __restrict__ low,
+                                                       uint8_t* __restrict__ br,
+                                                       int32_t* __restrict__ sig)
+{
+    __shared__ int64_t stop[1024];
+    __shared__ uint8_t sg[1024], sp[1024];
+    ks_sflags_body(a, w, n, low, br, sig, stop, sg, sp);
+}
+
 // limb m (any m >= 0, sign-extended past w) of v = sigma c - br for a w-limb
 // two's complement coefficient c whose lowest nonzero limb is `low` (w if c
 // is zero); negs: sigma = -1
@@ -229,15 +236,10 @@ FA_DEV uint64_t fa_ks_vlimb(const uint64_t* c, int w, int m, int negs, int br, i
 }
 
 // out[j] = limb j of sum_i d_i 2^(s i), d_i = (sigma c_i - br_i) mod 2^s
-__global__ void __launch_bounds__(256) fa_k_ks_spack(const uint64_t* __restrict__ a, int w,
-                                                     uint64_t n, const int32_t* __restrict__ low,
-                                                     const uint8_t* __restrict__ br,
-                                                     const int32_t* __restrict__ sig, uint64_t s,
-                                                     uint64_t* __restrict__ out, uint64_t L)
+FA_DEV uint64_t ks_spack_limb(const uint64_t* __restrict__ a, int w, uint64_t n,
+                              const int32_t* __restrict__ low, const uint8_t* __restrict__ br,
+                              int negs, uint64_t s, uint64_t j)
 {
-    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= L) return;
-    const int negs = *sig < 0;
     const uint64_t bit0 = 64 * j;
     const uint64_t i0 = bit0 / s;
     uint64_t acc = 0;
@@ -261,7 +263,18 @@ __global__ void __launch_bounds__(256) fa_k_ks_spack(const uint64_t* __restrict_
         mask &= ~((UINT64_C(1) << plo) - 1);
         acc |= win & mask;
     }
-    out[j] = acc;
+    return acc;
+}
+
+__global__ void __launch_bounds__(256) fa_k_ks_spack(const uint64_t* __restrict__ a, int w,
+                                                     uint64_t n, const int32_t* __restrict__ low,
+                                                     const uint8_t* __restrict__ br,
+                                                     const int32_t* __restrict__ sig, uint64_t s,
+                                                     uint64_t* __restrict__ out, uint64_t L)
+{
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= L) return;
+    out[j] = ks_spack_limb(a, w, n, low, br, *sig < 0, s, j);
 }
 
 // generate and propagate of slot k of the product: the slot's top bit, and
@@ -285,11 +298,9 @@ FA_DEV void fa_ks_slot_gp(const uint64_t* __restrict__ C, uint64_t nP, uint64_t 
 
 // one workgroup: t_k for the slots k < K of the product (the borrow each
 // slot passes up), as bytes
-__global__ void __launch_bounds__(1024) fa_k_ks_uscan(const uint64_t* __restrict__ C,
-                                                      uint64_t nP, uint64_t s, uint64_t K,
-                                                      uint8_t* __restrict__ tk)
+FA_DEV void ks_uscan_body(const uint64_t* __restrict__ C, uint64_t nP, uint64_t s, uint64_t K,
+                          uint8_t* __restrict__ tk, uint8_t* sg, uint8_t* sp)
 {
-    __shared__ uint8_t sg[1024], sp[1024];
     const int t = threadIdx.x;
     const uint64_t m = (K + 1023) / 1024;
     const uint64_t k0 = (uint64_t)t * m < K ? (uint64_t)t * m : K;
@@ -324,17 +335,20 @@ __global__ void __launch_bounds__(1024) fa_k_ks_uscan(const uint64_t* __restrict
     }
 }
 
+__global__ void __launch_bounds__(1024) fa_k_ks_uscan(const uint64_t* __restrict__ C,
+                                                      uint64_t nP, uint64_t s, uint64_t K,
+                                                      uint8_t* __restrict__ tk)
+{
+    __shared__ uint8_t sg[1024], sp[1024];
+    ks_uscan_body(C, nP, s, K, tk, sg, sp);
+}
+
 // z[t] = coefficient zl + t: the s-bit field e_k + t_(k-1) read as signed,
 // times sigma_a sigma_b, as zw limbs; zero past the product's length zn
-__global__ void __launch_bounds__(256) fa_k_ks_unpack(const uint64_t* __restrict__ C,
-                                                      uint64_t nP, uint64_t s,
-                                                      const uint8_t* __restrict__ tk,
-                                                      const int32_t* __restrict__ sig,
-                                                      uint64_t zl, uint64_t cnt, uint64_t zn,
-                                                      int zw, uint64_t* __restrict__ z)
+FA_DEV void ks_unpack_one(const uint64_t* __restrict__ C, uint64_t nP, uint64_t s,
+                          const uint8_t* __restrict__ tk, const int32_t* __restrict__ sig,
+                          uint64_t zl, uint64_t zn, int zw, uint64_t* __restrict__ z, uint64_t t)
 {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cnt) return;
     uint64_t* o = z + t * (uint64_t)zw;
     const uint64_t k = zl + t;
     if (k >= zn) {
@@ -367,6 +381,91 @@ __global__ void __launch_bounds__(256) fa_k_ks_unpack(const uint64_t* __restrict
         }
         o[l] = v;
     }
+}
+
+__global__ void __launch_bounds__(256) fa_k_ks_unpack(const uint64_t* __restrict__ C,
+                                                      uint64_t nP, uint64_t s,
+                                                      const uint8_t* __restrict__ tk,
+                                                      const int32_t* __restrict__ sig,
+                                                      uint64_t zl, uint64_t cnt, uint64_t zn,
+                                                      int zw, uint64_t* __restrict__ z)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    ks_unpack_one(C, nP, s, tk, sig, zl, zn, zw, z, t);
+}
+
+// ---------------------------------------------------------------------------
+// batched signed-slot route: every KS product of a batch in one launch per
+// step.  Flags and slot scans take one workgroup per operand or product;
+// packing and unpacking take work items (record | operand << 31 | block of
+// 256 limbs or coefficients << 32).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) fa_kb_ks_sflags(const fa_bks* __restrict__ R, unsigned nks)
+{
+    __shared__ int64_t stop[1024];
+    __shared__ uint8_t sg[1024], sp[1024];
+    if (blockIdx.x >= 2 * nks) return;
+    const fa_bks& r = R[blockIdx.x >> 1];
+    if (blockIdx.x & 1)
+        ks_sflags_body(r.b, (int)r.bw, r.bn, r.lowb, r.brb, r.sig + 1, stop, sg, sp);
+    else
+        ks_sflags_body(r.a, (int)r.aw, r.an, r.lowa, r.bra, r.sig, stop, sg, sp);
+}
+
+__global__ void __launch_bounds__(256) fa_kb_ks_spack(const fa_bks* __restrict__ R,
+                                                      const uint64_t* __restrict__ work,
+                                                      unsigned nwork)
+{
+    if (blockIdx.x >= nwork) return;
+    const uint64_t wi = work[blockIdx.x];
+    const fa_bks& r = R[(unsigned)(wi & 0x7fffffffu)];
+    const int op = (int)((wi >> 31) & 1);
+    const uint64_t j = (wi >> 32) * 256 + threadIdx.x;
+    if (op) {
+        if (j < r.Lb) r.pb[j] = ks_spack_limb(r.b, (int)r.bw, r.bn, r.lowb, r.brb, r.sig[1] < 0, r.s, j);
+    } else {
+        if (j < r.La) r.pa[j] = ks_spack_limb(r.a, (int)r.aw, r.an, r.lowa, r.bra, r.sig[0] < 0, r.s, j);
+    }
+}
+
+__global__ void __launch_bounds__(1024) fa_kb_ks_uscan(const fa_bks* __restrict__ R, unsigned nks)
+{
+    __shared__ uint8_t sg[1024], sp[1024];
+    if (blockIdx.x >= nks) return;
+    const fa_bks& r = R[blockIdx.x];
+    if (r.top > 0) ks_uscan_body(r.pr, r.nP, r.s, r.top, r.tk, sg, sp);
+}
+
+__global__ void __launch_bounds__(256) fa_kb_ks_unpack(const fa_bks* __restrict__ R,
+                                                       const uint64_t* __restrict__ work,
+                                                       unsigned nwork)
+{
+    if (blockIdx.x >= nwork) return;
+    const uint64_t wi = work[blockIdx.x];
+    const fa_bks& r = R[(unsigned)(wi & 0x7fffffffu)];
+    const uint64_t t = (wi >> 32) * 256 + threadIdx.x;
+    if (t < r.cnt) ks_unpack_one(r.pr, r.nP, r.s, r.tk, r.sig, r.zl, r.zn, (int)r.zw, r.z, t);
+}
+
+int fa_batch_ks_pre(const fa_bks* R, unsigned nks, const uint64_t* work, unsigned nwork,
+                    hipStream_t s)
+{
+    if (!nks) return FA_OK;
+    FA_LAUNCH("fa_kb_ks_sflags", fa_kb_ks_sflags, dim3(2 * nks), dim3(1024), 0, s, R, nks);
+    if (nwork)
+        FA_LAUNCH("fa_kb_ks_spack", fa_kb_ks_spack, dim3(nwork), dim3(256), 0, s, R, work, nwork);
+    return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "batch ks pack launch");
+}
+
+int fa_batch_ks_post(const fa_bks* R, unsigned nks, const uint64_t* work, unsigned nwork,
+                     hipStream_t s)
+{
+    if (!nks) return FA_OK;
+    FA_LAUNCH("fa_kb_ks_uscan", fa_kb_ks_uscan, dim3(nks), dim3(1024), 0, s, R, nks);
+    if (nwork)
+        FA_LAUNCH("fa_kb_ks_unpack", fa_kb_ks_unpack, dim3(nwork), dim3(256), 0, s, R, work, nwork);
+    return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "batch ks unpack launch");
 }
 
 static uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
@@ -456,6 +555,41 @@ int fa_ks_products_v(int variant, uint64_t sb, uint64_t an, uint64_t bn, uint64_
     lens[0] = K.La;
     lens[1] = K.Lb;
     return np;
+}
+
+// the device record of a signed-slot KS product (scratch of
+// fa_ks_scratch_words_v(1, ...)); operands and output as the caller's
+void fa_ks_record(uint64_t sb, const uint64_t* a, uint64_t aw, uint64_t an, const uint64_t* b,
+                  uint64_t bw, uint64_t bn, uint64_t* scratch, uint64_t* z, uint64_t zw,
+                  uint64_t zl, uint64_t cnt, fa_bks* r)
+{
+    ks_layout K = ks_lay(1, sb, an, bn, scratch);
+    memset(r, 0, sizeof(*r));
+    r->a = a;
+    r->b = b;
+    r->z = z;
+    r->aw = aw;
+    r->an = an;
+    r->bw = bw;
+    r->bn = bn;
+    r->s = sb;
+    r->La = K.La;
+    r->Lb = K.Lb;
+    r->nP = K.nP;
+    r->zw = zw;
+    r->zl = zl;
+    r->cnt = cnt;
+    r->zn = an + bn - 1;
+    r->top = zl + cnt < r->zn ? zl + cnt : r->zn;
+    r->lowa = K.lowa;
+    r->lowb = K.lowb;
+    r->bra = K.bra;
+    r->brb = K.brb;
+    r->tk = K.tk;
+    r->sig = K.sig;
+    r->pa = K.pa[0];
+    r->pb = K.pb[0];
+    r->pr = K.pr[0];
 }
 
 // packing before the products
