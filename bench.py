@@ -179,9 +179,10 @@ def batched_main(args, torch, fa):
     if cur:
         chunks.append(cur)
 
-    # host-pointer batches: one untimed pass sizes the per-thread device and
-    # pinned buffers the one-shot batch borrows, then the timed pass
-    for ch in chunks[:1]:
+    # host-pointer batches: one untimed pass over every chunk sizes the
+    # per-thread device and pinned buffers the one-shot batch borrows (to the
+    # largest chunk), then the timed pass
+    for ch in chunks:
         fa.batch_mul([reqs_p[k] for k in ch], ks=True)
     th0 = time.perf_counter()
     batch_out = {}
