@@ -357,7 +357,11 @@ static int grow(void** buf, size_t* size, size_t bytes, hipStream_t s)
 {
     if (*size >= bytes) return FA_OK;
     if (*buf) {
-        (void)hipStreamSynchronize(s);
+        // earlier calls may have used the buffer on a caller's stream (and
+        // the streams forked from it), not only on the thread's own: wait
+        // for the whole device before freeing (growth is rare)
+        (void)s;
+        (void)hipDeviceSynchronize();
         (void)hipFree(*buf);
         *buf = NULL;
         *size = 0;

@@ -680,9 +680,12 @@ extern "C" int fa_fmpz_poly_mul_ks(uint64_t* z, uint64_t zw, int64_t zl, int64_t
     const uint64_t cnt = (uint64_t)(zh - zl);
     const uint64_t wa = (uint64_t)an * aw, wb = (uint64_t)bn * bw, wz = cnt * zw;
     const uint64_t ws = fa_ks_scratch_words(b1, b2, (uint64_t)an, (uint64_t)bn);
-    uint64_t* d = nullptr;
-    if (hipMalloc((void**)&d, (wa + wb + wz + ws) * 8) != hipSuccess)
-        return fa_set_error(FA_E_NOMEM, "fa_fmpz_poly_mul_ks: device memory");
+    // operands, output and KS scratch in the thread's grow-only staging
+    // buffer (the integer product inside uses the separate scratch buffer)
+    void* hb = nullptr;
+    int rc0 = fa_hbuf_reserve(st, (wa + wb + wz + ws) * 8, &hb);
+    if (rc0) return rc0;
+    uint64_t* d = (uint64_t*)hb;
     uint64_t *da = d, *db = d + wa, *dz = db + wb, *dw = dz + wz;
     hipStream_t s = st->stream;
     int rc = FA_OK;
@@ -696,6 +699,5 @@ extern "C" int fa_fmpz_poly_mul_ks(uint64_t* z, uint64_t zw, int64_t zl, int64_t
         rc = fa_set_error(FA_E_HIP, "fa_fmpz_poly_mul_ks: download");
     if (hipStreamSynchronize(s) != hipSuccess && !rc)
         rc = fa_set_error(FA_E_HIP, "fa_fmpz_poly_mul_ks: %s", hipGetErrorString(hipGetLastError()));
-    (void)hipFree(d);
     return rc ? rc : 1;
 }
