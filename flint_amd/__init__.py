@@ -86,6 +86,7 @@ SIGNATURES = {
     "fa_set_fast_kernels": (None, [ctypes.c_int]),
     "fa_set_col_variant": (None, [ctypes.c_int]),
     "fa_set_row_variant": (None, [ctypes.c_int]),
+    "fa_set_mpn_block_cap": (None, [ctypes.c_uint64]),
     "fa_set_ks_variant": (None, [ctypes.c_int]),
     "fa_set_prime_groups": (None, [ctypes.c_int]),
     "fa_set_crt_variant": (None, [ctypes.c_int]),
@@ -207,6 +208,12 @@ def set_ks_variant(v):
     """Kronecker route for fmpz products beyond eight primes: 1 = signed slots,
     one integer product (default); 0 = three products of non-negative parts"""
     lib().fa_set_ks_variant(int(v))
+
+
+def set_mpn_block_cap(limbs):
+    """integer products above `limbs` total limbs go in device-added block
+    products (0: only those no single plan covers)"""
+    lib().fa_set_mpn_block_cap(ctypes.c_uint64(int(limbs)))
 
 
 def set_col_variant(v):

@@ -390,6 +390,13 @@ int fa_hbuf_reserve(fa_thread_state* st, size_t bytes, void** out)
     return rc;
 }
 
+int fa_tbuf_reserve(fa_thread_state* st, size_t bytes, void** out)
+{
+    int rc = grow(&st->tbuf, &st->tbuf_size, bytes, st->stream);
+    *out = st->tbuf;
+    return rc;
+}
+
 extern "C" int fa_sync(void)
 {
     fa_thread_state* st = fa_state();

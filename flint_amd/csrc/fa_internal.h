@@ -84,6 +84,8 @@ struct fa_thread_state {
     size_t scratch_size;
     void* hbuf;                    // device staging for host-pointer calls
     size_t hbuf_size;
+    void* tbuf;                    // block products of integer products above one plan
+    size_t tbuf_size;
 };
 
 // mpn plan chosen for the GPU
@@ -104,6 +106,7 @@ int fa_mulmod_satisfies_bounds(uint64_t n);
 fa_thread_state* fa_state();
 int fa_scratch_reserve(fa_thread_state* st, size_t bytes, void** out);
 int fa_hbuf_reserve(fa_thread_state* st, size_t bytes, void** out);
+int fa_tbuf_reserve(fa_thread_state* st, size_t bytes, void** out);
 void fa_crt_consts_init(fa_crt_consts* C, int np, int offset);
 uint64_t fa_crt_cop(int np, int i);         // (M/p_i) mod p_i
 int fa_mpn_choose_plan(uint64_t an, uint64_t bn, fa_mpn_plan_t* P);

@@ -1106,8 +1106,260 @@ static size_t mpn_scratch_bytes(const fa_mpn_plan_t& P, uint64_t zn, uint64_t* n
     return bytes;
 }
 
+// ------------------------------------------------------------------------
+// integer products above one 2^24-point plan: block products added into z.
+// z[off, zn) += p[0, len) with the carry rippling to the top of z: a local
+// pass (sums, per-limb generate/propagate, per-block aggregates), a scan of
+// the block aggregates (one workgroup), and a pass applying the carries.
+// Blocks are 1024 limbs, 4 per thread.
+// ------------------------------------------------------------------------
+#define FA_ADD_BLK 1024
+
+__global__ void __launch_bounds__(256) fa_k_add_local(uint64_t* __restrict__ z, uint64_t n,
+                                                      const uint64_t* __restrict__ p,
+                                                      uint64_t len, uint8_t* __restrict__ lf,
+                                                      uint8_t* __restrict__ bf)
+{
+    __shared__ uint8_t sg[256], sp[256];
+    const int t = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * FA_ADD_BLK + 4 * (uint64_t)t;
+    uint8_t G = 0, P = 1;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const uint64_t i = base + e;
+        if (i >= n) break;
+        const uint64_t x = z[i];
+        const uint64_t s = x + (i < len ? p[i] : 0);
+        const uint8_t g = s < x, q = s == ~UINT64_C(0);
+        z[i] = s;
+        lf[i] = (uint8_t)(g | (q << 1));
+        G = (uint8_t)(g | (q & G));
+        P = (uint8_t)(P & q);
+    }
+    sg[t] = G;
+    sp[t] = P;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {   // fold the 256 thread aggregates in order
+        uint8_t g = 0, q = 1;
+        if (t < h) {
+            const uint8_t ge = sg[2 * t], pe = sp[2 * t], gl = sg[2 * t + 1], pl = sp[2 * t + 1];
+            g = (uint8_t)(gl | (pl & ge));
+            q = (uint8_t)(pl & pe);
+        }
+        __syncthreads();
+        if (t < h) {
+            sg[t] = g;
+            sp[t] = q;
+        }
+        __syncthreads();
+    }
+    if (t == 0) bf[blockIdx.x] = (uint8_t)(sg[0] | (sp[0] << 1));
+}
+
+// one workgroup: carry into each block
+__global__ void __launch_bounds__(1024) fa_k_add_scan(uint8_t* __restrict__ bf, uint64_t nb)
+{
+    __shared__ uint8_t sg[1024], sp[1024];
+    const int t = threadIdx.x;
+    const uint64_t m = (nb + 1023) / 1024;
+    const uint64_t b0 = (uint64_t)t * m < nb ? (uint64_t)t * m : nb;
+    const uint64_t b1 = b0 + m < nb ? b0 + m : nb;
+    uint8_t G = 0, P = 1;
+    for (uint64_t b = b0; b < b1; b++) {
+        const uint8_t f = bf[b];
+        G = (uint8_t)((f & 1) | ((f >> 1) & G));
+        P = (uint8_t)(P & (f >> 1));
+    }
+    sg[t] = G;
+    sp[t] = P;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        uint8_t g = sg[t], q = sp[t];
+        if (t >= d) {
+            g = (uint8_t)(g | (q & sg[t - d]));
+            q = (uint8_t)(q & sp[t - d]);
+        }
+        __syncthreads();
+        sg[t] = g;
+        sp[t] = q;
+        __syncthreads();
+    }
+    uint8_t c = t ? sg[t - 1] : 0;
+    for (uint64_t b = b0; b < b1; b++) {
+        const uint8_t f = bf[b];
+        bf[b] = c;   // carry into block b
+        c = (uint8_t)((f & 1) | ((f >> 1) & c));
+    }
+}
+
+__global__ void __launch_bounds__(256) fa_k_add_apply(uint64_t* __restrict__ z, uint64_t n,
+                                                      const uint8_t* __restrict__ lf,
+                                                      const uint8_t* __restrict__ bf)
+{
+    __shared__ uint8_t sg[256], sp[256];
+    const int t = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x * FA_ADD_BLK + 4 * (uint64_t)t;
+    uint8_t G = 0, P = 1;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const uint64_t i = base + e;
+        if (i >= n) break;
+        const uint8_t f = lf[i];
+        G = (uint8_t)((f & 1) | ((f >> 1) & G));
+        P = (uint8_t)(P & (f >> 1));
+    }
+    sg[t] = G;
+    sp[t] = P;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        uint8_t g = sg[t], q = sp[t];
+        if (t >= d) {
+            g = (uint8_t)(g | (q & sg[t - d]));
+            q = (uint8_t)(q & sp[t - d]);
+        }
+        __syncthreads();
+        sg[t] = g;
+        sp[t] = q;
+        __syncthreads();
+    }
+    const uint8_t cb = bf[blockIdx.x];
+    // carry into this thread: the threads before it, after the block's carry
+    uint8_t c = t ? (uint8_t)(sg[t - 1] | (sp[t - 1] & cb)) : cb;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const uint64_t i = base + e;
+        if (i >= n) break;
+        const uint8_t f = lf[i];
+        if (c) z[i] += 1;
+        c = (uint8_t)((f & 1) | ((f >> 1) & c));
+    }
+}
+
+// z[off, zn) += p[0, len), len <= zn - off; flags: zn - off + (zn - off) /
+// 1024 + 1 bytes of device scratch
+static int mpn_add_device(uint64_t* z, uint64_t zn, uint64_t off, const uint64_t* p, uint64_t len,
+                          uint8_t* flags, hipStream_t s)
+{
+    const uint64_t n = zn - off;
+    if (!n) return FA_OK;
+    const uint64_t nb = (n + FA_ADD_BLK - 1) / FA_ADD_BLK;
+    uint8_t* lf = flags;
+    uint8_t* bf = flags + n;
+    FA_LAUNCH("fa_k_add_local", fa_k_add_local, dim3((unsigned)nb), dim3(256), 0, s, z + off, n, p,
+              len, lf, bf);
+    FA_LAUNCH("fa_k_add_scan", fa_k_add_scan, dim3(1), dim3(1024), 0, s, bf, nb);
+    FA_LAUNCH("fa_k_add_apply", fa_k_add_apply, dim3((unsigned)nb), dim3(256), 0, s, z + off, n, lf,
+              bf);
+    return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "add launch");
+}
+
+__global__ void __launch_bounds__(256) fa_k_zero(uint64_t* __restrict__ z, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) z[i] = 0;
+}
+
+// longest x <= n with x + other <= cap and a plan for x x other limbs
+static uint64_t fit_block(uint64_t n, uint64_t other, uint64_t cap)
+{
+    if (other >= cap) return 0;
+    uint64_t lo = 0, hi = n < cap - other ? n : cap - other;
+    fa_mpn_plan_t P;
+    while (lo < hi) {
+        const uint64_t mid = hi - (hi - lo) / 2;
+        if (fa_mpn_choose_plan(mid, other, &P) == FA_OK) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// total limbs above which integer products go in blocks (tests force small
+// blocks; 0 = only products that no single plan covers)
+uint64_t fa_mpn_block_cap = UINT64_MAX;
+extern "C" FA_API void fa_set_mpn_block_cap(uint64_t limbs)
+{
+    fa_mpn_block_cap = limbs ? limbs : UINT64_MAX;
+}
+
+static int mpn_mul_plan_device(uint64_t* z, const uint64_t* a, uint64_t an, const uint64_t* b,
+                               uint64_t bn, hipStream_t stream, int np_force, int bits_force);
+
+// a x b above one plan (or above the block cap): the longer operand in
+// blocks beside the whole shorter one when that leaves blocks at least as
+// long as the shorter operand, else both in equal blocks; every block
+// product goes through the ordinary path and is added into z
+static int mpn_mul_blocked(uint64_t* z, const uint64_t* a, uint64_t an, const uint64_t* b,
+                           uint64_t bn, hipStream_t stream)
+{
+    if (an < bn) {
+        const uint64_t* t = a; a = b; b = t;
+        const uint64_t tn = an; an = bn; bn = tn;
+    }
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    if (!stream) stream = st->stream;   // the block products run there too
+    const uint64_t cap = fa_mpn_block_cap;
+    // (1) the shorter operand whole
+    uint64_t lb = bn;
+    uint64_t la = fit_block(an, lb, cap);
+    uint64_t cost = la ? (an + la - 1) / la : UINT64_MAX;
+    // (2) equal blocks: the longest s with a plan for s x s within the cap
+    {
+        uint64_t lo = 0, hi = bn;
+        fa_mpn_plan_t Q;
+        while (lo < hi) {
+            const uint64_t mid = hi - (hi - lo) / 2;
+            if (2 * mid <= cap && fa_mpn_choose_plan(mid, mid, &Q) == FA_OK) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint64_t la2 = lo ? fit_block(an, lo, cap) : 0;
+        const uint64_t cost2 =
+            la2 ? ((an + la2 - 1) / la2) * ((bn + lo - 1) / lo) : UINT64_MAX;
+        if (cost2 < cost) {
+            lb = lo;
+            la = la2;
+            cost = cost2;
+        }
+    }
+    if (la == 0 || lb == 0)
+        return fa_set_error(FA_E_RANGE, "no block plan for %llu x %llu limbs",
+                            (unsigned long long)an, (unsigned long long)bn);
+    const uint64_t zn = an + bn;
+    const uint64_t plen = la + lb;
+    const size_t flag_bytes = (size_t)zn + zn / FA_ADD_BLK + 64;
+    void* tb;
+    int rc = fa_tbuf_reserve(st, plen * 8 + flag_bytes, &tb);
+    if (rc) return rc;
+    uint64_t* tmp = (uint64_t*)tb;
+    uint8_t* flags = (uint8_t*)(tmp + plen);
+    FA_LAUNCH("fa_k_zero", fa_k_zero, dim3((unsigned)((zn + 255) / 256)), dim3(256), 0, stream, z,
+              zn);
+    for (uint64_t i = 0; i < an; i += la) {
+        const uint64_t na = an - i < la ? an - i : la;
+        for (uint64_t j = 0; j < bn; j += lb) {
+            const uint64_t nbk = bn - j < lb ? bn - j : lb;
+            rc = mpn_mul_plan_device(tmp, a + i, na, b + j, nbk, stream, 0, 0);
+            if (rc) return rc;
+            rc = mpn_add_device(z, zn, i + j, tmp, na + nbk, flags, stream);
+            if (rc) return rc;
+        }
+    }
+    return FA_OK;
+}
+
 static int mpn_mul_device(uint64_t* z, const uint64_t* a, uint64_t an, const uint64_t* b,
                           uint64_t bn, hipStream_t stream, int np_force = 0, int bits_force = 0)
+{
+    if (!np_force) {
+        fa_mpn_plan_t Q;
+        if (an + bn > fa_mpn_block_cap || fa_mpn_choose_plan(an, bn, &Q) == FA_E_RANGE)
+            return mpn_mul_blocked(z, a, an, b, bn, stream);
+    }
+    return mpn_mul_plan_device(z, a, an, b, bn, stream, np_force, bits_force);
+}
+
+// one product on one plan
+static int mpn_mul_plan_device(uint64_t* z, const uint64_t* a, uint64_t an, const uint64_t* b,
+                               uint64_t bn, hipStream_t stream, int np_force, int bits_force)
 {
     fa_mpn_plan_t P;
     int rc = np_force ? fa_mpn_fixed_plan(an, bn, np_force, bits_force, &P)

@@ -158,6 +158,11 @@ FA_API void fa_set_col_variant(int v);
    held in registers, two workgroups per CU.  Tuning switch, no counterpart
    in the reference. */
 FA_API void fa_set_row_variant(int v);
+/* integer products whose operands together exceed `limbs` (0: only those no
+   single 2^24-point plan covers, the default) are computed as block products
+   added into the output on the device.  Lets tests exercise the block path
+   at small sizes; no counterpart in the reference. */
+FA_API void fa_set_mpn_block_cap(uint64_t limbs);
 /* 2^24-point products: the primes split into v groups (2 or 3; default 2),
    each on a stream of its own forked from and joined back to the caller's,
    so one group's row and inverse passes run beside the next group's forward

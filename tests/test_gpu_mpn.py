@@ -245,3 +245,59 @@ def test_crt_variants_agree(an, bn, np_, bits):
     assert np.array_equal(z0, z1)
     q = (1 << 61) - 1
     assert (to_int(a) % q) * (to_int(b) % q) % q == to_int(z1) % q
+
+
+@pytest.mark.parametrize("an,bn,cap", [(2000, 1500, 3000), (5000, 300, 3000), (4096, 4096, 3000),
+                                       (7000, 2000, 2500)])
+def test_blocked_product_exact(an, bn, cap):
+    """integer products in block products added on the device (forced by a
+    small block cap) against exact integers; all-ones operands make the
+    carries ripple across blocks"""
+    import flint_amd as fa
+    rng = np.random.default_rng(an * 3 + bn)
+    cases = [(rand_limbs(rng, an), rand_limbs(rng, bn)),
+             (np.full(an, ~np.uint64(0), dtype=np.uint64), np.full(bn, ~np.uint64(0), dtype=np.uint64))]
+    print("blocked", an, bn, cap, flush=True)
+    try:
+        fa.set_mpn_block_cap(cap)
+        for a, b in cases:
+            z = fa.mpn_mul(a, b)
+            want = to_int(a) * to_int(b)
+            assert to_int(z) == want
+    finally:
+        fa.set_mpn_block_cap(0)
+
+
+def test_blocked_matches_single_plan_on_device_stream():
+    import torch
+    import flint_amd as fa
+    an, bn = 200000, 150000
+    a, b = fa.synth_mpn_pair(0xB10C, an, bn)
+    z0 = fa.mpn_mul(a, b)
+    dev = torch.device("cuda:0")
+    ta = torch.from_numpy(a.view(np.int64)).to(dev)
+    tb = torch.from_numpy(b.view(np.int64)).to(dev)
+    tz = torch.zeros(an + bn, dtype=torch.int64, device=dev)
+    stream = torch.cuda.Stream()
+    try:
+        fa.set_mpn_block_cap(120000)
+        fa.mpn_mul_device(tz.data_ptr(), ta.data_ptr(), an, tb.data_ptr(), bn, stream.cuda_stream)
+        stream.synchronize()
+    finally:
+        fa.set_mpn_block_cap(0)
+    assert np.array_equal(tz.cpu().numpy().view(np.uint64), z0)
+
+
+def test_product_beyond_one_plan():
+    """2^25 x 2^25 limbs: no single 2^24-point plan covers it; the block
+    path computes it (checked mod three 61-bit primes)"""
+    import flint_amd as fa
+    n = 1 << 25
+    with pytest.raises(fa.FlintAmdError):
+        fa.mpn_plan(n, n)
+    a, b = fa.synth_mpn_pair(0xB1C, n, n)
+    print("beyond-plan", n, flush=True)
+    z = fa.mpn_mul(a, b)
+    A, B, Z = to_int(a), to_int(b), to_int(z)
+    for q in ((1 << 61) - 1, (1 << 61) - 31, (1 << 61) - 45):
+        assert (A % q) * (B % q) % q == Z % q
