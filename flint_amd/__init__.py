@@ -87,6 +87,7 @@ SIGNATURES = {
     "fa_set_col_variant": (None, [ctypes.c_int]),
     "fa_set_row_variant": (None, [ctypes.c_int]),
     "fa_set_mpn_block_cap": (None, [ctypes.c_uint64]),
+    "fa_set_poly_block_cap": (None, [ctypes.c_uint64]),
     "fa_set_ks_variant": (None, [ctypes.c_int]),
     "fa_set_prime_groups": (None, [ctypes.c_int]),
     "fa_set_crt_variant": (None, [ctypes.c_int]),
@@ -214,6 +215,12 @@ def set_mpn_block_cap(limbs):
     """integer products above `limbs` total limbs go in device-added block
     products (0: only those no single plan covers)"""
     lib().fa_set_mpn_block_cap(ctypes.c_uint64(int(limbs)))
+
+
+def set_poly_block_cap(n):
+    """nmod/fmpz products above n output coefficients go in device-added
+    block products (0: only above one 2^24-point transform)"""
+    lib().fa_set_poly_block_cap(ctypes.c_uint64(int(n)))
 
 
 def set_col_variant(v):

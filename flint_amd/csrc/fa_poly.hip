@@ -300,6 +300,146 @@ static int poly_buffers(fa_thread_state* st, uint64_t an_words, uint64_t bn_word
                                                    hipGetErrorString(hipGetLastError())); \
     } while (0)
 
+// ------------------------------------------------------------------------
+// polynomial products of more output coefficients than one 2^24-point
+// transform holds: block products, each one's share of the window [zl, zh)
+// added into the output on the device (mod n, or as zw-limb two's
+// complement integers, coefficient by coefficient)
+// ------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) fa_k_poly_zero(uint64_t* __restrict__ z, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) z[i] = 0;
+}
+
+__global__ void __launch_bounds__(256) fa_k_addmod(uint64_t* __restrict__ z,
+                                                   const uint64_t* __restrict__ t, uint64_t cnt,
+                                                   uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= cnt) return;
+    const uint64_t x = z[i];
+    uint64_t s = x + t[i];
+    if (s < x || s >= n) s -= n;   // both terms below n
+    z[i] = s;
+}
+
+__global__ void __launch_bounds__(256) fa_k_add_coeffs(uint64_t* __restrict__ z,
+                                                       const uint64_t* __restrict__ t,
+                                                       uint64_t cnt, uint64_t zw)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= cnt) return;
+    uint64_t* o = z + i * zw;
+    const uint64_t* p = t + i * zw;
+    uint64_t c = 0;
+    for (uint64_t l = 0; l < zw; l++) {
+        const uint64_t x = o[l], y = p[l];
+        const uint64_t s1 = x + y;
+        const uint64_t s2 = s1 + c;
+        c = (uint64_t)(s1 < x) + (uint64_t)(s2 < s1);
+        o[l] = s2;
+    }
+}
+
+// output coefficients above which nmod/fmpz products go in blocks (tests
+// force small blocks; 0 = only above one 2^24-point transform)
+uint64_t fa_poly_block_cap = 0;
+extern "C" FA_API void fa_set_poly_block_cap(uint64_t n) { fa_poly_block_cap = n; }
+
+static uint64_t poly_block_limit(uint64_t zn)
+{
+    const uint64_t full = UINT64_C(1) << FA_MAX_DEPTH;
+    if (fa_poly_block_cap && zn > fa_poly_block_cap) return fa_poly_block_cap < full ? fa_poly_block_cap : full;
+    return zn > full ? full : 0;   // 0: one product
+}
+
+// fmpz != 0: zw-limb signed coefficients (aw/bw limbs in), centred CRT;
+// else nmod.  zl < zh <= an + bn - 1; np/offset and C for the whole product
+static int poly_mul_blocked(int fmpz, uint64_t* z, uint64_t zw, int64_t zl, int64_t zh,
+                            const uint64_t* a, uint64_t aw, int64_t an, const uint64_t* b,
+                            uint64_t bw, int64_t bn, uint64_t n, int np, int offset,
+                            const fa_crt_consts& C, uint64_t limit)
+{
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    hipStream_t s = st->stream;
+    // blocks: b whole beside blocks of a, or equal blocks, whichever needs
+    // fewer block products; every block product has at most `limit` outputs
+    const uint64_t A_ = (uint64_t)an, B_ = (uint64_t)bn;
+    uint64_t la = B_ < limit ? limit - B_ + 1 : 0, lb = B_;
+    if (la > A_) la = A_;
+    uint64_t cost = la ? (A_ + la - 1) / la : UINT64_MAX;
+    {
+        const uint64_t h = (limit + 1) / 2;
+        const uint64_t la2 = A_ < h ? A_ : h, lb2 = B_ < h ? B_ : h;
+        const uint64_t c2 = ((A_ + la2 - 1) / la2) * ((B_ + lb2 - 1) / lb2);
+        if (c2 < cost) { la = la2; lb = lb2; cost = c2; }
+    }
+    int dmax = clog2_u64(la + lb - 1);
+    if (dmax < 4) dmax = 4;
+    const uint64_t cnt = (uint64_t)(zh - zl);
+    const uint64_t tcap = cnt < la + lb - 1 ? cnt : la + lb - 1;
+    const uint64_t zwords = fmpz ? zw : 1;
+    const size_t tr = (size_t)np * fa_res_stride(dmax) * sizeof(double);
+    const size_t bytes = 2 * tr + (A_ * aw + B_ * bw + (cnt + tcap) * zwords) * 8 + 256;
+    void* buf;
+    int rc = fa_scratch_reserve(st, bytes, &buf);
+    if (rc) return rc;
+    double* Ar = (double*)buf;
+    double* Br = (double*)((char*)buf + tr);
+    uint64_t* da = (uint64_t*)((char*)buf + 2 * tr);
+    uint64_t* db = da + A_ * aw;
+    uint64_t* dz = db + B_ * bw;
+    uint64_t* dt = dz + cnt * zwords;
+    HIPTRY(hipMemcpyAsync(da, a, A_ * aw * 8, hipMemcpyHostToDevice, s), "upload a");
+    HIPTRY(hipMemcpyAsync(db, b, B_ * bw * 8, hipMemcpyHostToDevice, s), "upload b");
+    FA_LAUNCH("fa_k_poly_zero", fa_k_poly_zero, dim3((unsigned)((cnt * zwords + 255) / 256)),
+              dim3(256), 0, s, dz, cnt * zwords);
+    int norm = 0;
+    uint64_t dinv = 0;
+    if (!fmpz) {
+        norm = 64 - nbits_u64(n);
+        const uint64_t d = n << norm;
+        dinv = (uint64_t)(((u128)0 - 1) / d - ((u128)1 << 64));
+    }
+    for (uint64_t i = 0; i < A_; i += la) {
+        const uint64_t na = A_ - i < la ? A_ - i : la;
+        for (uint64_t j = 0; j < B_; j += lb) {
+            const uint64_t nb = B_ - j < lb ? B_ - j : lb;
+            const uint64_t off = i + j, blen = na + nb - 1;
+            const uint64_t wlo = (uint64_t)zl > off ? (uint64_t)zl : off;
+            const uint64_t whi = (uint64_t)zh < off + blen ? (uint64_t)zh : off + blen;
+            if (wlo >= whi) continue;
+            int depth = clog2_u64(blen);
+            if (depth < 4) depth = 4;
+            fa_tables* T = fa_get_tables(depth);
+            if (!T) return FA_E_NODEV;
+            fa_pack_desc pa = {fmpz ? FA_PACK_S64W : FA_PACK_U64, da + i * aw, na, 0, (int)aw};
+            fa_pack_desc pb = {fmpz ? FA_PACK_S64W : FA_PACK_U64, db + j * bw, nb, 0, (int)bw};
+            rc = fa_convolve(st, T, np, offset, depth, &pa, &pb, Ar, Br, s);
+            if (rc) return rc;
+            const uint64_t wc = whi - wlo;
+            const unsigned g = (unsigned)((wc + 255) / 256);
+            if (fmpz) {
+                FA_LAUNCH("fa_k_crt_fmpz", fa_k_crt_fmpz, dim3(g), dim3(256), 0, s, Ar,
+                          fa_res_stride(depth), C, wlo - off, wc, zw, dt);
+                FA_LAUNCH("fa_k_add_coeffs", fa_k_add_coeffs, dim3(g), dim3(256), 0, s,
+                          dz + (wlo - (uint64_t)zl) * zw, dt, wc, zw);
+            } else {
+                FA_LAUNCH("fa_k_crt_nmod", fa_k_crt_nmod, dim3(g), dim3(256), 0, s, Ar,
+                          fa_res_stride(depth), C, wlo - off, wc, n, dinv, norm, dt);
+                FA_LAUNCH("fa_k_addmod", fa_k_addmod, dim3(g), dim3(256), 0, s,
+                          dz + (wlo - (uint64_t)zl), dt, wc, n);
+            }
+            HIPTRY(hipGetLastError(), "block launch");
+        }
+    }
+    HIPTRY(hipMemcpyAsync(z, dz, cnt * zwords * 8, hipMemcpyDeviceToHost, s), "download");
+    HIPTRY(hipStreamSynchronize(s), "synchronize");
+    return FA_OK;
+}
+
 extern "C" int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh, const uint64_t* a,
                                     int64_t an, const uint64_t* b, int64_t bn, fa_nmod_t mod)
 {
@@ -319,7 +459,10 @@ extern "C" int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh, const u
         memset(z, 0, (size_t)(zh - zl) * sizeof(uint64_t));
         return FA_OK;
     }
-    fa_tables* T = fa_get_tables(clog2_u64((uint64_t)zn));
+    // prime data from the tables of at most one transform's depth (a longer
+    // product goes in blocks)
+    const int tdepth = clog2_u64((uint64_t)zn) < FA_MAX_DEPTH ? clog2_u64((uint64_t)zn) : FA_MAX_DEPTH;
+    fa_tables* T = fa_get_tables(tdepth);
     if (!T) return FA_E_NODEV;
     fa_thread_state* st = fa_state();
     if (!st) return FA_E_NODEV;
@@ -331,9 +474,8 @@ extern "C" int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh, const u
     fa_crt_consts_init(&J.C, J.np, J.offset);
     J.depth = clog2_u64((uint64_t)zn);
     if (J.depth < 4) J.depth = 4;
-    if (J.depth > FA_MAX_DEPTH)
-        return fa_set_error(FA_E_RANGE, "nmod product length %lld above 2^%d", (long long)zn,
-                            FA_MAX_DEPTH);
+    if (const uint64_t lim = poly_block_limit((uint64_t)zn))
+        return poly_mul_blocked(0, z, 1, zl, zh, a, 1, an, b, 1, bn, n, J.np, J.offset, J.C, lim);
 
     uint64_t cnt = (uint64_t)(zh - zl);
     uint64_t *da, *db, *dz;
@@ -455,15 +597,16 @@ extern "C" int fa_fmpz_poly_mul_mid(uint64_t* z, uint64_t zw, int64_t zl, int64_
     if (!np) return 0;
     int depth = clog2_u64((uint64_t)zn);
     if (depth < 4) depth = 4;
-    if (depth > FA_MAX_DEPTH)
-        return fa_set_error(FA_E_RANGE, "fmpz product length %lld above 2^%d", (long long)zn,
-                            FA_MAX_DEPTH);
+    fa_crt_consts C;
+    fa_crt_consts_init(&C, np, 0);
+    if (const uint64_t lim = poly_block_limit((uint64_t)zn)) {
+        const int rcb = poly_mul_blocked(1, z, zw, zl, zh, a, aw, an, b, bw, bn, 0, np, 0, C, lim);
+        return rcb ? rcb : 1;
+    }
     fa_tables* T = fa_get_tables(depth);
     if (!T) return FA_E_NODEV;
     fa_thread_state* st = fa_state();
     if (!st) return FA_E_NODEV;
-    fa_crt_consts C;
-    fa_crt_consts_init(&C, np, 0);
     uint64_t cnt = (uint64_t)(zh - zl);
     uint64_t *da, *db, *dz;
     double *A, *B;

@@ -163,6 +163,12 @@ FA_API void fa_set_row_variant(int v);
    added into the output on the device.  Lets tests exercise the block path
    at small sizes; no counterpart in the reference. */
 FA_API void fa_set_mpn_block_cap(uint64_t limbs);
+/* nmod and fmpz products of more than n output coefficients (0: more than
+   one 2^24-point transform holds, the default) are computed as block
+   products whose shares of the output window are added on the device.  Lets
+   tests exercise the block path at small sizes; no counterpart in the
+   reference. */
+FA_API void fa_set_poly_block_cap(uint64_t n);
 /* 2^24-point products: the primes split into v groups (2 or 3; default 2),
    each on a stream of its own forked from and joined back to the caller's,
    so one group's row and inverse passes run beside the next group's forward

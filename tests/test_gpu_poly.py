@@ -117,3 +117,75 @@ def test_fmpz_random_vs_oracle(bits, signed):
     ok, ref = O.fmpz_poly_mul_mid(a, w, b, w, 5, zn + 3, zw)
     assert ok == 1 and out is not None
     assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("an,bn,cap,zl,zh", [(3000, 2500, 2000, 0, 5499), (5000, 40, 1000, 7, 3000),
+                                             (1200, 1300, 700, 600, 2400)])
+def test_nmod_blocked_vs_oracle(an, bn, cap, zl, zh):
+    """nmod products in block products (forced by a small cap) whose window
+    shares are added mod n on the device, against the oracle"""
+    import flint_amd as fa
+    rng = np.random.default_rng(an + cap)
+    n = (1 << 61) - 1
+    a = rng.integers(0, n, an, dtype=np.uint64)
+    b = rng.integers(0, n, bn, dtype=np.uint64)
+    a[::5] = n - 1
+    ref = O.nmod_poly_mul_mid(a, b, zl, zh, n)
+    try:
+        fa.set_poly_block_cap(cap)
+        out = fa.nmod_poly_mul_mid(a, b, zl, zh, n)
+    finally:
+        fa.set_poly_block_cap(0)
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("bits,cap", [(62, 2000), (180, 1500)])
+def test_fmpz_blocked_vs_oracle(bits, cap):
+    """fmpz products in block products whose shares are added as two's
+    complement coefficients on the device, against the oracle"""
+    import flint_amd as fa
+    rng = np.random.default_rng(bits + cap)
+    an, bn = 3000, 2200
+    w = bits // 64 + 1
+    zw = (2 * bits + 16) // 64 + 1
+
+    def draw(n):
+        v = []
+        for _ in range(n):
+            x = int.from_bytes(rng.bytes((bits + 7) // 8), "little") % (1 << bits)
+            v.append(-x if rng.integers(0, 2) else x)
+        return np.frombuffer(b"".join((x % (1 << (64 * w))).to_bytes(8 * w, "little")
+                                      for x in v), dtype=np.uint64)
+    a, b = draw(an), draw(bn)
+    zn = an + bn - 1
+    ok, ref = O.fmpz_poly_mul_mid(a, w, b, w, 3, zn + 2, zw)
+    try:
+        fa.set_poly_block_cap(cap)
+        out = fa.fmpz_poly_mul_mid(a, w, b, w, 3, zn + 2, zw)
+    finally:
+        fa.set_poly_block_cap(0)
+    assert ok == 1 and out is not None
+    assert np.array_equal(out, ref)
+
+
+def test_nmod_beyond_one_transform():
+    """a product of more than 2^24 output coefficients (no single transform
+    holds it) against the sum of two products that fit, formed on the host:
+    a b = a_lo b + x^h a_hi b"""
+    import flint_amd as fa
+    rng = np.random.default_rng(77)
+    n = (1 << 60) + 33   # any modulus works; the sum below stays below 2^64
+    an, bn = (1 << 23) + 5000, (1 << 23) + 3000
+    a = rng.integers(0, n, an, dtype=np.uint64)
+    b = rng.integers(0, n, bn, dtype=np.uint64)
+    zn = an + bn - 1
+    print("beyond-transform nmod", an, bn, flush=True)
+    z = fa.nmod_poly_mul_mid(a, b, 0, zn, n)
+    h = an // 2
+    lo = fa.nmod_poly_mul_mid(a[:h], b, 0, h + bn - 1, n)
+    hi = fa.nmod_poly_mul_mid(a[h:], b, 0, an - h + bn - 1, n)
+    want = np.zeros(zn, dtype=np.uint64)
+    want[:h + bn - 1] = lo
+    s = want[h:] + hi
+    want[h:] = np.where(s >= n, s - np.uint64(n), s)
+    assert np.array_equal(z, want)
