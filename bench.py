@@ -288,8 +288,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(multi-GPU sharding is design-only, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: runs on one GPU; --gpus %d is not supported (sharding the primes "
+              "or the requests across GPUs is designed but not built)" % args.gpus,
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
