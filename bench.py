@@ -11,7 +11,9 @@ second.  After the timed region the product is checked against
 The roofline entry is for the dominant kernel: algorithmic bytes per launch
 (8 bytes per transform point per array pass, at the reference plan's
 geometry np*T) divided by that kernel's average launch time, measured with
-HIP events recorded around each launch on the launching stream.
+HIP events recorded around each launch on the launching stream.  The timed
+products run the primes in two groups on forked streams (--prime-groups);
+the per-kernel table is taken with one launch per pass over all primes.
 
 The CPU baseline is the oracle (oracle/fsmall_oracle.c, a scalar C
 restatement of the reference) timed on a bounded sample, scaled to the full
@@ -129,7 +131,7 @@ def _plan_ws(q, fa):
     if q["kind"] == "fmpz":
         sb = abs(q["abits"]) + abs(q["bbits"]) + int(min(an, bn)).bit_length() + 1
         la, lb = -(-an * sb // 64), -(-bn * sb // 64)
-        ws += 3 * _mpn_ws(la, lb, fa) + 8 * 6 * (la + lb)
+        ws += _mpn_ws(la, lb, fa) + 8 * 2 * (la + lb)   # one product of the packed operands
     return ws
 
 
@@ -281,6 +283,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log2", type=int, default=24)
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--prime-groups", type=int, default=2,
+                    help="depth-24 products: primes in this many groups on forked streams "
+                         "(0 = one launch per pass, as profiler kernel traces should see them)")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_mpn_2p24.json"))
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the library's plain copy kernel over MIB MiB, 5 times (the "
@@ -317,6 +322,7 @@ def main():
                           "ms": ms, "GB/s": 2 * nbytes / (ms * 1e-3) / 1e9}))
         return
 
+    fa.set_prime_groups(args.prime_groups)
     if args.workload in ("nmod4096", "fmpz256", "stream"):
         return batched_main(args, torch, fa)
 
