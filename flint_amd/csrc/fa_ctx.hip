@@ -122,7 +122,7 @@ static uint64_t primitive_2power_root(uint64_t p)
     return fa_powmod_u64(a, (p - 1) >> v, p);
 }
 
-static void default_primes(uint64_t* out)
+static void find_default_primes(uint64_t* out)
 {
     uint64_t p = DEFAULT_PRIME;
     for (int i = 0; i < FA_NPRIMES; i++) {
@@ -130,6 +130,16 @@ static void default_primes(uint64_t* out)
         while (!is_prime_u64(p)) p = next_fft_number(p);
         out[i] = p;
     }
+}
+
+// the primes are found once per process (the search runs Miller-Rabin on
+// each candidate; plans and CRT constants ask for them on every product)
+static void default_primes(uint64_t* out)
+{
+    static uint64_t P[FA_NPRIMES];
+    static std::once_flag once;
+    std::call_once(once, [] { find_default_primes(P); });
+    memcpy(out, P, sizeof(P));
 }
 
 extern "C" int fa_default_primes(uint64_t* out8)
@@ -443,7 +453,27 @@ static uint64_t ml_mod_1(const uint64_t* a, int n, uint64_t d)
     return (uint64_t)rem;
 }
 
+static void crt_consts_build(fa_crt_consts* C, int np, int offset);
+
+// constants per (np, offset), built once per process
 void fa_crt_consts_init(fa_crt_consts* C, int np, int offset)
+{
+    static fa_crt_consts cache[FA_NPRIMES + 1][FA_NPRIMES + 1];
+    static bool have[FA_NPRIMES + 1][FA_NPRIMES + 1];
+    static std::mutex mu;
+    if (np < 1 || np > FA_NPRIMES || offset < 0 || offset + np > FA_NPRIMES) {
+        crt_consts_build(C, np, offset);
+        return;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!have[np][offset]) {
+        crt_consts_build(&cache[np][offset], np, offset);
+        have[np][offset] = true;
+    }
+    *C = cache[np][offset];
+}
+
+static void crt_consts_build(fa_crt_consts* C, int np, int offset)
 {
     uint64_t P[FA_NPRIMES];
     default_primes(P);

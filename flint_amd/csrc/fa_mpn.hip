@@ -1281,7 +1281,8 @@ extern "C" FA_API void fa_set_mpn_block_cap(uint64_t limbs)
 }
 
 static int mpn_mul_plan_device(uint64_t* z, const uint64_t* a, uint64_t an, const uint64_t* b,
-                               uint64_t bn, hipStream_t stream, int np_force, int bits_force);
+                               uint64_t bn, hipStream_t stream, int np_force, int bits_force,
+                               const fa_mpn_plan_t* chosen = nullptr);
 
 // a x b above one plan (or above the block cap): the longer operand in
 // blocks beside the whole shorter one when that leaves blocks at least as
@@ -1351,19 +1352,25 @@ static int mpn_mul_device(uint64_t* z, const uint64_t* a, uint64_t an, const uin
 {
     if (!np_force) {
         fa_mpn_plan_t Q;
-        if (an + bn > fa_mpn_block_cap || fa_mpn_choose_plan(an, bn, &Q) == FA_E_RANGE)
+        const int rq = fa_mpn_choose_plan(an, bn, &Q);
+        if (an + bn > fa_mpn_block_cap || rq == FA_E_RANGE)
             return mpn_mul_blocked(z, a, an, b, bn, stream);
+        if (rq) return rq;
+        return mpn_mul_plan_device(z, a, an, b, bn, stream, 0, 0, &Q);
     }
     return mpn_mul_plan_device(z, a, an, b, bn, stream, np_force, bits_force);
 }
 
-// one product on one plan
+// one product on one plan (`chosen`: the automatic plan, already made)
 static int mpn_mul_plan_device(uint64_t* z, const uint64_t* a, uint64_t an, const uint64_t* b,
-                               uint64_t bn, hipStream_t stream, int np_force, int bits_force)
+                               uint64_t bn, hipStream_t stream, int np_force, int bits_force,
+                               const fa_mpn_plan_t* chosen)
 {
     fa_mpn_plan_t P;
-    int rc = np_force ? fa_mpn_fixed_plan(an, bn, np_force, bits_force, &P)
-                      : fa_mpn_choose_plan(an, bn, &P);
+    int rc = FA_OK;
+    if (chosen) P = *chosen;
+    else rc = np_force ? fa_mpn_fixed_plan(an, bn, np_force, bits_force, &P)
+                       : fa_mpn_choose_plan(an, bn, &P);
     if (rc) return rc;
     fa_tables* T = fa_get_tables(P.depth);
     if (!T) return FA_E_NODEV;
