@@ -241,7 +241,11 @@ FA_DEV uint64_t ks_spack_limb(const uint64_t* __restrict__ a, int w, uint64_t n,
                               int negs, uint64_t s, uint64_t j)
 {
     const uint64_t bit0 = 64 * j;
-    const uint64_t i0 = bit0 / s;
+    // floor(bit0 / s) from the correctly rounded double quotient (both
+    // operands below 2^53, so exact as doubles), which is at most one above
+    // it: a 64-bit integer division is a long emulated sequence here
+    uint64_t i0 = (uint64_t)((double)bit0 / (double)s);
+    if (i0 * s > bit0) i0--;
     uint64_t acc = 0;
     for (uint64_t i = i0; i < n && s * i < bit0 + 64; i++) {
         const uint64_t* c = a + i * (uint64_t)w;
@@ -343,43 +347,87 @@ __global__ void __launch_bounds__(1024) fa_k_ks_uscan(const uint64_t* __restrict
     ks_uscan_body(C, nP, s, K, tk, sg, sp);
 }
 
+// limbs of each coefficient staged through LDS per step of the unpack
+#define FA_KS_UC 16
+
 // z[t] = coefficient zl + t: the s-bit field e_k + t_(k-1) read as signed,
-// times sigma_a sigma_b, as zw limbs; zero past the product's length zn
-FA_DEV void ks_unpack_one(const uint64_t* __restrict__ C, uint64_t nP, uint64_t s,
-                          const uint8_t* __restrict__ tk, const int32_t* __restrict__ sig,
-                          uint64_t zl, uint64_t zn, int zw, uint64_t* __restrict__ z, uint64_t t)
+// times sigma_a sigma_b, as zw limbs; zero past the product's length zn.
+// One workgroup of 256 threads takes t = t0 .. t0 + 255 (those below cnt),
+// one coefficient per thread, FA_KS_UC output limbs at a time: the slot's
+// limbs are read into LDS and the output limbs written from it as runs of
+// consecutive limbs, so neither side accesses memory one limb per
+// coefficient at a stride of zw limbs.  buf: 256 * (FA_KS_UC + 1) words.
+FA_DEV void ks_unpack_tile(const uint64_t* __restrict__ C, uint64_t nP, uint64_t s,
+                           const uint8_t* __restrict__ tk, const int32_t* __restrict__ sig,
+                           uint64_t zl, uint64_t zn, uint64_t cnt, int zw,
+                           uint64_t* __restrict__ z, uint64_t t0, uint64_t* __restrict__ buf)
 {
-    uint64_t* o = z + t * (uint64_t)zw;
-    const uint64_t k = zl + t;
-    if (k >= zn) {
-        for (int l = 0; l < zw; l++) o[l] = 0;
-        return;
-    }
+    constexpr int W = FA_KS_UC + 1;
+    const int c = threadIdx.x;
+    const int nc = (int)(cnt - t0 < 256 ? cnt - t0 : 256);
+    const uint64_t k = zl + t0 + (uint64_t)c;
+    const bool mine = c < nc;
+    const bool live = mine && k < zn;
     const int neg = (sig[0] < 0) != (sig[1] < 0);
-    uint64_t carry = k ? tk[k - 1] : 0;
+    const int r = (int)((s * k) & 63);   // bit offset of the slot in its first limb
+    uint64_t carry = (live && k) ? tk[k - 1] : 0;
     uint64_t fill = 0;
     uint64_t ncarry = 1;   // two's complement negation: ~v + 1
-    for (int l = 0; l < zw; l++) {
-        const uint64_t lb = 64 * (uint64_t)l;
-        uint64_t v;
-        if (lb < s) {
-            v = fa_ks_slot(C, nP, s, k, l) + carry;
-            carry = (v < carry) ? 1 : 0;
-            if (lb + 64 >= s) {
-                const uint64_t nb = s - lb;   // 1..64 field bits in this limb
-                const uint64_t m = nb == 64 ? ~UINT64_C(0) : ((UINT64_C(1) << nb) - 1);
-                v &= m;
-                fill = ((v >> (nb - 1)) & 1) ? ~UINT64_C(0) : 0;
-                v |= fill & ~m;
+    for (int lc = 0; lc < zw; lc += FA_KS_UC) {
+        const int nl = zw - lc < FA_KS_UC ? zw - lc : FA_KS_UC;
+        // limbs lc .. lc + nl of each slot's span (limb lc + nl feeds the
+        // shift of limb lc + nl - 1); only those that reach the field
+        for (int i = c; i < nc * (nl + 1); i += 256) {
+            const int cc = i / (nl + 1);
+            const int j = i - cc * (nl + 1);
+            const uint64_t kk = zl + t0 + (uint64_t)cc;
+            uint64_t v = 0;
+            if (kk < zn && 64 * (uint64_t)(lc + j) < s + 64) {
+                const uint64_t q = ((s * kk) >> 6) + (uint64_t)(lc + j);
+                if (q < nP) v = C[q];
             }
-        } else {
-            v = fill;
+            buf[cc * W + j] = v;
         }
-        if (neg) {
-            v = ~v + ncarry;
-            ncarry = (ncarry && v == 0) ? 1 : 0;
+        __syncthreads();
+        if (mine) {
+            uint64_t* row = buf + c * W;
+            for (int j = 0; j < nl; j++) {
+                const uint64_t lb = 64 * (uint64_t)(lc + j);
+                uint64_t v;
+                if (!live) {
+                    v = 0;
+                } else {
+                    if (lb < s) {
+                        uint64_t e = row[j] >> r;
+                        if (r) e |= row[j + 1] << (64 - r);
+                        if (lb + 64 > s) e &= (UINT64_C(1) << (s - lb)) - 1;
+                        v = e + carry;
+                        carry = (v < carry) ? 1 : 0;
+                        if (lb + 64 >= s) {
+                            const uint64_t nb = s - lb;   // 1..64 field bits in this limb
+                            const uint64_t m = nb == 64 ? ~UINT64_C(0) : ((UINT64_C(1) << nb) - 1);
+                            v &= m;
+                            fill = ((v >> (nb - 1)) & 1) ? ~UINT64_C(0) : 0;
+                            v |= fill & ~m;
+                        }
+                    } else {
+                        v = fill;
+                    }
+                    if (neg) {
+                        v = ~v + ncarry;
+                        ncarry = (ncarry && v == 0) ? 1 : 0;
+                    }
+                }
+                row[j] = v;   // limb j + 1 is still the slot's, read next
+            }
         }
-        o[l] = v;
+        __syncthreads();
+        for (int i = c; i < nc * nl; i += 256) {
+            const int cc = i / nl;
+            const int j = i - cc * nl;
+            z[(t0 + (uint64_t)cc) * (uint64_t)zw + (uint64_t)(lc + j)] = buf[cc * W + j];
+        }
+        __syncthreads();
     }
 }
 
@@ -390,9 +438,8 @@ __global__ void __launch_bounds__(256) fa_k_ks_unpack(const uint64_t* __restrict
                                                       uint64_t zl, uint64_t cnt, uint64_t zn,
                                                       int zw, uint64_t* __restrict__ z)
 {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cnt) return;
-    ks_unpack_one(C, nP, s, tk, sig, zl, zn, zw, z, t);
+    __shared__ uint64_t buf[256 * (FA_KS_UC + 1)];
+    ks_unpack_tile(C, nP, s, tk, sig, zl, zn, cnt, zw, z, (uint64_t)blockIdx.x * 256, buf);
 }
 
 // ---------------------------------------------------------------------------
@@ -441,11 +488,12 @@ __global__ void __launch_bounds__(256) fa_kb_ks_unpack(const fa_bks* __restrict_
                                                        const uint64_t* __restrict__ work,
                                                        unsigned nwork)
 {
+    __shared__ uint64_t buf[256 * (FA_KS_UC + 1)];
     if (blockIdx.x >= nwork) return;
     const uint64_t wi = work[blockIdx.x];
     const fa_bks& r = R[(unsigned)(wi & 0x7fffffffu)];
-    const uint64_t t = (wi >> 32) * 256 + threadIdx.x;
-    if (t < r.cnt) ks_unpack_one(r.pr, r.nP, r.s, r.tk, r.sig, r.zl, r.zn, (int)r.zw, r.z, t);
+    const uint64_t t0 = (wi >> 32) * 256;
+    if (t0 < r.cnt) ks_unpack_tile(r.pr, r.nP, r.s, r.tk, r.sig, r.zl, r.zn, r.cnt, (int)r.zw, r.z, t0, buf);
 }
 
 int fa_batch_ks_pre(const fa_bks* R, unsigned nks, const uint64_t* work, unsigned nwork,
