@@ -19,13 +19,23 @@
 
 typedef unsigned __int128 u128;
 
+// limbs that hold sum_i x_i (M/p_i) < np M with np <= 8: those of M plus
+// one, at most FA_CRT_MAXL
+__host__ __device__ inline int fa_crt_poly_limbs(const fa_crt_consts& C)
+{
+    return C.nl + 1 < FA_CRT_MAXL ? C.nl + 1 : FA_CRT_MAXL;
+}
+
 // exact integer coefficient from np residues (same recombination as the
-// mpn path: x_i = c * cop_i^-1, c = sum x_i (M/p_i) - floor(sum x_i/p_i) M)
+// mpn path: x_i = c * cop_i^-1, c = sum x_i (M/p_i) - floor(sum x_i/p_i) M),
+// on NL limbs: enough for sum x_i (M/p_i) < np M (fa_crt_poly_limbs), so
+// the borrow after subtracting floor(.) M still tells a negative result
+template <int NL>
 FA_DEV void fa_crt_coeff_poly(const double* __restrict__ Q, uint64_t stride, uint64_t j,
-                              const fa_crt_consts& C, uint64_t s[FA_CRT_MAXL])
+                              const fa_crt_consts& C, uint64_t s[NL])
 {
 #pragma unroll
-    for (int l = 0; l < FA_CRT_MAXL; l++) s[l] = 0;
+    for (int l = 0; l < NL; l++) s[l] = 0;
     double qe = 0.0;
     for (int i = 0; i < C.np; i++) {
         uint64_t x = fa_reduce_0n_u64(Q[(uint64_t)i * stride + fa_res_pos(j, stride)], C.p[i],
@@ -33,7 +43,7 @@ FA_DEV void fa_crt_coeff_poly(const double* __restrict__ Q, uint64_t stride, uin
         qe += (double)x * C.rp[i];
         uint64_t carry = 0;
 #pragma unroll
-        for (int l = 0; l < FA_CRT_MAXL; l++) {
+        for (int l = 0; l < NL; l++) {
             uint64_t lo = x * C.cof[i][l];
             uint64_t hi = __umul64hi(x, C.cof[i][l]);
             uint64_t t = s[l] + lo;
@@ -47,7 +57,7 @@ FA_DEV void fa_crt_coeff_poly(const double* __restrict__ Q, uint64_t stride, uin
     uint64_t q = (uint64_t)qe;
     uint64_t borrow = 0, carry = 0;
 #pragma unroll
-    for (int l = 0; l < FA_CRT_MAXL; l++) {
+    for (int l = 0; l < NL; l++) {
         uint64_t lo = q * C.M[l];
         uint64_t hi = __umul64hi(q, C.M[l]);
         lo += carry;
@@ -63,7 +73,7 @@ FA_DEV void fa_crt_coeff_poly(const double* __restrict__ Q, uint64_t stride, uin
     if (borrow) {
         uint64_t c2 = 0;
 #pragma unroll
-        for (int l = 0; l < FA_CRT_MAXL; l++) {
+        for (int l = 0; l < NL; l++) {
             uint64_t t = s[l] + C.M[l];
             uint64_t cc = (t < s[l]);
             t += c2;
@@ -74,12 +84,12 @@ FA_DEV void fa_crt_coeff_poly(const double* __restrict__ Q, uint64_t stride, uin
     } else {
         int ge = 1;
 #pragma unroll
-        for (int l = FA_CRT_MAXL - 1; l >= 0; l--)
+        for (int l = NL - 1; l >= 0; l--)
             if (ge == 1 && s[l] != C.M[l]) ge = (s[l] > C.M[l]) ? 2 : 0;
         if (ge) {
             uint64_t br = 0;
 #pragma unroll
-            for (int l = 0; l < FA_CRT_MAXL; l++) {
+            for (int l = 0; l < NL; l++) {
                 uint64_t t = s[l] - C.M[l];
                 uint64_t b1 = (s[l] < C.M[l]);
                 uint64_t t2 = t - br;
@@ -111,6 +121,19 @@ FA_DEV uint64_t fa_ll_mod_preinv(uint64_t hi, uint64_t lo, uint64_t n, uint64_t 
     return r >> norm;
 }
 
+template <int NL>
+FA_DEV uint64_t fa_crt_nmod_one(const double* __restrict__ Q, uint64_t stride, uint64_t j,
+                                const fa_crt_consts& C, uint64_t n, uint64_t dinv, int norm)
+{
+    uint64_t s[NL];
+    fa_crt_coeff_poly<NL>(Q, stride, j, C, s);
+    uint64_t r = 0;
+#pragma unroll
+    for (int l = NL - 1; l >= 0; l--) r = fa_ll_mod_preinv(r, s[l], n, dinv, norm);
+    return r;
+}
+
+template <int NL>
 __global__ void __launch_bounds__(256) fa_k_crt_nmod(const double* __restrict__ Q, uint64_t stride,
                                                      fa_crt_consts C, uint64_t zl, uint64_t cnt,
                                                      uint64_t n, uint64_t dinv, int norm,
@@ -118,86 +141,75 @@ __global__ void __launch_bounds__(256) fa_k_crt_nmod(const double* __restrict__ 
 {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cnt) return;
-    uint64_t s[FA_CRT_MAXL];
-    fa_crt_coeff_poly(Q, stride, zl + t, C, s);
-    uint64_t r = 0;
-#pragma unroll
-    for (int l = FA_CRT_MAXL - 1; l >= 0; l--) r = fa_ll_mod_preinv(r, s[l], n, dinv, norm);
-    z[t] = r;
+    z[t] = fa_crt_nmod_one<NL>(Q, stride, zl + t, C, n, dinv, norm);
 }
 
 // centred lift (fmpz_poly_mul.c:209-227): s > M/2 -> s - M, written as a
-// zw-limb two's complement value
+// zw-limb two's complement value (limbs from NL on are the sign)
+template <int NL>
+FA_DEV void fa_emit_fmpz(uint64_t* s, const fa_crt_consts& C, uint64_t* o, uint64_t zw)
+{
+    int gt = 0;
+    for (int l = NL - 1; l >= 0; l--) {
+        uint64_t h = (C.M[l] >> 1) | ((l + 1 < FA_CRT_MAXL) ? (C.M[l + 1] << 63) : 0);
+        if (s[l] != h) { gt = s[l] > h; break; }
+    }
+    uint64_t ext = 0;
+    if (gt) {
+        uint64_t br = 0;
+#pragma unroll
+        for (int l = 0; l < NL; l++) {
+            uint64_t t1 = s[l] - C.M[l];
+            uint64_t b1 = (s[l] < C.M[l]);
+            uint64_t t2 = t1 - br;
+            uint64_t b2 = (t1 < br);
+            s[l] = t2;
+            br = b1 | b2;
+        }
+        ext = ~UINT64_C(0);
+    }
+    for (uint64_t l = 0; l < zw; l++) {
+        uint64_t v = ext;
+#pragma unroll
+        for (int k = 0; k < NL; k++)
+            if ((uint64_t)k == l) v = s[k];
+        o[l] = v;
+    }
+}
+
+template <int NL>
+FA_DEV void fa_crt_fmpz_one(const double* __restrict__ Q, uint64_t stride, uint64_t j,
+                            const fa_crt_consts& C, uint64_t* o, uint64_t zw)
+{
+    uint64_t s[NL];
+    fa_crt_coeff_poly<NL>(Q, stride, j, C, s);
+    fa_emit_fmpz<NL>(s, C, o, zw);
+}
+
+template <int NL>
 __global__ void __launch_bounds__(256) fa_k_crt_fmpz(const double* __restrict__ Q, uint64_t stride,
                                                      fa_crt_consts C, uint64_t zl, uint64_t cnt,
                                                      uint64_t zw, uint64_t* __restrict__ z)
 {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cnt) return;
-    uint64_t s[FA_CRT_MAXL];
-    fa_crt_coeff_poly(Q, stride, zl + t, C, s);
-    // compare s with floor(M/2)
-    int gt = 0;
-    for (int l = FA_CRT_MAXL - 1; l >= 0; l--) {
-        uint64_t h = (C.M[l] >> 1) | ((l + 1 < FA_CRT_MAXL) ? (C.M[l + 1] << 63) : 0);
-        if (s[l] != h) { gt = s[l] > h; break; }
-    }
-    uint64_t ext = 0;
-    if (gt) {
-        uint64_t br = 0;
-#pragma unroll
-        for (int l = 0; l < FA_CRT_MAXL; l++) {
-            uint64_t t1 = s[l] - C.M[l];
-            uint64_t b1 = (s[l] < C.M[l]);
-            uint64_t t2 = t1 - br;
-            uint64_t b2 = (t1 < br);
-            s[l] = t2;
-            br = b1 | b2;
-        }
-        ext = ~UINT64_C(0);
-    }
-    uint64_t* o = z + t * zw;
-    for (uint64_t l = 0; l < zw; l++) {
-        uint64_t v = ext;
-#pragma unroll
-        for (int k = 0; k < FA_CRT_MAXL; k++)
-            if ((uint64_t)k == l) v = s[k];
-        o[l] = v;
-    }
+    fa_crt_fmpz_one<NL>(Q, stride, zl + t, C, z + t * zw, zw);
 }
+
+// NL = fa_crt_poly_limbs(C) (2 .. FA_CRT_MAXL) as a template argument
+#define FA_CRT_NL_SWITCH(nl, F) \
+    switch (nl) {               \
+    case 2: F(2); break;        \
+    case 3: F(3); break;        \
+    case 4: F(4); break;        \
+    case 5: F(5); break;        \
+    case 6: F(6); break;        \
+    case 7: F(7); break;        \
+    default: F(8); break;       \
+    }
 
 // batched nmod / fmpz CRT: work item = job | chunk << 32, 256 outputs per
 // chunk; outputs at or beyond the product length are zero
-FA_DEV void fa_emit_fmpz(uint64_t* s, const fa_crt_consts& C, uint64_t* o, uint64_t zw)
-{
-    int gt = 0;
-    for (int l = FA_CRT_MAXL - 1; l >= 0; l--) {
-        uint64_t h = (C.M[l] >> 1) | ((l + 1 < FA_CRT_MAXL) ? (C.M[l + 1] << 63) : 0);
-        if (s[l] != h) { gt = s[l] > h; break; }
-    }
-    uint64_t ext = 0;
-    if (gt) {
-        uint64_t br = 0;
-#pragma unroll
-        for (int l = 0; l < FA_CRT_MAXL; l++) {
-            uint64_t t1 = s[l] - C.M[l];
-            uint64_t b1 = (s[l] < C.M[l]);
-            uint64_t t2 = t1 - br;
-            uint64_t b2 = (t1 < br);
-            s[l] = t2;
-            br = b1 | b2;
-        }
-        ext = ~UINT64_C(0);
-    }
-    for (uint64_t l = 0; l < zw; l++) {
-        uint64_t v = ext;
-#pragma unroll
-        for (int k = 0; k < FA_CRT_MAXL; k++)
-            if ((uint64_t)k == l) v = s[k];
-        o[l] = v;
-    }
-}
-
 template <int FMPZ>
 __global__ void __launch_bounds__(256) fa_kb_crt_poly(const fa_bpoly* __restrict__ P,
                                                       const uint64_t* __restrict__ work,
@@ -218,20 +230,19 @@ __global__ void __launch_bounds__(256) fa_kb_crt_poly(const fa_bpoly* __restrict
             return;
         }
         const fa_crt_consts& C = *(const fa_crt_consts*)j.C;
-        uint64_t s[FA_CRT_MAXL];
-        fa_crt_coeff_poly(j.Q, j.stride, idx, C, s);
-        fa_emit_fmpz(s, C, o, (uint64_t)j.zw);
+#define FA_KB_FMPZ(NL) fa_crt_fmpz_one<NL>(j.Q, j.stride, idx, C, o, (uint64_t)j.zw)
+        FA_CRT_NL_SWITCH(fa_crt_poly_limbs(C), FA_KB_FMPZ)
+#undef FA_KB_FMPZ
     } else {
         if (idx >= j.zn || !j.Q) {
             j.z[t] = 0;
             return;
         }
         const fa_crt_consts& C = *(const fa_crt_consts*)j.C;
-        uint64_t s[FA_CRT_MAXL];
-        fa_crt_coeff_poly(j.Q, j.stride, idx, C, s);
         uint64_t r = 0;
-#pragma unroll
-        for (int l = FA_CRT_MAXL - 1; l >= 0; l--) r = fa_ll_mod_preinv(r, s[l], j.n, j.dinv, j.norm);
+#define FA_KB_NMOD(NL) r = fa_crt_nmod_one<NL>(j.Q, j.stride, idx, C, j.n, j.dinv, j.norm)
+        FA_CRT_NL_SWITCH(fa_crt_poly_limbs(C), FA_KB_NMOD)
+#undef FA_KB_NMOD
         j.z[t] = r;
     }
 }
@@ -422,13 +433,19 @@ static int poly_mul_blocked(int fmpz, uint64_t* z, uint64_t zw, int64_t zl, int6
             const uint64_t wc = whi - wlo;
             const unsigned g = (unsigned)((wc + 255) / 256);
             if (fmpz) {
-                FA_LAUNCH("fa_k_crt_fmpz", fa_k_crt_fmpz, dim3(g), dim3(256), 0, s, Ar,
-                          fa_res_stride(depth), C, wlo - off, wc, zw, dt);
+#define FA_L(NL)                                                                            \
+    FA_LAUNCH("fa_k_crt_fmpz", fa_k_crt_fmpz<NL>, dim3(g), dim3(256), 0, s, Ar,             \
+              fa_res_stride(depth), C, wlo - off, wc, zw, dt)
+                FA_CRT_NL_SWITCH(fa_crt_poly_limbs(C), FA_L)
+#undef FA_L
                 FA_LAUNCH("fa_k_add_coeffs", fa_k_add_coeffs, dim3(g), dim3(256), 0, s,
                           dz + (wlo - (uint64_t)zl) * zw, dt, wc, zw);
             } else {
-                FA_LAUNCH("fa_k_crt_nmod", fa_k_crt_nmod, dim3(g), dim3(256), 0, s, Ar,
-                          fa_res_stride(depth), C, wlo - off, wc, n, dinv, norm, dt);
+#define FA_L(NL)                                                                            \
+    FA_LAUNCH("fa_k_crt_nmod", fa_k_crt_nmod<NL>, dim3(g), dim3(256), 0, s, Ar,             \
+              fa_res_stride(depth), C, wlo - off, wc, n, dinv, norm, dt)
+                FA_CRT_NL_SWITCH(fa_crt_poly_limbs(C), FA_L)
+#undef FA_L
                 FA_LAUNCH("fa_k_addmod", fa_k_addmod, dim3(g), dim3(256), 0, s,
                           dz + (wlo - (uint64_t)zl), dt, wc, n);
             }
@@ -492,8 +509,12 @@ extern "C" int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh, const u
     int norm = 64 - modbits;
     uint64_t d = n << norm;
     uint64_t dinv = (uint64_t)(((u128)0 - 1) / d - ((u128)1 << 64));
-    FA_LAUNCH("fa_k_crt_nmod", fa_k_crt_nmod, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
-              A, fa_res_stride(J.depth), J.C, (uint64_t)zl, cnt, n, dinv, norm, dz);
+#define FA_L(NL)                                                                            \
+    FA_LAUNCH("fa_k_crt_nmod", fa_k_crt_nmod<NL>, dim3((unsigned)((cnt + 255) / 256)),      \
+              dim3(256), 0, s, A, fa_res_stride(J.depth), J.C, (uint64_t)zl, cnt, n, dinv,   \
+              norm, dz)
+    FA_CRT_NL_SWITCH(fa_crt_poly_limbs(J.C), FA_L)
+#undef FA_L
     HIPTRY(hipGetLastError(), "crt launch");
     HIPTRY(hipMemcpyAsync(z, dz, cnt * sizeof(uint64_t), hipMemcpyDeviceToHost, s), "download");
     HIPTRY(hipStreamSynchronize(s), "synchronize");
@@ -622,8 +643,11 @@ extern "C" int fa_fmpz_poly_mul_mid(uint64_t* z, uint64_t zw, int64_t zl, int64_
     fa_pack_desc pb = {FA_PACK_S64W, db, (uint64_t)bn, 0, (int)bw};
     rc = fa_convolve(st, T, np, 0, depth, &pa, &pb, A, B, s);
     if (rc) return rc;
-    FA_LAUNCH("fa_k_crt_fmpz", fa_k_crt_fmpz, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s,
-              A, fa_res_stride(depth), C, (uint64_t)zl, cnt, zw, dz);
+#define FA_L(NL)                                                                            \
+    FA_LAUNCH("fa_k_crt_fmpz", fa_k_crt_fmpz<NL>, dim3((unsigned)((cnt + 255) / 256)),      \
+              dim3(256), 0, s, A, fa_res_stride(depth), C, (uint64_t)zl, cnt, zw, dz)
+    FA_CRT_NL_SWITCH(fa_crt_poly_limbs(C), FA_L)
+#undef FA_L
     HIPTRY(hipGetLastError(), "crt launch");
     HIPTRY(hipMemcpyAsync(z, dz, cnt * zw * sizeof(uint64_t), hipMemcpyDeviceToHost, s),
            "download");
