@@ -544,7 +544,8 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
     B->carry_jobs = L.take(carry.size() * 4 + 8);
     B->ncarry = (unsigned)carry.size();
     // signed-slot KS products: record index k of ks_live, work items in
-    // blocks of 256 packed limbs (per operand) and 256 output coefficients
+    // blocks of FA_KS_PACK_BLK packed limbs (per operand) and 256 output
+    // coefficients
     std::vector<size_t> ks_live;
     std::vector<uint64_t> ks_pw, ks_uw;
     if (ks_variant) {
@@ -553,8 +554,9 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
             if (q.status != 1) continue;
             const uint64_t idx = ks_live.size();
             ks_live.push_back(k);
-            for (uint64_t j = 0; j < (ksv[k].La + 255) / 256; j++) ks_pw.push_back(idx | (j << 32));
-            for (uint64_t j = 0; j < (ksv[k].Lb + 255) / 256; j++)
+            for (uint64_t j = 0; j < (ksv[k].La + FA_KS_PACK_BLK - 1) / FA_KS_PACK_BLK; j++)
+                ks_pw.push_back(idx | (j << 32));
+            for (uint64_t j = 0; j < (ksv[k].Lb + FA_KS_PACK_BLK - 1) / FA_KS_PACK_BLK; j++)
                 ks_pw.push_back(idx | (UINT64_C(1) << 31) | (j << 32));
             const uint64_t cnt = (uint64_t)(q.zh - q.zl);
             for (uint64_t j = 0; j < (cnt + 255) / 256; j++) ks_uw.push_back(idx | (j << 32));

@@ -192,6 +192,7 @@ int fa_nmod_primes(const fa_tables* T, uint64_t len, uint64_t n, int* np, int* o
 uint64_t fa_ks_slot_bits(int64_t b1, int64_t b2, uint64_t an, uint64_t bn);
 // Kronecker route: variant 1 = signed slots, one integer product; 0 = three
 // products of non-negative parts (fa_set_ks_variant)
+#define FA_KS_PACK_BLK 1024      // packed limbs per workgroup of the signed-slot pack
 struct fa_bks {                 // one signed-slot KS product of a batch
     const uint64_t* a;
     const uint64_t* b;

@@ -270,15 +270,69 @@ FA_DEV uint64_t ks_spack_limb(const uint64_t* __restrict__ a, int w, uint64_t n,
     return acc;
 }
 
+// the same limb for slots of at least 64 bits: the window [64 j, 64 j + 64)
+// meets slot i0 = floor(64 j / s) and at most slot i0 + 1, so both are read
+// with loads that do not wait on each other (the second at a clamped index
+// and used only when that slot starts inside the window)
+FA_DEV uint64_t ks_spack_limb_wide(const uint64_t* __restrict__ a, int w, uint64_t n,
+                                   const int32_t* __restrict__ low, const uint8_t* __restrict__ br,
+                                   int negs, uint64_t s, uint64_t j)
+{
+    const uint64_t bit0 = 64 * j;
+    uint64_t i0 = (uint64_t)((double)bit0 / (double)s);
+    if (i0 * s > bit0) i0--;
+    if (i0 >= n) return 0;
+    const uint64_t i1 = i0 + 1 < n ? i0 + 1 : i0;
+    const uint64_t rel = bit0 - s * i0;   // slot bit at bit0, in [0, s)
+    const int q = (int)(rel >> 6), r = (int)(rel & 63);
+    const uint64_t* c0 = a + i0 * (uint64_t)w;
+    const uint64_t* c1 = a + i1 * (uint64_t)w;
+    const int lo0 = low[i0], b0 = br[i0], lo1 = low[i1], b1 = br[i1];
+    uint64_t win = fa_ks_vlimb(c0, w, q, negs, b0, lo0) >> r;
+    if (r) win |= fa_ks_vlimb(c0, w, q + 1, negs, b0, lo0) << (64 - r);
+    const uint64_t next = fa_ks_vlimb(c1, w, 0, negs, b1, lo1);
+    // window positions below s - rel belong to slot i0, the rest (when
+    // s - rel < 64) to slot i0 + 1, whose bit 0 sits at position s - rel
+    const uint64_t phi = s - rel;
+    if (phi >= 64) return win;
+    uint64_t acc = win & ((UINT64_C(1) << phi) - 1);
+    if (i0 + 1 < n) acc |= next << phi;
+    return acc;
+}
+
+// packed limbs j0 + 256 k + threadIdx.x (k < K) below L: all computed
+// before any is stored, so their loads can be in flight together.  A batch
+// takes K = FA_KS_PACK_BLK / 256; one product alone takes K = 1, which
+// leaves more waves per CU to hide the loads of a small grid.
+template <int K>
+FA_DEV void ks_spack_block(const uint64_t* __restrict__ a, int w, uint64_t n,
+                           const int32_t* __restrict__ low, const uint8_t* __restrict__ br,
+                           int negs, uint64_t s, uint64_t* __restrict__ out, uint64_t L,
+                           uint64_t j0)
+{
+    uint64_t v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const uint64_t j = j0 + 256 * (uint64_t)k + threadIdx.x;
+        v[k] = 0;
+        if (j < L)
+            v[k] = s >= 64 ? ks_spack_limb_wide(a, w, n, low, br, negs, s, j)
+                           : ks_spack_limb(a, w, n, low, br, negs, s, j);
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const uint64_t j = j0 + 256 * (uint64_t)k + threadIdx.x;
+        if (j < L) out[j] = v[k];
+    }
+}
+
 __global__ void __launch_bounds__(256) fa_k_ks_spack(const uint64_t* __restrict__ a, int w,
                                                      uint64_t n, const int32_t* __restrict__ low,
                                                      const uint8_t* __restrict__ br,
                                                      const int32_t* __restrict__ sig, uint64_t s,
                                                      uint64_t* __restrict__ out, uint64_t L)
 {
-    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= L) return;
-    out[j] = ks_spack_limb(a, w, n, low, br, *sig < 0, s, j);
+    ks_spack_block<1>(a, w, n, low, br, *sig < 0, s, out, L, (uint64_t)blockIdx.x * 256);
 }
 
 // generate and propagate of slot k of the product: the slot's top bit, and
@@ -446,7 +500,7 @@ __global__ void __launch_bounds__(256) fa_k_ks_unpack(const uint64_t* __restrict
 // batched signed-slot route: every KS product of a batch in one launch per
 // step.  Flags and slot scans take one workgroup per operand or product;
 // packing and unpacking take work items (record | operand << 31 | block of
-// 256 limbs or coefficients << 32).
+// FA_KS_PACK_BLK packed limbs or 256 output coefficients << 32).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(1024) fa_kb_ks_sflags(const fa_bks* __restrict__ R, unsigned nks)
 {
@@ -468,12 +522,11 @@ __global__ void __launch_bounds__(256) fa_kb_ks_spack(const fa_bks* __restrict__
     const uint64_t wi = work[blockIdx.x];
     const fa_bks& r = R[(unsigned)(wi & 0x7fffffffu)];
     const int op = (int)((wi >> 31) & 1);
-    const uint64_t j = (wi >> 32) * 256 + threadIdx.x;
-    if (op) {
-        if (j < r.Lb) r.pb[j] = ks_spack_limb(r.b, (int)r.bw, r.bn, r.lowb, r.brb, r.sig[1] < 0, r.s, j);
-    } else {
-        if (j < r.La) r.pa[j] = ks_spack_limb(r.a, (int)r.aw, r.an, r.lowa, r.bra, r.sig[0] < 0, r.s, j);
-    }
+    const uint64_t j0 = (wi >> 32) * FA_KS_PACK_BLK;
+    if (op)
+        ks_spack_block<FA_KS_PACK_BLK / 256>(r.b, (int)r.bw, r.bn, r.lowb, r.brb, r.sig[1] < 0, r.s, r.pb, r.Lb, j0);
+    else
+        ks_spack_block<FA_KS_PACK_BLK / 256>(r.a, (int)r.aw, r.an, r.lowa, r.bra, r.sig[0] < 0, r.s, r.pa, r.La, j0);
 }
 
 __global__ void __launch_bounds__(1024) fa_kb_ks_uscan(const fa_bks* __restrict__ R, unsigned nks)
