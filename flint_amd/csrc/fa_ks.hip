@@ -403,6 +403,8 @@ __global__ void __launch_bounds__(1024) fa_k_ks_uscan(const uint64_t* __restrict
 
 // limbs of each coefficient staged through LDS per step of the unpack
 #define FA_KS_UC 16
+// the reciprocal divisions of the unpack are exact for divisors below 64
+static_assert(FA_KS_UC + 1 < 64, "unpack staging width");
 
 // z[t] = coefficient zl + t: the s-bit field e_k + t_(k-1) read as signed,
 // times sigma_a sigma_b, as zw limbs; zero past the product's length zn.
@@ -431,8 +433,12 @@ FA_DEV void ks_unpack_tile(const uint64_t* __restrict__ C, uint64_t nP, uint64_t
         const int nl = zw - lc < FA_KS_UC ? zw - lc : FA_KS_UC;
         // limbs lc .. lc + nl of each slot's span (limb lc + nl feeds the
         // shift of limb lc + nl - 1); only those that reach the field
+        // i / d for i < 256 d and d <= FA_KS_UC + 1 as (i ceil(2^20 / d)) >> 20:
+        // exact (error below 256 d / 2^20 < 1 / d for d < 64) and within 32 bits
+        const uint32_t m1 = ((1u << 20) + (uint32_t)nl) / (uint32_t)(nl + 1);
+        const uint32_t m0 = ((1u << 20) + (uint32_t)nl - 1) / (uint32_t)nl;
         for (int i = c; i < nc * (nl + 1); i += 256) {
-            const int cc = i / (nl + 1);
+            const int cc = (int)(((uint32_t)i * m1) >> 20);
             const int j = i - cc * (nl + 1);
             const uint64_t kk = zl + t0 + (uint64_t)cc;
             uint64_t v = 0;
@@ -477,7 +483,7 @@ FA_DEV void ks_unpack_tile(const uint64_t* __restrict__ C, uint64_t nP, uint64_t
         }
         __syncthreads();
         for (int i = c; i < nc * nl; i += 256) {
-            const int cc = i / nl;
+            const int cc = (int)(((uint32_t)i * m0) >> 20);
             const int j = i - cc * nl;
             z[(t0 + (uint64_t)cc) * (uint64_t)zw + (uint64_t)(lc + j)] = buf[cc * W + j];
         }
