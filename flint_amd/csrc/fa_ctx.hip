@@ -490,6 +490,17 @@ static void crt_consts_build(fa_crt_consts* C, int np, int offset)
     C->nl = FA_CRT_MAXL;
     while (C->nl > 1 && C->M[C->nl - 1] == 0) C->nl--;
     for (int i = 0; i < np; i++) ml_divexact_1(C->cof[i], C->M, FA_CRT_MAXL, P[offset + i]);
+    for (int k = 0; k < np; k++) {
+        const uint64_t pk = P[offset + k];
+        C->gcm[k] = (double)ml_mod_1(C->cof[k], FA_CRT_MAXL, pk);
+        uint64_t pre = 1 % pk;
+        for (int j = 0; j < k; j++) {
+            const uint64_t pj = P[offset + j] % pk;
+            C->gpm[k][j] = (double)pj;
+            pre = fa_mulmod_u64(pre, pj, pk);
+        }
+        C->ginv[k] = (double)fa_powmod_u64(pre, pk - 2, pk);
+    }
 }
 
 // 24-bit digits of a multi-limb number

@@ -63,6 +63,10 @@ struct fa_crt_consts {
     double rp[FA_NPRIMES];         // 1/p_i for the quotient estimate
     uint64_t M[FA_CRT_MAXL];
     uint64_t cof[FA_NPRIMES][FA_CRT_MAXL];   // M / p_i
+    // mixed radix (Garner): c = t_0 + p_0 (t_1 + p_1 (t_2 + ...)), t_k in [0, p_k)
+    double gcm[FA_NPRIMES];                  // (M / p_k) mod p_k
+    double ginv[FA_NPRIMES];                 // (p_0 ... p_(k-1))^-1 mod p_k
+    double gpm[FA_NPRIMES][FA_NPRIMES];      // [k][j] = p_j mod p_k, j < k
 };
 
 // the kernel-side view of the per-prime tables

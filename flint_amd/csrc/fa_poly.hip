@@ -121,19 +121,53 @@ FA_DEV uint64_t fa_ll_mod_preinv(uint64_t hi, uint64_t lo, uint64_t n, uint64_t 
     return r >> norm;
 }
 
-template <int NL>
+// coefficient j modulo n from its NP residues, through the mixed-radix
+// digits of c (Garner): t_0 = c mod p_0 and
+//     t_k = (c - (t_0 + p_0 (t_1 + ... p_(k-2) t_(k-1)))) (p_0 ... p_(k-1))^-1 mod p_k,
+// in FP64 mulmods, then c mod n by Horner's rule over the digits with
+// 128-bit products.  c is the same integer in [0, M) as the limb-by-limb
+// recombination gives, found without its rounded quotient: every step is
+// exact.  Bounds: each mulmod sees |a b| < 2 p_k^2 (primes within a factor
+// 2 of each other, the running value reduced to [-p_k/2, p_k/2] first), and
+// v (p_k mod n) + t_k < n 2^64 keeps the high word below n.
+template <int NP>
 FA_DEV uint64_t fa_crt_nmod_one(const double* __restrict__ Q, uint64_t stride, uint64_t j,
                                 const fa_crt_consts& C, uint64_t n, uint64_t dinv, int norm)
 {
-    uint64_t s[NL];
-    fa_crt_coeff_poly<NL>(Q, stride, j, C, s);
-    uint64_t r = 0;
+    const uint64_t pos = fa_res_pos(j, stride);
+    double t[NP];
+    {
+        const double p = C.p[0], pinv = C.pinv[0];
+        const double x = (double)fa_reduce_0n_u64(Q[pos], p, pinv);
+        t[0] = (double)fa_reduce_0n_u64(fa_mulmod(x, C.gcm[0], p, pinv), p, pinv);
+    }
 #pragma unroll
-    for (int l = NL - 1; l >= 0; l--) r = fa_ll_mod_preinv(r, s[l], n, dinv, norm);
-    return r;
+    for (int k = 1; k < NP; k++) {
+        const double p = C.p[k], pinv = C.pinv[k];
+        const double x = (double)fa_reduce_0n_u64(Q[(uint64_t)k * stride + pos], p, pinv);
+        const double r = fa_mulmod(x, C.gcm[k], p, pinv);   // c mod p_k, in (-p_k, p_k)
+        double u = t[k - 1];
+#pragma unroll
+        for (int i = k - 2; i >= 0; i--)
+            u = fa_reduce_pm1n(fa_mulmod(u, C.gpm[k][i], p, pinv) + t[i], p, pinv);
+        u = fa_reduce_pm1n(u, p, pinv);
+        t[k] = (double)fa_reduce_0n_u64(fa_mulmod(r - u, C.ginv[k], p, pinv), p, pinv);
+    }
+    uint64_t v = fa_ll_mod_preinv(0, (uint64_t)t[NP - 1], n, dinv, norm);
+#pragma unroll
+    for (int k = NP - 2; k >= 0; k--) {
+        const uint64_t pk = fa_ll_mod_preinv(0, (uint64_t)C.p[k], n, dinv, norm);
+        const uint64_t tk = (uint64_t)t[k];
+        uint64_t lo = v * pk;
+        uint64_t hi = __umul64hi(v, pk);
+        lo += tk;
+        hi += (lo < tk);
+        v = fa_ll_mod_preinv(hi, lo, n, dinv, norm);
+    }
+    return v;
 }
 
-template <int NL>
+template <int NP>
 __global__ void __launch_bounds__(256) fa_k_crt_nmod(const double* __restrict__ Q, uint64_t stride,
                                                      fa_crt_consts C, uint64_t zl, uint64_t cnt,
                                                      uint64_t n, uint64_t dinv, int norm,
@@ -141,8 +175,21 @@ __global__ void __launch_bounds__(256) fa_k_crt_nmod(const double* __restrict__ 
 {
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cnt) return;
-    z[t] = fa_crt_nmod_one<NL>(Q, stride, zl + t, C, n, dinv, norm);
+    z[t] = fa_crt_nmod_one<NP>(Q, stride, zl + t, C, n, dinv, norm);
 }
+
+// NP = C.np (1 .. 8) as a template argument
+#define FA_CRT_NP_SWITCH(np, F) \
+    switch (np) {               \
+    case 1: F(1); break;        \
+    case 2: F(2); break;        \
+    case 3: F(3); break;        \
+    case 4: F(4); break;        \
+    case 5: F(5); break;        \
+    case 6: F(6); break;        \
+    case 7: F(7); break;        \
+    default: F(8); break;       \
+    }
 
 // centred lift (fmpz_poly_mul.c:209-227): s > M/2 -> s - M, written as a
 // zw-limb two's complement value (limbs from NL on are the sign)
@@ -240,8 +287,8 @@ __global__ void __launch_bounds__(256) fa_kb_crt_poly(const fa_bpoly* __restrict
         }
         const fa_crt_consts& C = *(const fa_crt_consts*)j.C;
         uint64_t r = 0;
-#define FA_KB_NMOD(NL) r = fa_crt_nmod_one<NL>(j.Q, j.stride, idx, C, j.n, j.dinv, j.norm)
-        FA_CRT_NL_SWITCH(fa_crt_poly_limbs(C), FA_KB_NMOD)
+#define FA_KB_NMOD(NP) r = fa_crt_nmod_one<NP>(j.Q, j.stride, idx, C, j.n, j.dinv, j.norm)
+        FA_CRT_NP_SWITCH(C.np, FA_KB_NMOD)
 #undef FA_KB_NMOD
         j.z[t] = r;
     }
@@ -441,10 +488,10 @@ static int poly_mul_blocked(int fmpz, uint64_t* z, uint64_t zw, int64_t zl, int6
                 FA_LAUNCH("fa_k_add_coeffs", fa_k_add_coeffs, dim3(g), dim3(256), 0, s,
                           dz + (wlo - (uint64_t)zl) * zw, dt, wc, zw);
             } else {
-#define FA_L(NL)                                                                            \
-    FA_LAUNCH("fa_k_crt_nmod", fa_k_crt_nmod<NL>, dim3(g), dim3(256), 0, s, Ar,             \
+#define FA_L(NP)                                                                            \
+    FA_LAUNCH("fa_k_crt_nmod", fa_k_crt_nmod<NP>, dim3(g), dim3(256), 0, s, Ar,             \
               fa_res_stride(depth), C, wlo - off, wc, n, dinv, norm, dt)
-                FA_CRT_NL_SWITCH(fa_crt_poly_limbs(C), FA_L)
+                FA_CRT_NP_SWITCH(C.np, FA_L)
 #undef FA_L
                 FA_LAUNCH("fa_k_addmod", fa_k_addmod, dim3(g), dim3(256), 0, s,
                           dz + (wlo - (uint64_t)zl), dt, wc, n);
@@ -509,11 +556,11 @@ extern "C" int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh, const u
     int norm = 64 - modbits;
     uint64_t d = n << norm;
     uint64_t dinv = (uint64_t)(((u128)0 - 1) / d - ((u128)1 << 64));
-#define FA_L(NL)                                                                            \
-    FA_LAUNCH("fa_k_crt_nmod", fa_k_crt_nmod<NL>, dim3((unsigned)((cnt + 255) / 256)),      \
+#define FA_L(NP)                                                                            \
+    FA_LAUNCH("fa_k_crt_nmod", fa_k_crt_nmod<NP>, dim3((unsigned)((cnt + 255) / 256)),      \
               dim3(256), 0, s, A, fa_res_stride(J.depth), J.C, (uint64_t)zl, cnt, n, dinv,   \
               norm, dz)
-    FA_CRT_NL_SWITCH(fa_crt_poly_limbs(J.C), FA_L)
+    FA_CRT_NP_SWITCH(J.C.np, FA_L)
 #undef FA_L
     HIPTRY(hipGetLastError(), "crt launch");
     HIPTRY(hipMemcpyAsync(z, dz, cnt * sizeof(uint64_t), hipMemcpyDeviceToHost, s), "download");
