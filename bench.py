@@ -108,33 +108,6 @@ def cpu_baseline(W, sample_log2):
     }
 
 
-def _res_words(depth):
-    """doubles of one prime's residue array (rows of 4096 at a pitch of 4224
-    above depth 12)"""
-    return (4224 << (depth - 12)) if depth > 12 else (1 << depth)
-
-
-def _mpn_ws(an, bn, fa):
-    p = fa.mpn_plan(an, bn)
-    return 16 * p["np"] * _res_words(p["depth"]) + 16 * (an + bn)
-
-
-def _plan_ws(q, fa):
-    """device workspace of a request (residue rows, KS scratch), for chunking
-    a stream"""
-    if q["kind"] == "mpn":
-        return _mpn_ws(len(q["a"]), len(q["b"]), fa)
-    an = len(q["a"]) // q.get("aw", 1)
-    bn = len(q["b"]) // q.get("bw", 1)
-    depth = max(4, int(np.ceil(np.log2(an + bn))))
-    ws = 16 * 8 * _res_words(depth) + 8 * (an + bn) * q.get("zw", 1)
-    if q["kind"] == "fmpz":
-        sb = abs(q["abits"]) + abs(q["bbits"]) + int(min(an, bn)).bit_length() + 1
-        la, lb = -(-an * sb // 64), -(-bn * sb // 64)
-        ws += _mpn_ws(la, lb, fa) + 8 * 2 * (la + lb)   # one product of the packed operands
-    return ws
-
-
 def batched_main(args, torch, fa):
     """batched workloads: one drop-in call per product, one host batch, and a
     device-resident batch replayed from a hipGraph, on the same requests"""
@@ -170,16 +143,7 @@ def batched_main(args, torch, fa):
     print("# plain: %.3f s" % t_plain, file=sys.stderr, flush=True)
 
     # chunks of at most 32 GB of residue workspace
-    chunks, cur, ws = [], [], 0
-    for k, q in enumerate(reqs_p):
-        w = _plan_ws(q, fa)
-        if cur and ws + w > (32 << 30):
-            chunks.append(cur)
-            cur, ws = [], 0
-        cur.append(k)
-        ws += w
-    if cur:
-        chunks.append(cur)
+    chunks = WL.chunks(reqs_p, 32 << 30)
 
     # host-pointer batches: one untimed pass over every chunk sizes the
     # per-thread device and pinned buffers the one-shot batch borrows (to the

@@ -54,6 +54,16 @@ class SdFftCtx(ctypes.Structure):
                 ("slot", ctypes.c_int32)]
 
 
+class MpnCtxStruct(ctypes.Structure):
+    """fa_mpn_ctx_struct (include/flint_amd.h)."""
+    _fields_ = [("p", _u64), ("primes", _u64 * 8), ("device", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("state", _VP), ("magic", _u64)]
+
+
+# C types of the drop-ins' CPU fallbacks (fa_set_cpu_fallback)
+MPN_MUL_FN = ctypes.CFUNCTYPE(None, _P64, _P64, _s64, _P64, _s64)
+NMOD_MUL_MID_FN = ctypes.CFUNCTYPE(None, _P64, _s64, _s64, _P64, _s64, _P64, _s64, NmodT)
+
 # exported symbols and their signatures: (restype, argtypes)
 SIGNATURES = {
     "mpn_mul_default_mpn_ctx": (None, [_P64, _P64, _s64, _P64, _s64]),
@@ -79,6 +89,15 @@ SIGNATURES = {
     "fa_stream": (_VP, []),
     "fa_synth_limbs": (None, [_u64, _P64, _u64, ctypes.c_int]),
     "fa_synth_mpn_pair": (None, [_u64, _P64, _u64, _P64, _u64]),
+    "fa_rng_seed": (None, [_VP, _u64]),
+    "fa_rng_next": (_u64, [_VP]),
+    "fa_rng_uniform": (_u64, [_VP, _u64, _u64]),
+    "fa_rng_double": (ctypes.c_double, [_VP]),
+    "fa_rng_log_uniform": (_u64, [_VP, ctypes.c_double, ctypes.c_double]),
+    "fa_rng_random_prime": (_u64, [_VP, ctypes.c_int]),
+    "fa_rng_fill_below": (None, [_VP, _VP, _u64, _u64]),
+    "fa_rng_fill_limbs": (None, [_VP, _VP, _u64, ctypes.c_int]),
+    "fa_rng_fill_signed": (None, [_VP, _VP, _u64, ctypes.c_int, _u64]),
     "fa_prof_enable": (None, [ctypes.c_int]),
     "fa_prof_reset": (None, []),
     "fa_prof_read": (ctypes.c_int, [ctypes.c_char_p, _PD, _P64, ctypes.c_int]),
@@ -91,7 +110,18 @@ SIGNATURES = {
     "fa_set_ks_variant": (None, [ctypes.c_int]),
     "fa_set_prime_groups": (None, [ctypes.c_int]),
     "fa_set_crt_variant": (None, [ctypes.c_int]),
-    "fa_probe_colcopy": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP]),
+    "fa_set_device_memory_limit": (None, [_u64]),
+    "fa_set_cpu_fallback": (None, [_VP, _VP]),
+    "fa_last_status": (ctypes.c_int, []),
+    "fa_mpn_ctx_init": (ctypes.c_int, [_VP, _u64]),
+    "fa_mpn_ctx_clear": (None, [_VP]),
+    "fa_get_default_mpn_ctx": (_VP, []),
+    "fa_mpn_ctx_mpn_mul": (ctypes.c_int, [_VP, _P64, _P64, _s64, _P64, _s64]),
+    "fa_mpn_ctx_mpn_mul_device": (ctypes.c_int, [_VP, _VP, _VP, _s64, _VP, _s64, _VP]),
+    "fa_nmod_poly_mul_mid_mpn_ctx": (ctypes.c_int, [_P64, _s64, _s64, _P64, _s64, _P64, _s64,
+                                                    NmodT, _VP]),
+    "fa_fmpz_poly_mul_mid_mpn_ctx": (ctypes.c_int, [_P64, _u64, _s64, _s64, _P64, _u64, _s64,
+                                                    _P64, _u64, _s64, _VP]),
     "fa_batch_create": (ctypes.c_int, [ctypes.POINTER(_VP), _VP, _s64, ctypes.c_int]),
     "fa_batch_launch": (ctypes.c_int, [_VP, _VP]),
     "fa_batch_destroy": (None, [_VP]),
@@ -268,6 +298,77 @@ def mpn_mul_device_plan(z_ptr, a_ptr, an, b_ptr, bn, np_, bits, stream=None):
 def mpn_mul_device(z_ptr, a_ptr, an, b_ptr, bn, stream=None):
     """Device-resident product: raw device addresses, enqueued on `stream`."""
     return _check(lib().fa_mpn_mul_device(z_ptr, a_ptr, an, b_ptr, bn, stream))
+
+
+def set_device_memory_limit(nbytes):
+    """cap on the device bytes the library's grow-only buffers may hold
+    (0: none); growth past it fails with FA_E_NOMEM"""
+    lib().fa_set_device_memory_limit(ctypes.c_uint64(int(nbytes)))
+
+
+def last_status():
+    """status of this thread's last drop-in call (0, or the FA_E_* code of
+    the device failure that sent it to the CPU fallback)"""
+    return int(lib().fa_last_status())
+
+
+class MpnCtx:
+    """mpn_ctx_t (fft_small.h:382-433): a context with its own HIP stream and
+    device buffers; MpnCtx.default() is the thread's (get_default_mpn_ctx)."""
+
+    def __init__(self, p=0x0003F00000000001, _handle=None):
+        if _handle is not None:
+            self.ptr = _handle
+            self._own = None
+            return
+        self._own = MpnCtxStruct()
+        self.ptr = ctypes.addressof(self._own)
+        _check(lib().fa_mpn_ctx_init(self.ptr, p))
+
+    @classmethod
+    def default(cls):
+        return cls(_handle=lib().fa_get_default_mpn_ctx())
+
+    def close(self):
+        if self._own is not None:
+            lib().fa_mpn_ctx_clear(self.ptr)
+            self._own = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def mpn_mul(self, a, b):
+        """mpn_ctx_mpn_mul"""
+        a = _arr(a)
+        b = _arr(b)
+        z = np.empty(len(a) + len(b), dtype=np.uint64)
+        _check(lib().fa_mpn_ctx_mpn_mul(self.ptr, _p(z), _p(a), len(a), _p(b), len(b)))
+        return z
+
+    def mpn_mul_device(self, z_ptr, a_ptr, an, b_ptr, bn, stream=None):
+        return _check(lib().fa_mpn_ctx_mpn_mul_device(self.ptr, z_ptr, a_ptr, an, b_ptr, bn,
+                                                      stream))
+
+    def nmod_poly_mul_mid(self, a, b, zl, zh, n):
+        """_nmod_poly_mul_mid_mpn_ctx"""
+        a = _arr(a)
+        b = _arr(b)
+        z = np.zeros(max(zh - zl, 1), dtype=np.uint64)
+        _check(lib().fa_nmod_poly_mul_mid_mpn_ctx(_p(z), zl, zh, _p(a), len(a), _p(b), len(b),
+                                                  nmod_init(n), self.ptr))
+        return z[: max(zh - zl, 0)]
+
+    def fmpz_poly_mul_mid(self, a, aw, b, bw, zl, zh, zw):
+        """_fmpz_poly_mul_mid_mpn_ctx; None where the reference declines"""
+        a = _arr(a)
+        b = _arr(b)
+        z = np.zeros(max((zh - zl) * zw, 1), dtype=np.uint64)
+        rc = _check(lib().fa_fmpz_poly_mul_mid_mpn_ctx(_p(z), zw, zl, zh, _p(a), aw, len(a) // aw,
+                                                       _p(b), bw, len(b) // bw, self.ptr))
+        return z[: (zh - zl) * zw] if rc == 1 else None
 
 
 def nmod_init(n):

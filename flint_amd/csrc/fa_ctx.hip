@@ -1,10 +1,14 @@
 // fa_ctx.hip -- primes, device twiddle tables, per-thread state, plans.
+#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
+#include <new>
+#include <vector>
 
 #include "fa_arith.h"
 #include "fa_internal.h"
@@ -283,19 +287,48 @@ int fa_build_prime_tables(const fa_prime_host* pr, int depth, double** w2_out,
     return FA_OK;
 }
 
+// Tables superseded by a deeper build stay allocated until the process
+// ends: their addresses may sit in recorded batches and captured graphs
+// (fa_batch_create copies them), or in the arguments of launches another
+// thread is about to make from a fa_tables pointer it read before the
+// growth.  A deeper table starts with the entries of the shallower one, so
+// every holder of an old address still reads correct twiddles.  Growth
+// happens at most a few times per process (depth 20, then up to 25).
+static std::vector<void*> fa_retired_tables;
+static std::mutex fa_retired_mutex;
+
+void fa_retire_device_table(void* p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> g(fa_retired_mutex);
+    fa_retired_tables.push_back(p);
+}
+
 static int build_tables(fa_tables* T, int depth)
 {
     uint64_t P[FA_NPRIMES];
     default_primes(P);
+    fa_prime_host pr[FA_NPRIMES];
+    double* w2[FA_NPRIMES] = {};
+    double* w2i[FA_NPRIMES] = {};
     for (int i = 0; i < FA_NPRIMES; i++) {
-        fa_prime_host* pr = &T->pr[i];
-        fa_prime_info(P[i], pr);
-        double *w2, *w2i;
-        int rc = fa_build_prime_tables(pr, depth, &w2, &w2i);
-        if (rc) return rc;
-        if (T->w2[i]) { (void)hipFree(T->w2[i]); (void)hipFree(T->w2inv[i]); }
-        T->w2[i] = w2;
-        T->w2inv[i] = w2i;
+        fa_prime_info(P[i], &pr[i]);
+        int rc = fa_build_prime_tables(&pr[i], depth, &w2[i], &w2i[i]);
+        if (rc) {
+            for (int j = 0; j < i; j++) {
+                (void)hipFree(w2[j]);
+                (void)hipFree(w2i[j]);
+            }
+            return rc;
+        }
+    }
+    // publish the whole new set, then retire the old one
+    for (int i = 0; i < FA_NPRIMES; i++) {
+        fa_retire_device_table(T->w2[i]);
+        fa_retire_device_table(T->w2inv[i]);
+        T->pr[i] = pr[i];
+        T->w2[i] = w2[i];
+        T->w2inv[i] = w2i[i];
     }
     T->depth = depth;
     return FA_OK;
@@ -332,6 +365,17 @@ fa_tables* fa_get_tables(int depth)
     return T;
 }
 
+// the current device's tables if they already hold `depth` (never grows
+// them, so never allocates)
+const fa_tables* fa_tables_ready(int depth)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return NULL;
+    std::lock_guard<std::mutex> g(fa_table_mutex);
+    const fa_tables* T = fa_table_cache[dev];
+    return (T && T->depth >= depth) ? T : NULL;
+}
+
 extern "C" int fa_init(void)
 {
     return fa_get_tables(20) ? FA_OK : FA_E_NODEV;
@@ -342,6 +386,45 @@ extern "C" int fa_init(void)
 // ------------------------------------------------------------------------
 
 static thread_local fa_thread_state* fa_tls_state = NULL;
+// an explicit context (fa_mpn_ctx_init) lends its own state to the calls
+// made through it, in place of the thread's default one
+static thread_local fa_thread_state* fa_tls_override = NULL;
+
+fa_thread_state* fa_state_new(int dev)
+{
+    fa_thread_state* st = (fa_thread_state*)calloc(1, sizeof(fa_thread_state));
+    if (!st) {
+        fa_set_error(FA_E_NOMEM, "host allocation failed");
+        return NULL;
+    }
+    st->device = dev;
+    if (hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking) != hipSuccess) {
+        fa_set_error(FA_E_HIP, "stream creation failed");
+        free(st);
+        return NULL;
+    }
+    return st;
+}
+
+void fa_state_free(fa_thread_state* st)
+{
+    if (!st) return;
+    (void)hipStreamSynchronize(st->stream);
+    if (st->last_stream && st->last_stream != st->stream) (void)hipEventSynchronize(st->last_ev);
+    if (st->scratch) (void)hipFree(st->scratch);
+    if (st->hbuf) (void)hipFree(st->hbuf);
+    if (st->tbuf) (void)hipFree(st->tbuf);
+    if (st->last_ev) (void)hipEventDestroy(st->last_ev);
+    (void)hipStreamDestroy(st->stream);
+    free(st);
+}
+
+fa_thread_state* fa_state_swap(fa_thread_state* st)
+{
+    fa_thread_state* prev = fa_tls_override;
+    fa_tls_override = st;
+    return prev;
+}
 
 fa_thread_state* fa_state()
 {
@@ -350,18 +433,49 @@ fa_thread_state* fa_state()
         fa_set_error(FA_E_NODEV, "no HIP device");
         return NULL;
     }
+    if (fa_tls_override && fa_tls_override->device == dev) return fa_tls_override;
     fa_thread_state* st = fa_tls_state;
     if (st && st->device == dev) return st;
-    st = (fa_thread_state*)calloc(1, sizeof(fa_thread_state));
-    st->device = dev;
-    if (hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking) != hipSuccess) {
-        fa_set_error(FA_E_HIP, "stream creation failed");
-        free(st);
-        return NULL;
-    }
-    fa_tls_state = st;
+    st = fa_state_new(dev);
+    if (st) fa_tls_state = st;
     return st;
 }
+
+// The buffers of a state (scratch, hbuf, tbuf) serve every call made with
+// it.  A device-pointer call returns as soon as its work is enqueued on the
+// caller's stream, so the next call, perhaps on another stream, could
+// overwrite buffers still being read.  fa_buffers_leave records an event
+// after a call's last use; fa_buffers_enter makes a call on a different
+// stream wait for it.  Streams being captured into a graph neither wait nor
+// record: a graph's replays are ordered by the stream they are launched on.
+static int stream_capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+int fa_buffers_enter(fa_thread_state* st, hipStream_t s)
+{
+    if (!st->last_stream || st->last_stream == s || stream_capturing(s)) return FA_OK;
+    HIPCHECK(hipStreamWaitEvent(s, st->last_ev, 0));
+    return FA_OK;
+}
+
+int fa_buffers_leave(fa_thread_state* st, hipStream_t s)
+{
+    if (stream_capturing(s)) return FA_OK;
+    if (!st->last_ev) HIPCHECK(hipEventCreateWithFlags(&st->last_ev, hipEventDisableTiming));
+    HIPCHECK(hipEventRecord(st->last_ev, s));
+    st->last_stream = s;
+    return FA_OK;
+}
+
+// device bytes the grow-only buffers of all states may hold together
+// (0: no limit); fa_set_device_memory_limit
+static std::atomic<uint64_t> fa_mem_limit{0};
+static std::atomic<uint64_t> fa_mem_held{0};
+
+extern "C" void fa_set_device_memory_limit(uint64_t bytes) { fa_mem_limit = bytes; }
 
 static int grow(void** buf, size_t* size, size_t bytes, hipStream_t s)
 {
@@ -373,38 +487,52 @@ static int grow(void** buf, size_t* size, size_t bytes, hipStream_t s)
         (void)s;
         (void)hipDeviceSynchronize();
         (void)hipFree(*buf);
+        fa_mem_held -= *size;
         *buf = NULL;
         *size = 0;
     }
     size_t want = bytes + bytes / 8;
     want = (want + 0xFFFFF) & ~(size_t)0xFFFFF;
+    const uint64_t lim = fa_mem_limit;
+    if (lim && fa_mem_held + want > lim)
+        return fa_set_error(FA_E_NOMEM,
+                            "device allocation of %zu bytes exceeds the library limit of %llu "
+                            "bytes (%llu held)",
+                            want, (unsigned long long)lim, (unsigned long long)fa_mem_held.load());
     if (hipMalloc(buf, want) != hipSuccess) {
         *buf = NULL;
+        (void)hipGetLastError();
         return fa_set_error(FA_E_NOMEM, "device allocation of %zu bytes failed", want);
     }
+    fa_mem_held += want;
     *size = want;
     return FA_OK;
 }
 
-int fa_scratch_reserve(fa_thread_state* st, size_t bytes, void** out)
+// s: the stream the caller is about to use the buffer on (null: the
+// state's own); it first waits for the previous call's use on another stream
+static int reserve(fa_thread_state* st, void** buf, size_t* size, size_t bytes, void** out,
+                   hipStream_t s)
 {
-    int rc = grow(&st->scratch, &st->scratch_size, bytes, st->stream);
-    *out = st->scratch;
+    int rc = fa_buffers_enter(st, s ? s : st->stream);
+    if (rc == FA_OK) rc = grow(buf, size, bytes, s);
+    *out = *buf;
     return rc;
 }
 
-int fa_hbuf_reserve(fa_thread_state* st, size_t bytes, void** out)
+int fa_scratch_reserve(fa_thread_state* st, size_t bytes, void** out, hipStream_t s)
 {
-    int rc = grow(&st->hbuf, &st->hbuf_size, bytes, st->stream);
-    *out = st->hbuf;
-    return rc;
+    return reserve(st, &st->scratch, &st->scratch_size, bytes, out, s);
 }
 
-int fa_tbuf_reserve(fa_thread_state* st, size_t bytes, void** out)
+int fa_hbuf_reserve(fa_thread_state* st, size_t bytes, void** out, hipStream_t s)
 {
-    int rc = grow(&st->tbuf, &st->tbuf_size, bytes, st->stream);
-    *out = st->tbuf;
-    return rc;
+    return reserve(st, &st->hbuf, &st->hbuf_size, bytes, out, s);
+}
+
+int fa_tbuf_reserve(fa_thread_state* st, size_t bytes, void** out, hipStream_t s)
+{
+    return reserve(st, &st->tbuf, &st->tbuf_size, bytes, out, s);
 }
 
 extern "C" int fa_sync(void)
@@ -745,4 +873,106 @@ extern "C" void fa_synth_mpn_pair(uint64_t seed, uint64_t* a, uint64_t an, uint6
     synth_rng r(seed);
     r.limbs(a, an, 1);
     r.limbs(b, bn, 1);
+}
+
+// ------------------------------------------------------------------------
+// the request generator of the batched workloads: one xoshiro256** stream
+// (seeded by SplitMix64) per workload, drawn in a fixed order by the caller
+// ------------------------------------------------------------------------
+
+static synth_rng* rng_of(fa_rng* r) { return reinterpret_cast<synth_rng*>(r); }
+
+extern "C" void fa_rng_seed(fa_rng* r, uint64_t seed)
+{
+    static_assert(sizeof(fa_rng) == sizeof(synth_rng), "rng state");
+    new (rng_of(r)) synth_rng(seed);
+}
+
+extern "C" uint64_t fa_rng_next(fa_rng* r) { return rng_of(r)->next(); }
+
+// uniform in [lo, hi] by rejection of the top partial range
+extern "C" uint64_t fa_rng_uniform(fa_rng* r, uint64_t lo, uint64_t hi)
+{
+    const uint64_t range = hi - lo + 1;
+    if (range == 0) return rng_of(r)->next();
+    const uint64_t lim = ~UINT64_C(0) - (~UINT64_C(0) % range + 1) % range;
+    for (;;) {
+        const uint64_t x = rng_of(r)->next();
+        if (x <= lim) return lo + x % range;
+    }
+}
+
+// uniform double in [0, 1) from the top 53 bits
+extern "C" double fa_rng_double(fa_rng* r)
+{
+    return (double)(rng_of(r)->next() >> 11) * 0x1p-53;
+}
+
+// floor(2^U), U uniform in [x, y)
+extern "C" uint64_t fa_rng_log_uniform(fa_rng* r, double x, double y)
+{
+    const double u = x + (y - x) * fa_rng_double(r);
+    return (uint64_t)exp2(u);
+}
+
+// odd draws in [2^(bits-1), 2^bits) until one is prime
+extern "C" uint64_t fa_rng_random_prime(fa_rng* r, int bits)
+{
+    if (bits < 2) bits = 2;
+    const uint64_t lo = UINT64_C(1) << (bits - 1);
+    const uint64_t hi = bits == 64 ? ~UINT64_C(0) : (UINT64_C(1) << bits) - 1;
+    for (;;) {
+        uint64_t x = fa_rng_uniform(r, lo, hi) | 1;
+        if (bits == 2) x = fa_rng_uniform(r, 2, 3);
+        if (is_prime_u64(x)) return x;
+    }
+}
+
+// n values uniform in [0, bound)
+extern "C" void fa_rng_fill_below(fa_rng* r, uint64_t* out, uint64_t n, uint64_t bound)
+{
+    for (uint64_t i = 0; i < n; i++) out[i] = bound ? fa_rng_uniform(r, 0, bound - 1) : 0;
+}
+
+extern "C" void fa_rng_fill_limbs(fa_rng* r, uint64_t* out, uint64_t n, int top_nonzero)
+{
+    rng_of(r)->limbs(out, n, top_nonzero);
+}
+
+// n coefficients uniform in (-2^bits, 2^bits) as w-limb two's complement
+// (w >= bits / 64 + 1): t uniform in [0, 2^(bits+1) - 2] by rejection of
+// the all-ones (bits+1)-bit value, then c = t - (2^bits - 1)
+extern "C" void fa_rng_fill_signed(fa_rng* r, uint64_t* out, uint64_t n, int bits, uint64_t w)
+{
+    const uint64_t tw = (uint64_t)(bits + 1 + 63) / 64;     // limbs of t
+    const int topb = bits + 1 - 64 * (int)(tw - 1);          // bits in t's top limb
+    const uint64_t topmask = topb == 64 ? ~UINT64_C(0) : (UINT64_C(1) << topb) - 1;
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t* c = out + i * w;
+        for (;;) {
+            int all_ones = 1;
+            for (uint64_t k = 0; k < tw; k++) {
+                uint64_t v = rng_of(r)->next();
+                if (k == tw - 1) v &= topmask;
+                c[k] = v;
+                all_ones &= (v == (k == tw - 1 ? topmask : ~UINT64_C(0)));
+            }
+            if (!all_ones) break;
+        }
+        for (uint64_t k = tw; k < w; k++) c[k] = 0;
+        // subtract 2^bits - 1: limb k of the subtrahend is all ones below
+        // bit `bits`, zero above
+        uint64_t borrow = 0;
+        for (uint64_t k = 0; k < w; k++) {
+            const int lo = 64 * (int)k;
+            uint64_t m;
+            if (bits >= lo + 64) m = ~UINT64_C(0);
+            else if (bits <= lo) m = 0;
+            else m = (UINT64_C(1) << (bits - lo)) - 1;
+            const uint64_t x = c[k];
+            const uint64_t d = x - m - borrow;
+            borrow = (x < m) || (x - m < borrow);
+            c[k] = d;
+        }
+    }
 }

@@ -90,6 +90,8 @@ struct fa_thread_state {
     size_t hbuf_size;
     void* tbuf;                    // block products of integer products above one plan
     size_t tbuf_size;
+    hipEvent_t last_ev;            // recorded after the last call's last use of the buffers
+    hipStream_t last_stream;       // the stream it was recorded on (null: none pending)
 };
 
 // mpn plan chosen for the GPU
@@ -104,13 +106,30 @@ struct fa_mpn_plan_t {
 extern "C++" {
 int fa_set_error(int code, const char* fmt, ...);
 fa_tables* fa_get_tables(int depth);
+const fa_tables* fa_tables_ready(int depth);
+// one automatic-plan integer product in caller-provided device scratch of
+// cap bytes; allocates nothing (the tables must already hold its depth)
+int fa_mpn_mul_prealloc(uint64_t* z, const uint64_t* a, uint64_t an, const uint64_t* b,
+                        uint64_t bn, hipStream_t stream, void* buf, size_t cap);
 int fa_prime_info(uint64_t n, fa_prime_host* pr);
 int fa_build_prime_tables(const fa_prime_host* pr, int depth, double** w2, double** w2inv);
 int fa_mulmod_satisfies_bounds(uint64_t n);
+// keeps a superseded device table allocated until the process ends
+void fa_retire_device_table(void* p);
 fa_thread_state* fa_state();
-int fa_scratch_reserve(fa_thread_state* st, size_t bytes, void** out);
-int fa_hbuf_reserve(fa_thread_state* st, size_t bytes, void** out);
-int fa_tbuf_reserve(fa_thread_state* st, size_t bytes, void** out);
+fa_thread_state* fa_state_new(int device);
+void fa_state_free(fa_thread_state* st);
+// makes fa_state() return st on this thread (null: the thread's own); returns the previous
+fa_thread_state* fa_state_swap(fa_thread_state* st);
+// order the state's buffers across streams: enter before the first enqueue
+// of a call on stream s, leave after its last
+int fa_buffers_enter(fa_thread_state* st, hipStream_t s);
+int fa_buffers_leave(fa_thread_state* st, hipStream_t s);
+// grow-only buffers of a state; s = the stream the caller uses them on
+// (null: the state's own), which first waits for a call on another stream
+int fa_scratch_reserve(fa_thread_state* st, size_t bytes, void** out, hipStream_t s);
+int fa_hbuf_reserve(fa_thread_state* st, size_t bytes, void** out, hipStream_t s);
+int fa_tbuf_reserve(fa_thread_state* st, size_t bytes, void** out, hipStream_t s);
 void fa_crt_consts_init(fa_crt_consts* C, int np, int offset);
 uint64_t fa_crt_cop(int np, int i);         // (M/p_i) mod p_i
 int fa_mpn_choose_plan(uint64_t an, uint64_t bn, fa_mpn_plan_t* P);

@@ -790,7 +790,7 @@ extern "C" int fa_fmpz_poly_mul_ks(uint64_t* z, uint64_t zw, int64_t zl, int64_t
     // operands, output and KS scratch in the thread's grow-only staging
     // buffer (the integer product inside uses the separate scratch buffer)
     void* hb = nullptr;
-    int rc0 = fa_hbuf_reserve(st, (wa + wb + wz + ws) * 8, &hb);
+    int rc0 = fa_hbuf_reserve(st, (wa + wb + wz + ws) * 8, &hb, st->stream);
     if (rc0) return rc0;
     uint64_t* d = (uint64_t*)hb;
     uint64_t *da = d, *db = d + wa, *dz = db + wb, *dw = dz + wz;

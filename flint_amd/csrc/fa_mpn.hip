@@ -1328,7 +1328,7 @@ static int mpn_mul_blocked(uint64_t* z, const uint64_t* a, uint64_t an, const ui
     const uint64_t plen = la + lb;
     const size_t flag_bytes = (size_t)zn + zn / FA_ADD_BLK + 64;
     void* tb;
-    int rc = fa_tbuf_reserve(st, plen * 8 + flag_bytes, &tb);
+    int rc = fa_tbuf_reserve(st, plen * 8 + flag_bytes, &tb, stream);
     if (rc) return rc;
     uint64_t* tmp = (uint64_t*)tb;
     uint8_t* flags = (uint8_t*)(tmp + plen);
@@ -1361,27 +1361,14 @@ static int mpn_mul_device(uint64_t* z, const uint64_t* a, uint64_t an, const uin
     return mpn_mul_plan_device(z, a, an, b, bn, stream, np_force, bits_force);
 }
 
-// one product on one plan (`chosen`: the automatic plan, already made)
-static int mpn_mul_plan_device(uint64_t* z, const uint64_t* a, uint64_t an, const uint64_t* b,
-                               uint64_t bn, hipStream_t stream, int np_force, int bits_force,
-                               const fa_mpn_plan_t* chosen)
+// the launches of one product on plan P, with scratch `buf` of at least
+// mpn_scratch_bytes(P, an + bn) bytes
+static int mpn_mul_run(uint64_t* z, const uint64_t* a, uint64_t an, const uint64_t* b,
+                       uint64_t bn, hipStream_t stream, const fa_mpn_plan_t& P,
+                       const fa_tables* T, void* buf)
 {
-    fa_mpn_plan_t P;
-    int rc = FA_OK;
-    if (chosen) P = *chosen;
-    else rc = np_force ? fa_mpn_fixed_plan(an, bn, np_force, bits_force, &P)
-                       : fa_mpn_choose_plan(an, bn, &P);
-    if (rc) return rc;
-    fa_tables* T = fa_get_tables(P.depth);
-    if (!T) return FA_E_NODEV;
-    fa_thread_state* st = fa_state();
-    if (!st) return FA_E_NODEV;
-    if (!stream) stream = st->stream;
     uint64_t zn = an + bn, ngroups, nblk;
-    size_t bytes = mpn_scratch_bytes(P, zn, &ngroups, &nblk);
-    void* buf;
-    rc = fa_scratch_reserve(st, bytes, &buf);
-    if (rc) return rc;
+    (void)mpn_scratch_bytes(P, zn, &ngroups, &nblk);
     size_t tr = (size_t)P.np * fa_res_stride(P.depth) * sizeof(double);
     mpn_scratch W;
     W.A = (double*)buf;
@@ -1394,7 +1381,7 @@ static int mpn_mul_plan_device(uint64_t* z, const uint64_t* a, uint64_t an, cons
 
     fa_pack_desc da = {FA_PACK_MPN, a, an, P.bits, 1};
     fa_pack_desc db = {FA_PACK_MPN, b, bn, P.bits, 1};
-    rc = fa_convolve(st, T, P.np, 0, P.depth, &da, &db, W.A, W.B, stream);
+    int rc = fa_convolve(NULL, T, P.np, 0, P.depth, &da, &db, W.A, W.B, stream);
     if (rc) return rc;
 
     fa_crt_consts C;
@@ -1419,12 +1406,69 @@ static int mpn_mul_plan_device(uint64_t* z, const uint64_t* a, uint64_t an, cons
     return FA_OK;
 }
 
+// one product on one plan (`chosen`: the automatic plan, already made)
+static int mpn_mul_plan_device(uint64_t* z, const uint64_t* a, uint64_t an, const uint64_t* b,
+                               uint64_t bn, hipStream_t stream, int np_force, int bits_force,
+                               const fa_mpn_plan_t* chosen)
+{
+    fa_mpn_plan_t P;
+    int rc = FA_OK;
+    if (chosen) P = *chosen;
+    else rc = np_force ? fa_mpn_fixed_plan(an, bn, np_force, bits_force, &P)
+                       : fa_mpn_choose_plan(an, bn, &P);
+    if (rc) return rc;
+    fa_tables* T = fa_get_tables(P.depth);
+    if (!T) return FA_E_NODEV;
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    if (!stream) stream = st->stream;
+    uint64_t ngroups, nblk;
+    size_t bytes = mpn_scratch_bytes(P, an + bn, &ngroups, &nblk);
+    void* buf;
+    rc = fa_scratch_reserve(st, bytes, &buf, stream);
+    if (rc) return rc;
+    return mpn_mul_run(z, a, an, b, bn, stream, P, T, buf);
+}
+
+int fa_mpn_mul_prealloc(uint64_t* z, const uint64_t* a, uint64_t an, const uint64_t* b,
+                        uint64_t bn, hipStream_t stream, void* buf, size_t cap)
+{
+    fa_mpn_plan_t P;
+    int rc = fa_mpn_choose_plan(an, bn, &P);
+    if (rc) return rc;
+    uint64_t ngroups, nblk;
+    const size_t bytes = mpn_scratch_bytes(P, an + bn, &ngroups, &nblk);
+    if (bytes > cap)
+        return fa_set_error(FA_E_RANGE,
+                            "%llu x %llu limbs need %zu bytes of scratch, above the %zu available",
+                            (unsigned long long)an, (unsigned long long)bn, bytes, cap);
+    const fa_tables* T = fa_tables_ready(P.depth);
+    if (!T)
+        return fa_set_error(FA_E_ARG,
+                            "twiddle tables of depth %d are not built on this device (fa_init)",
+                            P.depth);
+    return mpn_mul_run(z, a, an, b, bn, stream, P, T, buf);
+}
+
+// a device-pointer call: enqueued on the caller's stream, after which the
+// thread's buffers are marked busy on that stream (fa_buffers_leave)
+static int mpn_mul_device_call(uint64_t* z, const uint64_t* a, uint64_t an, const uint64_t* b,
+                               uint64_t bn, hipStream_t stream, int np, int bits)
+{
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    hipStream_t s = stream ? stream : st->stream;
+    const int rc = mpn_mul_device(z, a, an, b, bn, s, np, bits);
+    const int rc2 = fa_buffers_leave(st, s);
+    return rc ? rc : rc2;
+}
+
 extern "C" int fa_mpn_mul_device(uint64_t* z, const uint64_t* a, int64_t an, const uint64_t* b,
                                  int64_t bn, void* stream)
 {
     if (an <= 0 || bn <= 0 || !z || !a || !b)
         return fa_set_error(FA_E_ARG, "fa_mpn_mul_device: bad arguments");
-    return mpn_mul_device(z, a, (uint64_t)an, b, (uint64_t)bn, (hipStream_t)stream);
+    return mpn_mul_device_call(z, a, (uint64_t)an, b, (uint64_t)bn, (hipStream_t)stream, 0, 0);
 }
 
 static int mpn_mul_host(uint64_t* z, const uint64_t* a, int64_t an, const uint64_t* b, int64_t bn,
@@ -1437,7 +1481,7 @@ static int mpn_mul_host(uint64_t* z, const uint64_t* a, int64_t an, const uint64
     uint64_t zn = (uint64_t)(an + bn);
     size_t bytes = ((size_t)an + (size_t)bn + zn) * sizeof(uint64_t);
     void* hb;
-    int rc = fa_hbuf_reserve(st, bytes, &hb);
+    int rc = fa_hbuf_reserve(st, bytes, &hb, st->stream);
     if (rc) return rc;
     uint64_t* da = (uint64_t*)hb;
     uint64_t* db = da + an;
@@ -1477,15 +1521,6 @@ extern "C" int fa_mpn_mul_device_plan(uint64_t* z, const uint64_t* a, int64_t an
 {
     if (an <= 0 || bn <= 0 || !z || !a || !b || np == 0)
         return fa_set_error(FA_E_ARG, "fa_mpn_mul_device_plan: bad arguments");
-    return mpn_mul_device(z, a, (uint64_t)an, b, (uint64_t)bn, (hipStream_t)stream, np, bits);
-}
-
-extern "C" void mpn_mul_default_mpn_ctx(uint64_t* r1, const uint64_t* i1, int64_t n1,
-                                        const uint64_t* i2, int64_t n2)
-{
-    int rc = fa_mpn_mul(r1, i1, n1, i2, n2);
-    if (rc != FA_OK) {
-        fprintf(stderr, "mpn_mul_default_mpn_ctx: %s\n", fa_last_error());
-        abort();
-    }
+    return mpn_mul_device_call(z, a, (uint64_t)an, b, (uint64_t)bn, (hipStream_t)stream, np,
+                               bits);
 }

@@ -342,7 +342,7 @@ static int poly_buffers(fa_thread_state* st, uint64_t an_words, uint64_t bn_word
     size_t tr = (size_t)np * fa_res_stride(depth) * sizeof(double);
     size_t bytes = 2 * tr + (an_words + bn_words + zwords) * sizeof(uint64_t) + 256;
     void* buf;
-    int rc = fa_scratch_reserve(st, bytes, &buf);
+    int rc = fa_scratch_reserve(st, bytes, &buf, st->stream);
     if (rc) return rc;
     *A = (double*)buf;
     *B = (double*)((char*)buf + tr);
@@ -442,7 +442,7 @@ static int poly_mul_blocked(int fmpz, uint64_t* z, uint64_t zw, int64_t zl, int6
     const size_t tr = (size_t)np * fa_res_stride(dmax) * sizeof(double);
     const size_t bytes = 2 * tr + (A_ * aw + B_ * bw + (cnt + tcap) * zwords) * 8 + 256;
     void* buf;
-    int rc = fa_scratch_reserve(st, bytes, &buf);
+    int rc = fa_scratch_reserve(st, bytes, &buf, st->stream);
     if (rc) return rc;
     double* Ar = (double*)buf;
     double* Br = (double*)((char*)buf + tr);
@@ -566,17 +566,6 @@ extern "C" int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh, const u
     HIPTRY(hipMemcpyAsync(z, dz, cnt * sizeof(uint64_t), hipMemcpyDeviceToHost, s), "download");
     HIPTRY(hipStreamSynchronize(s), "synchronize");
     return FA_OK;
-}
-
-extern "C" void _nmod_poly_mul_mid_default_mpn_ctx(uint64_t* res, int64_t zl, int64_t zh,
-                                                   const uint64_t* a, int64_t an,
-                                                   const uint64_t* b, int64_t bn, fa_nmod_t mod)
-{
-    int rc = fa_nmod_poly_mul_mid(res, zl, zh, a, an, b, bn, mod);
-    if (rc != FA_OK) {
-        fprintf(stderr, "_nmod_poly_mul_mid_default_mpn_ctx: %s\n", fa_last_error());
-        abort();
-    }
 }
 
 // _fmpz_vec_max_bits: max bit size of |c|, negated if some c < 0

@@ -121,43 +121,3 @@ extern "C" int fa_copy_device(void* dst, const void* src, uint64_t nbytes, void*
               (const uint4*)src, (uint4*)dst, n);
     return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "copy launch failed");
 }
-
-// Access-pattern probe: copies a 2^d1 x 2^d2 array of doubles (row-major)
-// with the memory pattern of the column passes (tiles of 2^lgC columns x
-// 2^d1 rows, each thread owning 16 rows of one column, XCD-aware tile
-// order) and no arithmetic.  Measures what that pattern alone sustains.
-__global__ void __launch_bounds__(1024) fa_k_probe_colcopy(const double* __restrict__ src,
-                                                           double* __restrict__ dst, int d1,
-                                                           int d2, int lgC, int ntiles)
-{
-    unsigned nb = gridDim.x, bid = blockIdx.x;
-    unsigned L = (bid & 7) * (nb >> 3) + (bid >> 3);
-    if ((int)L >= ntiles) return;
-    const int C = 1 << lgC;
-    const int c = threadIdx.x & (C - 1), tc = threadIdx.x >> lgC;
-    const int tpc = 1024 >> lgC;                    // threads per column
-    const uint64_t col = ((uint64_t)L << lgC) + c;
-    double x[16];
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        uint64_t row = (uint64_t)tc + (uint64_t)k * tpc;
-        x[k] = (row < (UINT64_C(1) << d1)) ? src[(row << d2) + col] : 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        uint64_t row = (uint64_t)tc + (uint64_t)k * tpc;
-        if (row < (UINT64_C(1) << d1)) dst[(row << d2) + col] = x[k];
-    }
-}
-
-// requires 16 * 1024 / 2^lgC == 2^d1 (each tile covers all rows)
-extern "C" int fa_probe_colcopy(const void* src, void* dst, int d1, int d2, int lgC, void* stream)
-{
-    if (d1 < 4 || d1 > 14 || d2 < lgC || (16 * 1024 >> lgC) != (1 << d1))
-        return fa_set_error(FA_E_ARG, "fa_probe_colcopy: tile does not cover the columns");
-    int ntiles = 1 << (d2 - lgC);
-    unsigned grid = (unsigned)((ntiles + 7) & ~7);
-    FA_LAUNCH("fa_k_probe_colcopy", fa_k_probe_colcopy, dim3(grid), dim3(1024), 0,
-              (hipStream_t)stream, (const double*)src, (double*)dst, d1, d2, lgC, ntiles);
-    return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "probe launch failed");
-}

@@ -273,7 +273,10 @@ int slot_fit_depth(slot_t* S, int L)
     double *w2, *w2i;
     int rc = fa_build_prime_tables(&S->pr, want, &w2, &w2i);
     if (rc) return rc;
-    if (S->w2) { (void)hipFree(S->w2); (void)hipFree(S->w2inv); }
+    // the old tables may still be read by launches in flight from other
+    // threads: retire them instead of freeing
+    fa_retire_device_table(S->w2);
+    fa_retire_device_table(S->w2inv);
     S->w2 = w2;
     S->w2inv = w2i;
     S->depth = want;
@@ -294,7 +297,7 @@ int prepare(fa_sd_fft_ctx_struct* Q, uint64_t L, sd_run* R, double** dbuf)
     fa_thread_state* st = fa_state();
     if (!st) return FA_E_NODEV;
     void* buf;
-    rc = fa_scratch_reserve(st, (size_t)2 * ((size_t)1 << L) * sizeof(double) + 256, &buf);
+    rc = fa_scratch_reserve(st, (size_t)2 * ((size_t)1 << L) * sizeof(double) + 256, &buf, st->stream);
     if (rc) return rc;
     R->S = S;
     R->s = st->stream;

@@ -16,9 +16,17 @@ fmpz products beyond the eight-prime capacity are computed by Kronecker
 substitution onto the integer product in every leg (the route FLINT's
 _fmpz_poly_mul takes when fft_small declines).
 
-Sizes are drawn with numpy's seeded generator (seed 0xF1A70000 + the
-workload's index: 3, 4, 5).  Limb-products of a request: an*bn for integer
-and nmod products, an*bn*wa*wb for fmpz with w = ceil((b+1)/64).
+Every draw comes from one xoshiro256** stream per workload, seeded by
+SplitMix64 with 0xF1A70000 + the workload's config index (3, 4, 5), through
+the library's generator (fa_rng_*): uniform integers by rejection,
+log-uniform sizes as floor(2^U), primes as odd uniform draws until
+Miller-Rabin accepts.  Per request the draws are, in order:
+    nmod:  an, bn (log-uniform), modulus bits (uniform), the prime, a, b
+    fmpz:  an, bn (uniform), b (uniform), a, b (signed, b // 64 + 1 limbs)
+    mpn:   an, bn (log-uniform), a, b (limbs, top limb nonzero)
+and in the stream a uniform double in [0, 1) first picks the kind.
+Limb-products of a request: an*bn for integer and nmod products,
+an*bn*wa*wb for fmpz with w = ceil((b+1)/64).
 """
 import ctypes
 import time
@@ -27,107 +35,138 @@ import numpy as np
 
 import flint_amd as fa
 
-
-def _is_prime(n):
-    if n < 2:
-        return False
-    for p in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
-        if n % p == 0:
-            return n == p
-    d, s = n - 1, 0
-    while d % 2 == 0:
-        d //= 2
-        s += 1
-    for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
-        x = pow(a, d, n)
-        if x in (1, n - 1):
-            continue
-        for _ in range(s - 1):
-            x = x * x % n
-            if x == n - 1:
-                break
-        else:
-            return False
-    return True
+SEEDS = {"nmod4096": 0xF1A70003, "fmpz256": 0xF1A70004, "stream": 0xF1A70005}
+COUNTS = {"nmod4096": 4096, "fmpz256": 256, "stream": 200000}
 
 
-def random_prime(rng, bits):
-    while True:
-        n = int(rng.integers(1 << (bits - 1), 1 << bits, dtype=np.uint64)) | 1
-        if _is_prime(n):
-            return n
+class Rng:
+    """the library's seeded generator (fa_rng_*)"""
+
+    def __init__(self, seed):
+        self.s = (ctypes.c_uint64 * 4)()
+        L = fa.lib()
+        self._L = L
+        L.fa_rng_seed(self.s, seed)
+
+    def uniform(self, lo, hi):
+        return int(self._L.fa_rng_uniform(self.s, lo, hi))
+
+    def double(self):
+        return float(self._L.fa_rng_double(self.s))
+
+    def log_uniform(self, x, y):
+        return int(self._L.fa_rng_log_uniform(self.s, x, y))
+
+    def prime(self, bits):
+        return int(self._L.fa_rng_random_prime(self.s, bits))
+
+    def below(self, n, bound):
+        out = np.empty(n, dtype=np.uint64)
+        self._L.fa_rng_fill_below(self.s, out.ctypes.data, n, bound)
+        return out
+
+    def limbs(self, n):
+        out = np.empty(n, dtype=np.uint64)
+        self._L.fa_rng_fill_limbs(self.s, out.ctypes.data, n, 1)
+        return out
+
+    def signed(self, n, bits, w):
+        out = np.empty(n * w, dtype=np.uint64)
+        self._L.fa_rng_fill_signed(self.s, out.ctypes.data, n, bits, w)
+        return out
 
 
-def signed_coeffs(rng, n, bits, w):
-    """n coefficients uniform in (-2^bits, 2^bits) as w-limb two's complement"""
-    mag = rng.integers(0, 2**64, (n, w), dtype=np.uint64, endpoint=False)
-    full = bits // 64
-    mag[:, full + 1:] = 0
-    if full < w:
-        rem = bits - 64 * full
-        mag[:, full] &= np.uint64((1 << rem) - 1) if rem else np.uint64(0)
-    neg = rng.integers(0, 2, n).astype(bool)
-    out = mag.copy()
-    if neg.any():
-        v = ~mag[neg]
-        carry = np.ones(len(v), dtype=np.uint64)
-        for k in range(w):
-            s = v[:, k] + carry
-            carry = ((s == 0) & (carry == 1)).astype(np.uint64)
-            v[:, k] = s
-        out[neg] = v
-    return out.ravel()
-
-
-def nmod_request(rng):
-    an = int(2 ** rng.uniform(6, 16))
-    bn = int(2 ** rng.uniform(6, 16))
-    n = random_prime(rng, int(rng.integers(20, 64)))
-    a = rng.integers(0, n, an, dtype=np.uint64)
-    b = rng.integers(0, n, bn, dtype=np.uint64)
+def nmod_request(r):
+    an = r.log_uniform(6, 16)
+    bn = r.log_uniform(6, 16)
+    n = r.prime(r.uniform(20, 63))
+    a = r.below(an, n)
+    b = r.below(bn, n)
     return {"kind": "nmod", "a": a, "b": b, "n": n, "zl": 0, "zh": an + bn - 1, "lp": an * bn}
 
 
-def fmpz_request(rng):
-    an = int(rng.integers(1024, 16385))
-    bn = int(rng.integers(1024, 16385))
-    bits = int(rng.integers(64, 4097))
+def fmpz_request(r):
+    an = r.uniform(1024, 16384)
+    bn = r.uniform(1024, 16384)
+    bits = r.uniform(64, 4096)
     w = bits // 64 + 1
+    a = r.signed(an, bits, w)
+    b = r.signed(bn, bits, w)
     zw = (2 * bits + 64) // 64 + 1
-    a = signed_coeffs(rng, an, bits, w)
-    b = signed_coeffs(rng, bn, bits, w)
     wa = -(-(bits + 1) // 64)
     return {"kind": "fmpz", "a": a, "aw": w, "b": b, "bw": w, "zl": 0, "zh": an + bn - 1,
-            "zw": zw, "abits": fa.fmpz_vec_max_bits(a, w), "bbits": fa.fmpz_vec_max_bits(b, w),
-            "lp": an * bn * wa * wa}
+            "zw": zw, "bits": bits, "abits": fa.fmpz_vec_max_bits(a, w),
+            "bbits": fa.fmpz_vec_max_bits(b, w), "lp": an * bn * wa * wa}
 
 
-def mpn_request(rng):
-    an = int(2 ** rng.uniform(4, 22))
-    bn = int(2 ** rng.uniform(4, 22))
-    a = rng.integers(0, 2**64, an, dtype=np.uint64, endpoint=False)
-    b = rng.integers(0, 2**64, bn, dtype=np.uint64, endpoint=False)
-    a[-1] |= np.uint64(1)
-    b[-1] |= np.uint64(1)
+def mpn_request(r):
+    an = r.log_uniform(4, 22)
+    bn = r.log_uniform(4, 22)
+    a = r.limbs(an)
+    b = r.limbs(bn)
     return {"kind": "mpn", "a": a, "b": b, "lp": an * bn}
 
 
+def generate(workload, n=None):
+    """yields the requests of a workload in order (n: how many; default all)"""
+    if workload not in SEEDS:
+        raise ValueError(workload)
+    r = Rng(SEEDS[workload])
+    for _ in range(COUNTS[workload] if n is None else n):
+        if workload == "nmod4096":
+            yield nmod_request(r)
+        elif workload == "fmpz256":
+            yield fmpz_request(r)
+        else:
+            u = r.double()
+            yield mpn_request(r) if u < 0.5 else nmod_request(r) if u < 0.8 else fmpz_request(r)
+
+
 def make(workload, n=None):
-    if workload == "nmod4096":
-        rng = np.random.default_rng(0xF1A70003)
-        return [nmod_request(rng) for _ in range(n or 4096)]
-    if workload == "fmpz256":
-        rng = np.random.default_rng(0xF1A70004)
-        return [fmpz_request(rng) for _ in range(n or 256)]
-    if workload == "stream":
-        rng = np.random.default_rng(0xF1A70005)
-        out = []
-        for _ in range(n or 200000):
-            u = rng.random()
-            out.append(mpn_request(rng) if u < 0.5 else
-                       nmod_request(rng) if u < 0.8 else fmpz_request(rng))
-        return out
-    raise ValueError(workload)
+    return list(generate(workload, n))
+
+
+def res_words(depth):
+    """doubles of one prime's residue array (rows of 4096 at a pitch of 4224
+    above depth 12)"""
+    return (4224 << (depth - 12)) if depth > 12 else (1 << depth)
+
+
+def mpn_workspace(an, bn):
+    p = fa.mpn_plan(an, bn)
+    return 16 * p["np"] * res_words(p["depth"]) + 16 * (an + bn)
+
+
+def request_workspace(q):
+    """device workspace of a request (residue rows, Kronecker scratch), for
+    cutting a stream into batches"""
+    if q["kind"] == "mpn":
+        return mpn_workspace(len(q["a"]), len(q["b"]))
+    an = len(q["a"]) // q.get("aw", 1)
+    bn = len(q["b"]) // q.get("bw", 1)
+    depth = max(4, int(np.ceil(np.log2(an + bn))))
+    ws = 16 * 8 * res_words(depth) + 8 * (an + bn) * q.get("zw", 1)
+    if q["kind"] == "fmpz":
+        sb = abs(q["abits"]) + abs(q["bbits"]) + int(min(an, bn)).bit_length() + 1
+        la, lb = -(-an * sb // 64), -(-bn * sb // 64)
+        ws += mpn_workspace(la, lb) + 8 * 2 * (la + lb)   # one product of the packed operands
+    return ws
+
+
+def chunks(reqs, cap):
+    """request indices in consecutive groups of at most `cap` bytes of
+    device workspace (a request above the cap is a group of its own)"""
+    out, cur, ws = [], [], 0
+    for k, q in enumerate(reqs):
+        w = request_workspace(q)
+        if cur and ws + w > cap:
+            out.append(cur)
+            cur, ws = [], 0
+        cur.append(k)
+        ws += w
+    if cur:
+        out.append(cur)
+    return out
 
 
 def plain(reqs):

@@ -45,17 +45,79 @@ typedef struct {
 /* ---------------------------------------------------------------------- */
 
 /* replaces mpn_mul_default_mpn_ctx, fft_small.h:852, default_ctx.c:41-45:
-   r1[0, n1+n2) = i1 * i2.  Aborts the process on device failure, as the
-   reference's flint_throw does (sd_fft_ctx.c:63-74). */
+   r1[0, n1+n2) = i1 * i2.  Never aborts: on a device failure it calls the
+   registered CPU fallback (fa_set_cpu_fallback), or reports on stderr and
+   leaves r1 unspecified; fa_last_status() tells which. */
 FA_API void mpn_mul_default_mpn_ctx(uint64_t* r1, const uint64_t* i1, int64_t n1,
                              const uint64_t* i2, int64_t n2);
 
 /* replaces _nmod_poly_mul_mid_default_mpn_ctx, fft_small.h:853,
-   default_ctx.c:47-51: res[0, zh-zl) = coefficients [zl, zh) of a*b mod n */
+   default_ctx.c:47-51: res[0, zh-zl) = coefficients [zl, zh) of a*b mod n.
+   Failure policy as mpn_mul_default_mpn_ctx. */
 FA_API void _nmod_poly_mul_mid_default_mpn_ctx(uint64_t* res, int64_t zl, int64_t zh,
                                         const uint64_t* a, int64_t an,
                                         const uint64_t* b, int64_t bn,
                                         fa_nmod_t mod);
+
+/* Failure policy of the two drop-ins above (no counterpart in the reference,
+   whose flint_throw aborts).  The caller registers the CPU implementations to
+   run when the device path fails -- in FLINT, its own functions under another
+   name -- so the output is still the reference's.  NULL: no fallback. */
+typedef void (*fa_mpn_mul_fn)(uint64_t* r1, const uint64_t* i1, int64_t n1,
+                              const uint64_t* i2, int64_t n2);
+typedef void (*fa_nmod_poly_mul_mid_fn)(uint64_t* res, int64_t zl, int64_t zh,
+                                        const uint64_t* a, int64_t an,
+                                        const uint64_t* b, int64_t bn, fa_nmod_t mod);
+FA_API void fa_set_cpu_fallback(fa_mpn_mul_fn mpn, fa_nmod_poly_mul_mid_fn nmod);
+/* status of the calling thread's last drop-in call: FA_OK, or the FA_E_*
+   code of the device failure (then the fallback ran, if one is registered) */
+FA_API int fa_last_status(void);
+
+/* ---------------------------------------------------------------------- */
+/* the mpn_ctx plugin surface, fft_small.h:382-433, 769-774, 850-860      */
+/* ---------------------------------------------------------------------- */
+
+/* A context: the prime set found from p (mpn_mul.c:1076-1091) plus the
+   execution state its calls use (a HIP stream and grow-only device buffers,
+   as the reference's context owns its scratch buffer, mpn_mul.c:1342-1362).
+   The layout differs from the reference's mpn_ctx_struct, whose fields are
+   host tables; callers treat it as opaque.  One context serves one thread at
+   a time, as in the reference. */
+#define FA_MPN_CTX_MAGIC 0x66616d70636f6e74ULL
+typedef struct {
+    uint64_t p;              /* the first prime */
+    uint64_t primes[8];      /* the eight FFT primes */
+    int32_t device;          /* HIP device of the state (-1: the current one) */
+    int32_t pad;
+    void* state;             /* library-internal; NULL for the thread default */
+    uint64_t magic;
+} fa_mpn_ctx_struct;
+typedef fa_mpn_ctx_struct fa_mpn_ctx_t[1];
+
+/* mpn_ctx_init, mpn_mul.c:1076.  This build's device tables carry the
+   prime set of p = 0x0003f00000000001 (default_ctx.c:17, the only p the
+   reference's sources pass); another p returns FA_E_ARG. */
+FA_API int fa_mpn_ctx_init(fa_mpn_ctx_struct* R, uint64_t p);
+/* mpn_ctx_clear, mpn_mul.c:1220 */
+FA_API void fa_mpn_ctx_clear(fa_mpn_ctx_struct* R);
+/* get_default_mpn_ctx, default_ctx.c:29-38: the calling thread's context */
+FA_API fa_mpn_ctx_struct* fa_get_default_mpn_ctx(void);
+/* mpn_ctx_mpn_mul, fft_small.h:433, mpn_mul.c:1980 */
+FA_API int fa_mpn_ctx_mpn_mul(fa_mpn_ctx_struct* R, uint64_t* z, const uint64_t* a,
+                              int64_t an, const uint64_t* b, int64_t bn);
+/* the same on device operands, enqueued on `stream` (NULL: the context's) */
+FA_API int fa_mpn_ctx_mpn_mul_device(fa_mpn_ctx_struct* R, uint64_t* z, const uint64_t* a,
+                                     int64_t an, const uint64_t* b, int64_t bn, void* stream);
+/* _nmod_poly_mul_mid_mpn_ctx, fft_small.h:769-774, nmod_poly_mul.c:904 */
+FA_API int fa_nmod_poly_mul_mid_mpn_ctx(uint64_t* z, int64_t zl, int64_t zh,
+                                        const uint64_t* a, int64_t an, const uint64_t* b,
+                                        int64_t bn, fa_nmod_t mod, fa_mpn_ctx_struct* R);
+/* _fmpz_poly_mul_mid_mpn_ctx, fft_small.h:856-860, fmpz_poly_mul.c:319;
+   coefficients as fa_fmpz_poly_mul_mid */
+FA_API int fa_fmpz_poly_mul_mid_mpn_ctx(uint64_t* z, uint64_t zw, int64_t zl, int64_t zh,
+                                        const uint64_t* a, uint64_t aw, int64_t an,
+                                        const uint64_t* b, uint64_t bw, int64_t bn,
+                                        fa_mpn_ctx_struct* R);
 
 /* ---------------------------------------------------------------------- */
 /* status-returning C ABI                                                 */
@@ -138,6 +200,26 @@ FA_API void fa_synth_limbs(uint64_t seed, uint64_t* a, uint64_t n, int top_nonze
 /* both operands of one product from one stream: a, then b, top limbs nonzero */
 FA_API void fa_synth_mpn_pair(uint64_t seed, uint64_t* a, uint64_t an, uint64_t* b, uint64_t bn);
 
+/* the request generator of the batched workloads (the same xoshiro256**
+   stream seeded by SplitMix64; every draw advances it) */
+typedef struct { uint64_t s[4]; } fa_rng;
+FA_API void fa_rng_seed(fa_rng* r, uint64_t seed);
+FA_API uint64_t fa_rng_next(fa_rng* r);
+/* uniform in [lo, hi], by rejection */
+FA_API uint64_t fa_rng_uniform(fa_rng* r, uint64_t lo, uint64_t hi);
+/* uniform in [0, 1): the top 53 bits of one draw */
+FA_API double fa_rng_double(fa_rng* r);
+/* floor(2^U) with U uniform in [x, y) */
+FA_API uint64_t fa_rng_log_uniform(fa_rng* r, double x, double y);
+/* a prime of `bits` bits: odd uniform draws until Miller-Rabin accepts */
+FA_API uint64_t fa_rng_random_prime(fa_rng* r, int bits);
+/* n values uniform in [0, bound) */
+FA_API void fa_rng_fill_below(fa_rng* r, uint64_t* out, uint64_t n, uint64_t bound);
+/* n limbs, the last redrawn until nonzero if top_nonzero */
+FA_API void fa_rng_fill_limbs(fa_rng* r, uint64_t* out, uint64_t n, int top_nonzero);
+/* n coefficients uniform in (-2^bits, 2^bits), w-limb two's complement */
+FA_API void fa_rng_fill_signed(fa_rng* r, uint64_t* out, uint64_t n, int bits, uint64_t w);
+
 /* ---------------------------------------------------------------------- */
 /* measurement support                                                    */
 /* ---------------------------------------------------------------------- */
@@ -182,9 +264,10 @@ FA_API void fa_set_crt_variant(int v);
 /* plain device-to-device copy kernel (nbytes multiple of 16), used to
    calibrate hardware byte counters */
 FA_API int fa_copy_device(void* dst, const void* src, uint64_t nbytes, void* stream);
-/* copy of a 2^d1 x 2^d2 double array with the column passes' access
-   pattern (tiles of 2^lgC columns, 16 rows per thread) and no arithmetic */
-FA_API int fa_probe_colcopy(const void* src, void* dst, int d1, int d2, int lgC, void* stream);
+/* cap on the device bytes the library's grow-only buffers may hold
+   together (0: no cap, the default).  A call that would grow past it fails
+   with FA_E_NOMEM, as when HBM runs out.  No counterpart in the reference. */
+FA_API void fa_set_device_memory_limit(uint64_t bytes);
 
 /* ------------------------------------------------------------------------
    Batches of heterogeneous products (no counterpart in the reference: the

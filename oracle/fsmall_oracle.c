@@ -1011,3 +1011,98 @@ void or_rng_limbs(or_rng* r, u64* a, u64 n, int top_nonzero)
     if (top_nonzero && n > 0)
         while (a[n - 1] == 0) a[n - 1] = or_rng_next(r);
 }
+
+/* ------------------------------------------------------------------ */
+/* request generator of the batched workloads, restated independently  */
+/* of the library's (tests compare the two draw for draw)              */
+/* ------------------------------------------------------------------ */
+
+double or_rng_double(or_rng* r)
+{
+    return ldexp((double)(or_rng_next(r) >> 11), -53);
+}
+
+u64 or_rng_log_uniform(or_rng* r, double x, double y)
+{
+    double d = or_rng_double(r);
+    return (u64)exp2(x + (y - x) * d);
+}
+
+void or_rng_fill_below(or_rng* r, u64* out, u64 n, u64 bound)
+{
+    for (u64 i = 0; i < n; i++) out[i] = bound ? or_rng_uniform(r, 0, bound - 1) : 0;
+}
+
+/* t uniform over the (bits+1)-bit values except the largest, c = t + 1 - 2^bits:
+   add one, then subtract 2^bits by complementing from bit `bits` upwards
+   with the borrow */
+void or_rng_fill_signed(or_rng* r, u64* out, u64 n, int bits, u64 w)
+{
+    u64 tw = (u64)(bits + 64) / 64;
+    int topb = bits + 1 - 64 * (int)(tw - 1);
+    u64 mask = (topb == 64) ? ~(u64)0 : (((u64)1 << topb) - 1);
+    for (u64 i = 0; i < n; i++) {
+        u64* c = out + i * w;
+        int again;
+        do {
+            again = 1;
+            for (u64 k = 0; k < tw; k++) {
+                c[k] = or_rng_next(r);
+                if (k + 1 == tw) c[k] &= mask;
+                if (c[k] != ((k + 1 == tw) ? mask : ~(u64)0)) again = 0;
+            }
+        } while (again);
+        for (u64 k = tw; k < w; k++) c[k] = 0;
+        /* t + 1 */
+        for (u64 k = 0; k < w; k++) if (++c[k] != 0) break;
+        /* - 2^bits */
+        u64 k0 = (u64)bits / 64;
+        u64 sub = (u64)1 << (bits % 64);
+        for (u64 k = k0; k < w; k++) {
+            u64 x = c[k];
+            c[k] = x - sub;
+            sub = (x < sub) ? 1 : 0;
+            if (!sub) break;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* checkers at full size: residues mod a word-size q                   */
+/* ------------------------------------------------------------------ */
+
+/* the integer with n little-endian limbs, mod q */
+u64 or_mpn_mod_q(const u64* a, u64 n, u64 q)
+{
+    u64 r = 0;
+    for (u64 i = n; i-- > 0;)
+        r = (u64)((((u128)r << 64) | a[i]) % q);
+    return r;
+}
+
+/* sum c_i x^i mod q over n coefficients (each reduced mod q first) */
+u64 or_nmod_poly_eval(const u64* c, u64 n, u64 x, u64 q)
+{
+    u64 r = 0;
+    x %= q;
+    for (u64 i = n; i-- > 0;) r = addm(mulm(r, x, q), c[i] % q, q);
+    return r;
+}
+
+/* sum c_i x^i mod q, c_i w-limb two's complement */
+u64 or_fmpz_poly_eval_mod_q(const u64* c, u64 w, u64 n, u64 x, u64 q)
+{
+    u64 r = 0;
+    x %= q;
+    for (u64 i = n; i-- > 0;) {
+        const u64* ci = c + i * w;
+        u64 v = or_mpn_mod_q(ci, w, q);
+        if ((s64)ci[w - 1] < 0) {
+            /* value = limbs - 2^(64 w) */
+            u64 t = powm(2, 64 * w, q);
+            v = subm(v, t, q);
+        }
+        r = addm(mulm(r, x, q), v, q);
+    }
+    return r;
+}

@@ -80,6 +80,15 @@ uint64_t or_rng_next(or_rng* r);
 uint64_t or_rng_uniform(or_rng* r, uint64_t lo, uint64_t hi);   /* [lo, hi] */
 uint64_t or_rng_random_prime(or_rng* r, uint64_t bits);
 void     or_rng_limbs(or_rng* r, uint64_t* a, uint64_t n, int top_nonzero);
+double   or_rng_double(or_rng* r);
+uint64_t or_rng_log_uniform(or_rng* r, double x, double y);
+void     or_rng_fill_below(or_rng* r, uint64_t* out, uint64_t n, uint64_t bound);
+void     or_rng_fill_signed(or_rng* r, uint64_t* out, uint64_t n, int bits, uint64_t w);
+
+/* full-size checkers: residues mod a word-size q */
+uint64_t or_mpn_mod_q(const uint64_t* a, uint64_t n, uint64_t q);
+uint64_t or_nmod_poly_eval(const uint64_t* c, uint64_t n, uint64_t x, uint64_t q);
+uint64_t or_fmpz_poly_eval_mod_q(const uint64_t* c, uint64_t w, uint64_t n, uint64_t x, uint64_t q);
 
 #ifdef __cplusplus
 }

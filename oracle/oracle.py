@@ -79,6 +79,26 @@ def lib():
         L.or_fmpz_poly_mul_mid.restype = ctypes.c_int
         L.or_fmpz_poly_nprimes.argtypes = [P64, u64, s64, P64, u64, s64]
         L.or_fmpz_poly_nprimes.restype = ctypes.c_int
+        L.or_rng_seed.argtypes = [ctypes.c_void_p, u64]
+        L.or_rng_next.argtypes = [ctypes.c_void_p]
+        L.or_rng_next.restype = u64
+        L.or_rng_uniform.argtypes = [ctypes.c_void_p, u64, u64]
+        L.or_rng_uniform.restype = u64
+        L.or_rng_random_prime.argtypes = [ctypes.c_void_p, u64]
+        L.or_rng_random_prime.restype = u64
+        L.or_rng_limbs.argtypes = [ctypes.c_void_p, P64, u64, ctypes.c_int]
+        L.or_rng_double.argtypes = [ctypes.c_void_p]
+        L.or_rng_double.restype = ctypes.c_double
+        L.or_rng_log_uniform.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.or_rng_log_uniform.restype = u64
+        L.or_rng_fill_below.argtypes = [ctypes.c_void_p, P64, u64, u64]
+        L.or_rng_fill_signed.argtypes = [ctypes.c_void_p, P64, u64, ctypes.c_int, u64]
+        L.or_mpn_mod_q.argtypes = [P64, u64, u64]
+        L.or_mpn_mod_q.restype = u64
+        L.or_nmod_poly_eval.argtypes = [P64, u64, u64, u64]
+        L.or_nmod_poly_eval.restype = u64
+        L.or_fmpz_poly_eval_mod_q.argtypes = [P64, u64, u64, u64, u64]
+        L.or_fmpz_poly_eval_mod_q.restype = u64
         _lib = L
     return _lib
 
@@ -177,3 +197,96 @@ def fmpz_poly_nprimes(a, aw, b, bw):
     a = _u64(a)
     b = _u64(b)
     return lib().or_fmpz_poly_nprimes(_ptr(a), aw, len(a) // aw, _ptr(b), bw, len(b) // bw)
+
+
+# ----------------------------------------------------------------------
+# full-size checkers: residues mod a word-size q
+# ----------------------------------------------------------------------
+
+def mpn_mod_q(a, q):
+    """the integer with little-endian limbs a, mod q"""
+    a = _u64(a)
+    return int(lib().or_mpn_mod_q(_ptr(a), len(a), q))
+
+
+def nmod_poly_eval(c, x, q):
+    """sum c_i x^i mod q"""
+    c = _u64(c)
+    return int(lib().or_nmod_poly_eval(_ptr(c), len(c), x, q))
+
+
+def fmpz_poly_eval_mod_q(c, w, x, q):
+    """sum c_i x^i mod q over w-limb two's complement coefficients"""
+    c = _u64(c)
+    return int(lib().or_fmpz_poly_eval_mod_q(_ptr(c), w, len(c) // w, x, q))
+
+
+# ----------------------------------------------------------------------
+# the batched workloads' request generator, restated (SplitMix64 seeding a
+# xoshiro256** stream, seed 0xF1A70000 + config index; draws in this order)
+# ----------------------------------------------------------------------
+
+class Rng:
+    def __init__(self, seed):
+        self.s = (ctypes.c_uint64 * 4)()
+        lib().or_rng_seed(self.s, seed)
+
+    def uniform(self, lo, hi):
+        return int(lib().or_rng_uniform(self.s, lo, hi))
+
+    def double(self):
+        return float(lib().or_rng_double(self.s))
+
+    def log_uniform(self, x, y):
+        return int(lib().or_rng_log_uniform(self.s, x, y))
+
+    def prime(self, bits):
+        return int(lib().or_rng_random_prime(self.s, bits))
+
+    def below(self, n, bound):
+        out = np.zeros(n, dtype=np.uint64)
+        lib().or_rng_fill_below(self.s, _ptr(out), n, bound)
+        return out
+
+    def limbs(self, n):
+        out = np.zeros(n, dtype=np.uint64)
+        lib().or_rng_limbs(self.s, _ptr(out), n, 1)
+        return out
+
+    def signed(self, n, bits, w):
+        out = np.zeros(n * w, dtype=np.uint64)
+        lib().or_rng_fill_signed(self.s, _ptr(out), n, bits, w)
+        return out
+
+
+def workload_requests(workload, count):
+    """the first `count` requests of a workload: tuples
+    ("nmod", a, b, n) / ("fmpz", a, b, w, bits) / ("mpn", a, b)"""
+    seed = {"nmod4096": 0xF1A70003, "fmpz256": 0xF1A70004, "stream": 0xF1A70005}[workload]
+    r = Rng(seed)
+
+    def nmod():
+        an, bn = r.log_uniform(6, 16), r.log_uniform(6, 16)
+        n = r.prime(r.uniform(20, 63))
+        return ("nmod", r.below(an, n), r.below(bn, n), n)
+
+    def fmpz():
+        an, bn = r.uniform(1024, 16384), r.uniform(1024, 16384)
+        bits = r.uniform(64, 4096)
+        w = bits // 64 + 1
+        return ("fmpz", r.signed(an, bits, w), r.signed(bn, bits, w), w, bits)
+
+    def mpn():
+        an, bn = r.log_uniform(4, 22), r.log_uniform(4, 22)
+        return ("mpn", r.limbs(an), r.limbs(bn))
+
+    out = []
+    for _ in range(count):
+        if workload == "nmod4096":
+            out.append(nmod())
+        elif workload == "fmpz256":
+            out.append(fmpz())
+        else:
+            u = r.double()
+            out.append(mpn() if u < 0.5 else nmod() if u < 0.8 else fmpz())
+    return out
