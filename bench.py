@@ -15,9 +15,12 @@ HIP events recorded around each launch on the launching stream.  The timed
 products run the primes in two groups on forked streams (--prime-groups);
 the per-kernel table is taken with one launch per pass over all primes.
 
-The CPU baseline is the oracle (oracle/fsmall_oracle.c, a scalar C
-restatement of the reference) timed on a bounded sample, scaled to the full
-workload by the n log n cost of the transform.
+The CPU baseline describes the host (CPU model, core count), times FLINT's
+own mpn_mul_default_mpn_ctx at 1 and nproc threads when a system libflint is
+present, and otherwise GMP's mpn_mul and the oracle (oracle/fsmall_oracle.c,
+a scalar C restatement of the reference) on the same inputs, each checked
+word for word against the GPU product; FLINT's own times from the survey
+host are carried beside them, labelled as another host.
 """
 import argparse
 import json
@@ -35,10 +38,12 @@ FP64_LANE_OPS_PEAK = 3.93e13  # 78.6 TFLOP/s FP64 vector = 3.93e13 lane-ops/s
 
 # reference-plan geometry (mpn_ctx_best_profile at 1 thread, mpn_mul.c:1248-1340)
 WORKLOADS = {
-    "mpn_2p24": dict(an=1 << 24, bn=1 << 24, seed=0xF1A70002, ref_np=6, ref_bits=136,
+    "mpn_2p24": dict(key="mpn_2p24", an=1 << 24, bn=1 << 24, seed=0xF1A70002, ref_np=6,
+                     ref_bits=136,
                      ref_T=15790336, ref_depth=24,
                      desc="plain mpn_mul, 2^24 x 2^24 limbs, one product per call"),
-    "mpn_65536": dict(an=65536, bn=65536, seed=0xF1A70001, ref_np=6, ref_bits=140,
+    "mpn_65536": dict(key="mpn_65536", an=65536, bn=65536, seed=0xF1A70001, ref_np=6,
+                      ref_bits=140,
                       ref_T=60160, ref_depth=16,
                       desc="plain mpn_mul, 65536 x 65536 limbs, one product per call"),
 }
@@ -82,30 +87,194 @@ def mod_q(limbs, q):
     return int.from_bytes(np.ascontiguousarray(limbs).tobytes(), "little") % q
 
 
-def cpu_baseline(W, sample_log2):
-    """oracle mpn_mul on a 2^s x 2^s sample, scaled to the workload"""
+# FP64 VALU lane-ops per transform point of each kernel of the two-level
+# pipeline (4.5 per point per radix-2 level: the radix-4 count of
+# sd_fft.c:331-355; 12 per point for the pointwise product with the
+# normalizer): the column passes do 12 levels of each operand (forward) or of
+# the product (inverse), the row pass 12 forward levels of both operands, the
+# pointwise product and 12 inverse levels
+KERNEL_FP64_OPS = {
+    "fa_k_col_fwd16w": 2 * 12 * 4.5,
+    "fa_k_col_fwd16": 2 * 12 * 4.5,
+    "fa_k_row_conv16": 2 * 12 * 4.5 + 12 + 12 * 4.5,
+    "fa_k_col_inv16w": 12 * 4.5,
+    "fa_k_col_inv16": 12 * 4.5,
+}
+
+# FLINT's own CPU path on the same configurations, measured in the survey
+# container (Intel Xeon, Sapphire Rapids flags, 8 vCPU; FLINT 3.7.0-dev,
+# GMP 6.2.1): not this host
+FLINT_SURVEY_HOST = {
+    "mpn_2p24": {"1_thread_s": 3.13, "8_threads_s": 0.694},
+    "mpn_65536": {"1_thread_s": 0.01010, "8_threads_s": 0.00210},
+}
+
+
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable}
+
+
+def find_libflint():
+    """a system libflint exporting mpn_mul_default_mpn_ctx, or None"""
+    import ctypes
+    import subprocess
+    try:
+        out = subprocess.run(["ldconfig", "-p"], capture_output=True, text=True,
+                             timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        return None
+    for line in out.splitlines():
+        if "libflint" in line and "=>" in line:
+            path = line.split("=>", 1)[1].strip()
+            try:
+                L = ctypes.CDLL(path)
+                if hasattr(L, "mpn_mul_default_mpn_ctx"):
+                    return path
+            except OSError:
+                continue
+    return None
+
+
+def _median_time(fn, reps):
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def cpu_baseline(W, a_h, b_h, z_gpu, budget_s=30.0):
+    """host description; FLINT's CPU path if a system
+    libflint is present (1 and nproc threads); otherwise GMP's mpn_mul and the
+    oracle (the scalar C restatement), labelled, on the same inputs; and
+    FLINT's own times from the survey host, labelled as another host"""
+    import ctypes
+    info = host_info()
+    an, bn = W["an"], W["bn"]
+    lp = an * bn
+    res = {"unit": "limb-products/s", "host": info}
+    legs = {}
+    flint = find_libflint()
+    res["system_libflint"] = flint
+    if flint:
+        L = ctypes.CDLL(flint)
+        L.mpn_mul_default_mpn_ctx.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
+                                              ctypes.c_void_p, ctypes.c_long]
+        L.flint_set_num_threads.argtypes = [ctypes.c_int]
+        z = np.empty(an + bn, dtype=np.uint64)
+        for th in (1, info["usable_cpus"]):
+            L.flint_set_num_threads(th)
+            L.mpn_mul_default_mpn_ctx(z.ctypes.data, a_h.ctypes.data, an, b_h.ctypes.data, bn)
+            t = _median_time(lambda: L.mpn_mul_default_mpn_ctx(
+                z.ctypes.data, a_h.ctypes.data, an, b_h.ctypes.data, bn), 5)
+            legs["flint_%d_threads" % th] = {"s": t, "value": lp / t, "threads": th,
+                                             "bit_exact_vs_gpu": bool(np.array_equal(z, z_gpu))}
+    # GMP mpn_mul (libgmp.so.10), the oracle of the reference's t-mul.c:54
+    try:
+        G = ctypes.CDLL("libgmp.so.10")
+        G.__gmpn_mul.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
+                                 ctypes.c_void_p, ctypes.c_long]
+        z = np.empty(an + bn, dtype=np.uint64)
+        x, y = (a_h, b_h) if an >= bn else (b_h, a_h)
+        t0 = time.perf_counter()
+        G.__gmpn_mul(z.ctypes.data, x.ctypes.data, len(x), y.ctypes.data, len(y))
+        t = time.perf_counter() - t0
+        legs["gmp_mpn_mul_1_thread"] = {"s": t, "value": lp / t, "threads": 1,
+                                        "bit_exact_vs_gpu": bool(np.array_equal(z, z_gpu)),
+                                        "library": "libgmp.so.10 __gmpn_mul, full product"}
+    except OSError as e:
+        legs["gmp_mpn_mul_1_thread"] = {"unavailable": str(e)}
+    # the oracle, full product when it fits the budget, else a sample scaled
+    # by n log n
     from oracle import oracle as O
-    n = 1 << sample_log2
+    n = 1 << 16
     rng = np.random.default_rng(12345)
-    a = rng.integers(0, 2**64, n, dtype=np.uint64)
-    b = rng.integers(0, 2**64, n, dtype=np.uint64)
-    a[-1] |= np.uint64(1)
-    b[-1] |= np.uint64(1)
+    sa = rng.integers(0, 2**64, n, dtype=np.uint64)
+    sb = rng.integers(0, 2**64, n, dtype=np.uint64)
+    sa[-1] |= np.uint64(1)
+    sb[-1] |= np.uint64(1)
     t0 = time.perf_counter()
-    O.mpn_mul(a, b)
-    dt = time.perf_counter() - t0
-    N = W["an"]
+    O.mpn_mul(sa, sb)
+    t_small = time.perf_counter() - t0
+    N = max(an, bn)
     scale = (N * np.log2(N)) / (n * np.log2(n))
-    t_full = dt * scale
-    return {
-        "value": W["an"] * W["bn"] / t_full,
-        "unit": "limb-products/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": "oracle mpn_mul of 2^%d x 2^%d limbs took %.2f s on 1 thread; scaled to "
-                  "2^%d x 2^%d by n log n (x%.2f)" % (sample_log2, sample_log2, dt,
-                                                      int(np.log2(N)), int(np.log2(N)), scale),
-    }
+    if t_small * scale <= budget_s:
+        t0 = time.perf_counter()
+        zo = O.mpn_mul(a_h, b_h)
+        t_or = time.perf_counter() - t0
+        legs["oracle_1_thread"] = {"s": t_or, "value": lp / t_or, "threads": 1,
+                                   "bit_exact_vs_gpu": bool(np.array_equal(zo, z_gpu)),
+                                   "sample": "full product"}
+    else:
+        t_or = t_small * scale
+        legs["oracle_1_thread"] = {"s": t_or, "value": lp / t_or, "threads": 1,
+                                   "sample": "2^16 x 2^16 took %.3f s, scaled by n log n (x%.1f)"
+                                             % (t_small, scale)}
+    ref = FLINT_SURVEY_HOST.get(W.get("key"))
+    if ref:
+        res["flint_same_code_different_host"] = {
+            k: {"s": v, "value": lp / v} for k, v in ref.items()}
+        res["flint_same_code_different_host"]["host"] = (
+            "survey container: Intel Xeon (Sapphire Rapids flags), 8 vCPU, FLINT 3.7.0-dev, "
+            "GMP 6.2.1")
+    res["legs"] = legs
+    if flint:
+        best = max((v for k, v in legs.items() if k.startswith("flint")), key=lambda v: v["value"])
+        res.update(value=best["value"], cores=best["threads"], kind="reference",
+                   sample="system libflint mpn_mul_default_mpn_ctx, full product, median of 5")
+    else:
+        o = legs["oracle_1_thread"]
+        res.update(value=o["value"], cores=1, kind="port",
+                   sample="the oracle (scalar C restatement of the reference), %s, 1 thread; "
+                          "no system libflint on this host; GMP mpn_mul beside it in legs"
+                          % o.get("sample", ""))
+    return res
+
+
+def _k_passes(depth):
+    """array passes of the byte model at a transform depth (0: one LDS-resident
+    kernel; 10 for a two-level transform)"""
+    return 0 if depth <= 12 else 6 * ((depth + 11) // 12) - 2
+
+
+def _mpn_model_bytes(an, bn, np_, bits, depth):
+    T = -(-64 * an // bits) + -(-64 * bn // bits) - 1
+    return 16 * (an + bn) + 8 * np_ * T * _k_passes(depth)
+
+
+def request_model_bytes(q, r, fa):
+    """algorithmic HBM bytes of one request at the plan the batch chose
+    (np, bits, depth written back into the request r)"""
+    if q["kind"] == "mpn":
+        return _mpn_model_bytes(len(q["a"]), len(q["b"]), r.np, r.bits, r.depth)
+    aw, bw = q.get("aw", 1), q.get("bw", 1)
+    an, bn = len(q["a"]) // aw, len(q["b"]) // bw
+    zn = an + bn - 1
+    zw = q.get("zw", 1)
+    io = 8 * (an * aw + bn * bw + (q["zh"] - q["zl"]) * zw)
+    if r.np > 0:
+        return io + 8 * r.np * zn * _k_passes(r.depth)
+    # fmpz beyond eight primes: one integer product of the signed-slot
+    # packed operands (packed limbs written, read by the transform, product
+    # limbs read back by the unpack)
+    sb = abs(q["abits"]) + abs(q["bbits"]) + int(min(an, bn)).bit_length() + 1
+    la, lb = -(-an * sb // 64), -(-bn * sb // 64)
+    p = fa.mpn_plan(la, lb)
+    return io + 16 * (la + lb) + _mpn_model_bytes(la, lb, p["np"], p["bits"], p["depth"])
 
 
 def batched_main(args, torch, fa):
@@ -167,8 +336,11 @@ def batched_main(args, torch, fa):
     dev = torch.device("cuda:0")
     ms_dev = 0.0
     dev_exact = True
+    model_bytes = 0
     for ch in chunks:
         B, outs, keep = WL.device_batch([reqs_p[k] for k in ch], torch, dev)
+        R = keep[-1]
+        model_bytes += sum(request_model_bytes(reqs_p[k], R[j], fa) for j, k in enumerate(ch))
         ms_dev += WL.time_device(B, torch, args.steps, args.warmup)
         for k, tz in zip(ch, outs):
             if plain_out[k] is not None:
@@ -182,10 +354,12 @@ def batched_main(args, torch, fa):
 
     cpu = None
     if not args.no_cpu_baseline:
+        from concurrent.futures import ThreadPoolExecutor
         from oracle import oracle as O
-        tc0 = time.perf_counter()
-        lp_s, n_s = 0, 0
-        for k in done:
+        info = host_info()
+        nth = min(info["usable_cpus"] or 1, 16)
+
+        def cpu_one(k):
             q = reqs[k]
             if q["kind"] == "mpn":
                 O.mpn_mul(q["a"], q["b"])
@@ -195,15 +369,26 @@ def batched_main(args, torch, fa):
                 ok, _ = O.fmpz_poly_mul_mid(q["a"], q["aw"], q["b"], q["bw"], q["zl"], q["zh"],
                                             q["zw"])
                 if not ok:      # the oracle has no Kronecker route: not part of the sample
-                    continue
-            lp_s += q["lp"]
-            n_s += 1
-            if time.perf_counter() - tc0 > 15.0:
-                break
+                    return 0
+            return q["lp"]
+
+        # one request per thread over the usable cores (the oracle releases the
+        # GIL), in waves until about 15 s have passed
+        tc0 = time.perf_counter()
+        lp_s, n_s, pos = 0, 0, 0
+        with ThreadPoolExecutor(nth) as ex:
+            while pos < len(done) and time.perf_counter() - tc0 < 15.0:
+                wave = done[pos:pos + 4 * nth]
+                pos += len(wave)
+                got = list(ex.map(cpu_one, wave))
+                lp_s += sum(got)
+                n_s += sum(1 for g in got if g)
         tc = time.perf_counter() - tc0
-        cpu = {"value": lp_s / tc, "unit": "limb-products/s", "cores": 1, "kind": "port",
+        cpu = {"value": lp_s / tc, "unit": "limb-products/s", "cores": nth, "kind": "port",
+               "host": info, "system_libflint": find_libflint(),
                "sample": "oracle (scalar C restatement) on the first %d computed requests of the "
-                         "same stream, %.1f s on 1 thread" % (n_s, tc)}
+                         "same stream, one request per thread over %d threads, %.1f s"
+                         % (n_s, nth, tc)}
 
     value = lp_done / (ms_dev * 1e-3)
     out = {
@@ -227,7 +412,18 @@ def batched_main(args, torch, fa):
             "speedup_batched_host_vs_plain": t_plain / t_host,
             "bit_exact_vs_one_call_per_product": bool(exact and dev_exact),
         },
-        "roofline": None,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "whole batch: algorithmic bytes of every request at the plan it ran "
+                      "(8 B per transform point per array pass, 10 passes above depth 12, "
+                      "operands and outputs once)",
+            "achieved": model_bytes / (ms_dev * 1e-3) / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": model_bytes / (ms_dev * 1e-3) / HBM_PEAK,
+            "traffic": None,
+            "model_bytes": model_bytes,
+        },
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -245,7 +441,6 @@ def main():
     ap.add_argument("--no-planner", action="store_true",
                     help="batched workloads: automatic plans instead of the learned planner")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-log2", type=int, default=24)
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--prime-groups", type=int, default=2,
                     help="depth-24 products: primes in this many groups on forked streams "
@@ -346,8 +541,8 @@ def main():
         traffic = tj.get("kernels", {}).get(dominant, {}).get("hbm_bytes_per_launch")
 
     checked = None
+    zh = z.cpu().numpy().view(np.uint64)
     if not args.no_check:
-        zh = z.cpu().numpy().view(np.uint64)
         ok = True
         for q in (2305843009213693951, 2305843009213693921, 2305843009213693907):
             ok &= (mod_q(a_h, q) * mod_q(b_h, q)) % q == mod_q(zh, q)
@@ -359,9 +554,11 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(W, args.cpu_sample_log2)
+            cpu = cpu_baseline(W, a_h, b_h, zh)
         except Exception as e:  # oracle missing: report null
             print("# cpu baseline unavailable: %s" % e, file=sys.stderr)
+    fops = KERNEL_FP64_OPS.get(dominant)
+    fp64_achieved = fops * W["ref_np"] * W["ref_T"] / (dk["avg_ms"] * 1e-3) if fops else None
 
     value = an * bn / (ms * 1e-3)
     out = {
@@ -396,6 +593,9 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
+            "fp64_lane_ops_per_s": fp64_achieved,
+            "fp64_peak_lane_ops_per_s": FP64_LANE_OPS_PEAK,
+            "fp64_frac": fp64_achieved / FP64_LANE_OPS_PEAK if fp64_achieved else None,
         },
         "cpu_baseline": cpu,
     }

@@ -60,6 +60,22 @@ class MpnCtxStruct(ctypes.Structure):
                 ("pad", ctypes.c_int32), ("state", _VP), ("magic", _u64)]
 
 
+class PlanStruct(ctypes.Structure):
+    """fa_plan_struct (include/flint_amd.h)."""
+    _fields_ = [("kind", ctypes.c_int32), ("np", ctypes.c_int32), ("offset", ctypes.c_int32),
+                ("bits", ctypes.c_int32), ("depth", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("stride", _u64), ("zl", _u64), ("zh", _u64), ("zn", _u64),
+                ("an_max", _u64), ("bn_max", _u64), ("len_bound", _u64), ("mod", NmodT)]
+
+
+class OpStruct(ctypes.Structure):
+    """fa_op_struct (include/flint_amd.h)."""
+    _fields_ = [("data", _VP), ("stride", _u64), ("np", _u64), ("offset", _u64),
+                ("depth", _u64), ("domain", ctypes.c_int32), ("owns_data", ctypes.c_int32)]
+
+
+OP_PRIMAL, OP_PRODUCT, OP_COEFF = 0, 1, 2
+
 # C types of the drop-ins' CPU fallbacks (fa_set_cpu_fallback)
 MPN_MUL_FN = ctypes.CFUNCTYPE(None, _P64, _P64, _s64, _P64, _s64)
 NMOD_MUL_MID_FN = ctypes.CFUNCTYPE(None, _P64, _s64, _s64, _P64, _s64, _P64, _s64, NmodT)
@@ -70,6 +86,10 @@ SIGNATURES = {
     "_nmod_poly_mul_mid_default_mpn_ctx": (None, [_P64, _s64, _s64, _P64, _s64, _P64, _s64, NmodT]),
     "fa_mpn_mul": (ctypes.c_int, [_P64, _P64, _s64, _P64, _s64]),
     "fa_mpn_mul_device": (ctypes.c_int, [_VP, _VP, _s64, _VP, _s64, _VP]),
+    "fa_mpn_mul_range": (ctypes.c_int, [_P64, _u64, _u64, _P64, _s64, _P64, _s64]),
+    "fa_mpn_mulmod_2expp1": (ctypes.c_int, [_P64, _P64, _P64, _s64]),
+    "fa_mpn_mulmod_2expp1_device": (ctypes.c_int, [_VP, _VP, _VP, _s64, _VP]),
+    "fa_mpn_mul_range_device": (ctypes.c_int, [_VP, _u64, _u64, _VP, _s64, _VP, _s64, _VP]),
     "fa_mpn_plan": (ctypes.c_int, [_s64, _s64, _P64]),
     "fa_mpn_mul_plan": (ctypes.c_int, [_P64, _P64, _s64, _P64, _s64, ctypes.c_int, ctypes.c_int]),
     "fa_mpn_mul_device_plan": (ctypes.c_int, [_VP, _VP, _s64, _VP, _s64, ctypes.c_int,
@@ -110,6 +130,27 @@ SIGNATURES = {
     "fa_set_ks_variant": (None, [ctypes.c_int]),
     "fa_set_prime_groups": (None, [ctypes.c_int]),
     "fa_set_crt_variant": (None, [ctypes.c_int]),
+    "fa_set_square_route": (None, [ctypes.c_int]),
+    "fa_plan_init_mpn": (ctypes.c_int, [_VP, _u64, _u64, _u64, ctypes.c_int]),
+    "fa_plan_init_nmod": (ctypes.c_int, [_VP, _u64, _u64, _u64, _u64, _u64, _u64, NmodT]),
+    "fa_plan_clear": (None, [_VP]),
+    "fa_op_init": (ctypes.c_int, [_VP, _VP]),
+    "fa_op_init_borrowed": (ctypes.c_int, [_VP, _VP, _VP]),
+    "fa_op_sizeof_data": (_u64, [_VP]),
+    "fa_op_clear": (None, [_VP]),
+    "fa_fft_mpn": (ctypes.c_int, [_VP, _VP, _s64, _VP, _VP]),
+    "fa_fft_nmod": (ctypes.c_int, [_VP, _VP, _s64, NmodT, _VP, _VP]),
+    "fa_op_mul": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "fa_op_sqr": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+    "fa_op_addmul": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "fa_op_submul": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "fa_op_add": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "fa_op_sub": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "fa_op_neg": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+    "fa_ifft": (ctypes.c_int, [_VP, _VP, _VP]),
+    "fa_export_mpn": (ctypes.c_int, [_VP, _s64, _VP, _VP, _VP]),
+    "fa_export_nmod": (ctypes.c_int, [_VP, _VP, NmodT, _VP, _VP]),
+    "fa_export_nmod_range": (ctypes.c_int, [_VP, _VP, _u64, _u64, NmodT, _VP, _VP]),
     "fa_set_device_memory_limit": (None, [_u64]),
     "fa_set_cpu_fallback": (None, [_VP, _VP]),
     "fa_last_status": (ctypes.c_int, []),
@@ -258,6 +299,11 @@ def set_col_variant(v):
     lib().fa_set_col_variant(int(v))
 
 
+def set_square_route(on=True):
+    """squares: one forward transform per prime (default) or the general product"""
+    lib().fa_set_square_route(1 if on else 0)
+
+
 def set_crt_variant(v):
     """CRT of integer products: 1 = FP64 radix-2^24 digits, 0 = 64-bit limbs"""
     lib().fa_set_crt_variant(int(v))
@@ -279,6 +325,28 @@ def mpn_mul(a, b):
     b = _arr(b)
     z = np.empty(len(a) + len(b), dtype=np.uint64)
     _check(lib().fa_mpn_mul(_p(z), _p(a), len(a), _p(b), len(b)))
+    return z
+
+
+def mpn_mul_range(a, b, lo, hi):
+    """limbs [lo, hi) of a*b (_mpn_ctx_mpn_mul_range, flint_mpn_mulmid_fft_small)"""
+    a = _arr(a)
+    b = _arr(b)
+    z = np.empty(hi - lo, dtype=np.uint64)
+    _check(lib().fa_mpn_mul_range(_p(z), lo, hi, _p(a), len(a), _p(b), len(b)))
+    return z
+
+
+def mpn_mul_range_device(z_ptr, lo, hi, a_ptr, an, b_ptr, bn, stream=None):
+    return _check(lib().fa_mpn_mul_range_device(z_ptr, lo, hi, a_ptr, an, b_ptr, bn, stream))
+
+
+def mpn_mulmod_2expp1(x, y, n):
+    """x*y mod 2^(64 n) + 1 for (n+1)-limb x, y <= 2^(64 n) (sd_fft_mpn_mulmod_2expp1)"""
+    x = _arr(x)
+    y = x if y is x else _arr(y)
+    z = np.empty(n + 1, dtype=np.uint64)
+    _check(lib().fa_mpn_mulmod_2expp1(_p(z), _p(x), _p(y), n))
     return z
 
 
@@ -369,6 +437,92 @@ class MpnCtx:
         rc = _check(lib().fa_fmpz_poly_mul_mid_mpn_ctx(_p(z), zw, zl, zh, _p(a), aw, len(a) // aw,
                                                        _p(b), bw, len(b) // bw, self.ptr))
         return z[: (zh - zl) * zw] if rc == 1 else None
+
+
+class Plan:
+    """fft_small_plan_t on the device path: Plan.mpn(an_max, bn_max, len_bound)
+    or Plan.nmod(zl, zh, zn, xtrunc_max, len_bound, prod_bits, n)."""
+
+    def __init__(self):
+        self.s = PlanStruct()
+        self.ptr = ctypes.addressof(self.s)
+
+    @classmethod
+    def mpn(cls, an_max, bn_max, len_bound=1):
+        P = cls()
+        _check(lib().fa_plan_init_mpn(P.ptr, an_max, bn_max, len_bound, 0))
+        return P
+
+    @classmethod
+    def nmod(cls, zl, zh, zn, xtrunc_max, len_bound, prod_bits, n):
+        P = cls()
+        _check(lib().fa_plan_init_nmod(P.ptr, zl, zh, zn, xtrunc_max, len_bound, prod_bits,
+                                       nmod_init(n)))
+        return P
+
+
+class Op:
+    """fft_small_op_t: a transformed operand in device memory."""
+
+    def __init__(self, plan):
+        self.s = OpStruct()
+        self.ptr = ctypes.addressof(self.s)
+        self.plan = plan
+        _check(lib().fa_op_init(self.ptr, plan.ptr))
+
+    def close(self):
+        if self.s.data:
+            lib().fa_op_clear(self.ptr)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def domain(self):
+        return self.s.domain
+
+    def fft_mpn(self, a_ptr, an, stream=None):
+        return _check(lib().fa_fft_mpn(self.ptr, a_ptr, an, self.plan.ptr, stream))
+
+    def fft_nmod(self, a_ptr, an, n, stream=None):
+        return _check(lib().fa_fft_nmod(self.ptr, a_ptr, an, nmod_init(n), self.plan.ptr, stream))
+
+    def mul(self, A, B, stream=None):
+        return _check(lib().fa_op_mul(self.ptr, A.ptr, B.ptr, self.plan.ptr, stream))
+
+    def sqr(self, A, stream=None):
+        return _check(lib().fa_op_sqr(self.ptr, A.ptr, self.plan.ptr, stream))
+
+    def addmul(self, A, B, stream=None):
+        return _check(lib().fa_op_addmul(self.ptr, A.ptr, B.ptr, self.plan.ptr, stream))
+
+    def submul(self, A, B, stream=None):
+        return _check(lib().fa_op_submul(self.ptr, A.ptr, B.ptr, self.plan.ptr, stream))
+
+    def add(self, A, B, stream=None):
+        return _check(lib().fa_op_add(self.ptr, A.ptr, B.ptr, self.plan.ptr, stream))
+
+    def sub(self, A, B, stream=None):
+        return _check(lib().fa_op_sub(self.ptr, A.ptr, B.ptr, self.plan.ptr, stream))
+
+    def neg(self, A, stream=None):
+        return _check(lib().fa_op_neg(self.ptr, A.ptr, self.plan.ptr, stream))
+
+    def ifft(self, stream=None):
+        return _check(lib().fa_ifft(self.ptr, self.plan.ptr, stream))
+
+    def export_mpn(self, z_ptr, zn, stream=None):
+        return _check(lib().fa_export_mpn(z_ptr, zn, self.ptr, self.plan.ptr, stream))
+
+    def export_nmod(self, z_ptr, n, zl=None, zh=None, stream=None):
+        if zl is None:
+            return _check(lib().fa_export_nmod(z_ptr, self.ptr, nmod_init(n), self.plan.ptr,
+                                               stream))
+        return _check(lib().fa_export_nmod_range(z_ptr, self.ptr, zl, zh, nmod_init(n),
+                                                 self.plan.ptr, stream))
 
 
 def nmod_init(n):

@@ -286,8 +286,9 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
             p.zn = (uint64_t)(q.an + q.bn - 1);
             p.cnt = (uint64_t)(q.zh - q.zl);
             if (p.cnt == 0) { p.live = 0; continue; }
-            p.depth = clog2_u64(p.zn);
-            if (p.depth < 4) p.depth = 4;
+            p.depth = fa_poly_window_depth((uint64_t)q.zl,
+                                           (uint64_t)q.zh < p.zn ? (uint64_t)q.zh : p.zn, p.zn,
+                                           (uint64_t)(q.an > q.bn ? q.an : q.bn));
             if (q.kind == FA_REQ_NMOD) {
                 if (q.mod.n == 0) { q.status = FA_E_ARG; continue; }
                 p.kind = FA_PACK_U64;

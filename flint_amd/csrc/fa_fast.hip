@@ -304,7 +304,10 @@ FA_DEV void fa_row_conv_tile(double* __restrict__ arow, const double* __restrict
     for (int k = 0; k < 16; k++) arow[t + 256 * k] = x[k];
 }
 
-// fa_row_conv_tile with the last-stage twiddles formed on the fly
+// fa_row_conv_tile with the last-stage twiddles formed on the fly.
+// SQR: a square (brow unused): one forward transform, pointwise x^2
+// (mpn_mul.c:1425-1452, sd_fft_ctx_point_sqr)
+template <bool SQR = false>
 FA_DEV void fa_row_conv_tile_lr(double* __restrict__ arow, const double* __restrict__ brow,
                                 uint64_t r, double n, double ninv, double m,
                                 const double* __restrict__ w2, const double* __restrict__ w2inv,
@@ -315,7 +318,7 @@ FA_DEV void fa_row_conv_tile_lr(double* __restrict__ arow, const double* __restr
     double x[16], y[16];
     fa_build_row_table(tab, w2, r, rho, n, ninv);
 #pragma unroll
-    for (int z = 0; z < 2; z++) {
+    for (int z = SQR ? 1 : 0; z < 2; z++) {
         double* v = z ? x : y;
         const double* src = z ? arow : brow;
         __builtin_amdgcn_sched_barrier(0);
@@ -342,7 +345,8 @@ FA_DEV void fa_row_conv_tile_lr(double* __restrict__ arow, const double* __restr
         __syncthreads();
     }
 #pragma unroll
-    for (int k = 0; k < 16; k++) x[k] = fa_mulmod(fa_mulmod(x[k], m, n, ninv), y[k], n, ninv);
+    for (int k = 0; k < 16; k++)
+        x[k] = fa_mulmod(fa_mulmod(x[k], m, n, ninv), SQR ? x[k] : y[k], n, ninv);
     fa_build_row_table(tab, w2inv, r, rho, n, ninv);
     __builtin_amdgcn_sched_barrier(0);
     fa_inv16_rowlast(x, rho, w2inv, t, n, ninv);
@@ -363,6 +367,7 @@ FA_DEV void fa_row_conv_tile_lr(double* __restrict__ arow, const double* __restr
     for (int k = 0; k < 16; k++) arow[t + 256 * k] = x[k];
 }
 
+template <bool SQR>
 __global__ void __launch_bounds__(256, 3) fa_k_row_conv16lr(double* __restrict__ A,
                                                              const double* __restrict__ B,
                                                              uint64_t stride, fa_prime_set S)
@@ -372,9 +377,88 @@ __global__ void __launch_bounds__(256, 3) fa_k_row_conv16lr(double* __restrict__
     __shared__ double rho[16];
     const int i = blockIdx.y;
     const uint64_t r = blockIdx.x;
-    fa_row_conv_tile_lr(A + (uint64_t)i * stride + r * FA_ROW_PITCH,
+    fa_row_conv_tile_lr<SQR>(A + (uint64_t)i * stride + r * FA_ROW_PITCH,
                         B + (uint64_t)i * stride + r * FA_ROW_PITCH,
                         r, S.p[i], S.pinv[i], S.norm[i], S.w2[i], S.w2inv[i], xch, tab, rho);
+}
+
+// Row passes of the separate forward and inverse transforms (the
+// transformed-operand interface): the last 12 forward levels of one row,
+// stored back in place with point 16t + k of the transformed row at
+// position 16t + k; and its inverse from that layout.
+__global__ void __launch_bounds__(256) fa_k_row_fwd16(double* __restrict__ A, uint64_t stride,
+                                                      fa_prime_set S)
+{
+    __shared__ double xch[4352];
+    __shared__ double tab[256];
+    __shared__ double rho[16];
+    const int i = blockIdx.y;
+    const uint64_t r = blockIdx.x;
+    const double n = S.p[i], ninv = S.pinv[i];
+    double* row = A + (uint64_t)i * stride + r * FA_ROW_PITCH;
+    const int t = threadIdx.x;
+    const int b = t >> 4, i0 = t & 15;
+    double v[16];
+    fa_build_row_table(tab, S.w2[i], r, rho, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = row[t + 256 * k];
+    fa_fwd16_rowtab(v, tab, 0, 0, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 16; k++) xch[PADX(t + 256 * k)] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = xch[PADX(256 * b + i0 + 16 * k)];
+    fa_fwd16_rowtab(v, tab, 1, b, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 16; k++) xch[PADX(256 * b + i0 + 16 * k)] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = xch[PADX(16 * t + k)];
+    fa_fwd16_rowlast(v, rho, S.w2[i], t, n, ninv);
+    __syncthreads();
+    // through LDS so that the stores are coalesced
+#pragma unroll
+    for (int k = 0; k < 16; k++) xch[PADX(16 * t + k)] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) row[t + 256 * k] = xch[PADX(t + 256 * k)];
+}
+
+__global__ void __launch_bounds__(256) fa_k_row_inv16(double* __restrict__ A, uint64_t stride,
+                                                      fa_prime_set S)
+{
+    __shared__ double xch[4352];
+    __shared__ double tab[256];
+    __shared__ double rho[16];
+    const int i = blockIdx.y;
+    const uint64_t r = blockIdx.x;
+    const double n = S.p[i], ninv = S.pinv[i];
+    double* row = A + (uint64_t)i * stride + r * FA_ROW_PITCH;
+    const int t = threadIdx.x;
+    const int b = t >> 4, i0 = t & 15;
+    double x[16];
+    fa_build_row_table(tab, S.w2inv[i], r, rho, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 16; k++) xch[PADX(t + 256 * k)] = row[t + 256 * k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = xch[PADX(16 * t + k)];
+    fa_inv16_rowlast(x, rho, S.w2inv[i], t, n, ninv);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) xch[PADX(16 * t + k)] = x[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = xch[PADX(256 * b + i0 + 16 * k)];
+    fa_inv16_rowtab(x, tab, 1, b, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 16; k++) xch[PADX(256 * b + i0 + 16 * k)] = x[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = xch[PADX(t + 256 * k)];
+    fa_inv16_rowtab(x, tab, 0, 0, n, ninv);
+#pragma unroll
+    for (int k = 0; k < 16; k++) row[t + 256 * k] = x[k];
 }
 
 // blockIdx.x = row, blockIdx.y = prime
@@ -1197,10 +1281,11 @@ extern "C" FA_API void fa_set_prime_groups(int v) { fa_pipe_groups = v; }
 // of later groups (arithmetic-bound) run beside the row and inverse column
 // passes of earlier ones (the latter bound by HBM).  Same kernels, same
 // results as one launch per pass over all primes.
+// nops = 1: a square (only a is transformed; the row pass squares)
 template <int D1, int KIND>
-static int launch_fast_pipe(const fa_operand& oa, const fa_operand& ob, int np, int ng, double* A,
-                            double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s,
-                            fa_aux* X)
+static int launch_fast_pipe(const fa_operand& oa, const fa_operand& ob, int nops, int np, int ng,
+                            double* A, double* B, uint64_t stride, const fa_prime_set& S,
+                            hipStream_t s, fa_aux* X)
 {
     const int d2 = FA_ROW_LG;
     const int ngroups = (1 << d2) / 8;
@@ -1214,13 +1299,17 @@ static int launch_fast_pipe(const fa_operand& oa, const fa_operand& ob, int np, 
         if (g && (hipStreamWaitEvent(t, X->fork, 0) != hipSuccess ||
                   hipStreamWaitEvent(t, X->fwd[g - 1], 0) != hipSuccess))
             return fa_set_error(FA_E_HIP, "pipeline fork");
-        const unsigned gf = ((unsigned)(m * ngroups * 2) + 7) & ~7u;
+        const unsigned gf = ((unsigned)(m * ngroups * nops) + 7) & ~7u;
         FA_LAUNCH("fa_k_col_fwd16w", (fa_k_col_fwd16w<8, KIND>), dim3(gf), dim3(1024), 0, t, oa,
-                  ob, d2, Ag, Bg, stride, Sg, m, ngroups, 2);
+                  ob, d2, Ag, Bg, stride, Sg, m, ngroups, nops);
         if (hipEventRecord(X->fwd[g], t) != hipSuccess)
             return fa_set_error(FA_E_HIP, "pipeline event");
-        FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16lr, dim3(1u << D1, m), dim3(256), 0, t, Ag, Bg,
-                  stride, Sg);
+        if (nops == 1)
+            FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16lr<true>, dim3(1u << D1, m), dim3(256), 0,
+                      t, Ag, Ag, stride, Sg);
+        else
+            FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16lr<false>, dim3(1u << D1, m), dim3(256), 0,
+                      t, Ag, Bg, stride, Sg);
         const unsigned gi = ((unsigned)(m * ngroups) + 7) & ~7u;
         FA_LAUNCH("fa_k_col_inv16w", fa_k_col_inv16w<8>, dim3(gi), dim3(1024), 0, t, Ag, d2, stride,
                   Sg, m, ngroups);
@@ -1238,14 +1327,15 @@ static int launch_fast(const fa_operand& oa, const fa_operand& ob, int nops, int
                        double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s)
 {
     if constexpr (D1 == 12) {
-        if (fa_pipe_groups >= 2 && nops == 2 && np >= 2 && fa_col_variant == 1 && fa_row_variant == 1 &&
+        if (fa_pipe_groups >= 2 && np >= 2 && fa_col_variant == 1 && fa_row_variant == 1 &&
             !fa_prof_on()) {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
             if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
                 fa_aux* X = aux_streams();
                 const int ng = fa_pipe_groups < np ? fa_pipe_groups : np;
                 if (X && ng >= 2 && ng <= FA_PIPE_MAX)
-                    return launch_fast_pipe<D1, KIND>(oa, ob, np, ng, A, B, stride, S, s, X);
+                    return launch_fast_pipe<D1, KIND>(oa, ob, nops, np, ng, A, B, stride, S, s,
+                                                      X);
             }
         }
     }
@@ -1265,12 +1355,15 @@ static int launch_fast(const fa_operand& oa, const fa_operand& ob, int nops, int
     else
         FA_LAUNCH("fa_k_col_fwd16", (fa_k_col_fwd16<D1, KIND>), dim3(grid), dim3(1024), 0, s, oa, ob,
                   d2, A, B, stride, S, np, ngroups, nops);
-    if (fa_row_variant == 1)
-        FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16lr, dim3(1u << D1, np), dim3(256), 0, s, A, B,
-                  stride, S);
+    if (fa_row_variant == 1 && nops == 1)
+        FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16lr<true>, dim3(1u << D1, np), dim3(256), 0, s,
+                  A, A, stride, S);
+    else if (fa_row_variant == 1)
+        FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16lr<false>, dim3(1u << D1, np), dim3(256), 0, s,
+                  A, B, stride, S);
     else
-        FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16, dim3(1u << D1, np), dim3(256), 0, s, A, B,
-                  stride, S);
+        FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16, dim3(1u << D1, np), dim3(256), 0, s, A,
+                  nops == 1 ? A : B, stride, S);
     unsigned total2 = (unsigned)(np * ngroups);
     unsigned grid2 = (total2 + 7) & ~7u;
     if (wide == 8)
@@ -1286,33 +1379,114 @@ static int launch_fast(const fa_operand& oa, const fa_operand& ob, int nops, int
 }
 
 template <int D1>
-static int launch_fast_any(const fa_operand& oa, const fa_operand& ob, int np, double* A,
-                           double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s)
+static int launch_fast_any(const fa_operand& oa, const fa_operand& ob, int nops, int np,
+                           double* A, double* B, uint64_t stride, const fa_prime_set& S,
+                           hipStream_t s)
 {
     switch (fa_pack_kind(oa, ob)) {
-    case FA_PACK_MPN8: return launch_fast<D1, FA_PACK_MPN8>(oa, ob, 2, np, A, B, stride, S, s);
-    case FA_PACK_U64: return launch_fast<D1, FA_PACK_U64>(oa, ob, 2, np, A, B, stride, S, s);
-    case FA_PACK_S64W: return launch_fast<D1, FA_PACK_S64W>(oa, ob, 2, np, A, B, stride, S, s);
-    default: return launch_fast<D1, FA_PACK_ANY>(oa, ob, 2, np, A, B, stride, S, s);
+    case FA_PACK_MPN8: return launch_fast<D1, FA_PACK_MPN8>(oa, ob, nops, np, A, B, stride, S, s);
+    case FA_PACK_U64: return launch_fast<D1, FA_PACK_U64>(oa, ob, nops, np, A, B, stride, S, s);
+    case FA_PACK_S64W: return launch_fast<D1, FA_PACK_S64W>(oa, ob, nops, np, A, B, stride, S, s);
+    default: return launch_fast<D1, FA_PACK_ANY>(oa, ob, nops, np, A, B, stride, S, s);
     }
 }
 
-int fa_convolve_fast(int depth, const fa_operand& oa, const fa_operand& ob, int np, double* A,
-                     double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s)
+// forward transform of one operand into A (column pass with the import,
+// then the row pass), for the transformed-operand interface
+template <int D1, int KIND>
+static int launch_forward(const fa_operand& oa, int np, double* A, uint64_t stride,
+                          const fa_prime_set& S, hipStream_t s)
 {
+    typedef col_geom<D1> G;
+    const int d2 = FA_ROW_LG;
+    const int wide = D1 == 12 ? 8 : 0;
+    const int ngroups = (1 << d2) / (wide ? wide : G::CPW);
+    const unsigned grid = ((unsigned)(np * ngroups) + 7) & ~7u;
+    if (wide)
+        FA_LAUNCH("fa_k_col_fwd16w", (fa_k_col_fwd16w<8, KIND>), dim3(grid), dim3(1024), 0, s, oa,
+                  oa, d2, A, A, stride, S, np, ngroups, 1);
+    else
+        FA_LAUNCH("fa_k_col_fwd16", (fa_k_col_fwd16<D1, KIND>), dim3(grid), dim3(1024), 0, s, oa,
+                  oa, d2, A, A, stride, S, np, ngroups, 1);
+    FA_LAUNCH("fa_k_row_fwd16", fa_k_row_fwd16, dim3(1u << D1, np), dim3(256), 0, s, A, stride, S);
+    return 1;
+}
+
+template <int D1>
+static int launch_inverse(int np, double* A, uint64_t stride, const fa_prime_set& S,
+                          hipStream_t s)
+{
+    typedef col_geom<D1> G;
+    const int d2 = FA_ROW_LG;
+    const int wide = D1 == 12 ? 8 : 0;
+    const int ngroups = (1 << d2) / (wide ? wide : G::CPW);
+    const unsigned grid = ((unsigned)(np * ngroups) + 7) & ~7u;
+    FA_LAUNCH("fa_k_row_inv16", fa_k_row_inv16, dim3(1u << D1, np), dim3(256), 0, s, A, stride, S);
+    if (wide)
+        FA_LAUNCH("fa_k_col_inv16w", fa_k_col_inv16w<8>, dim3(grid), dim3(1024), 0, s, A, d2,
+                  stride, S, np, ngroups);
+    else
+        FA_LAUNCH("fa_k_col_inv16", fa_k_col_inv16<D1>, dim3(grid), dim3(1024), 0, s, A, d2, stride,
+                  S, np, ngroups);
+    return 1;
+}
+
+template <int D1>
+static int launch_forward_any(const fa_operand& oa, int np, double* A, uint64_t stride,
+                              const fa_prime_set& S, hipStream_t s)
+{
+    switch (fa_pack_kind(oa, oa)) {
+    case FA_PACK_MPN8: return launch_forward<D1, FA_PACK_MPN8>(oa, np, A, stride, S, s);
+    case FA_PACK_U64: return launch_forward<D1, FA_PACK_U64>(oa, np, A, stride, S, s);
+    case FA_PACK_S64W: return launch_forward<D1, FA_PACK_S64W>(oa, np, A, stride, S, s);
+    default: return launch_forward<D1, FA_PACK_ANY>(oa, np, A, stride, S, s);
+    }
+}
+
+#define FA_D1_SWITCH(d1, F)                                                                \
+    switch (d1) {                                                                          \
+    case 1: return F(1); case 2: return F(2); case 3: return F(3); case 4: return F(4);    \
+    case 5: return F(5); case 6: return F(6); case 7: return F(7); case 8: return F(8);    \
+    case 9: return F(9); case 10: return F(10); case 11: return F(11);                     \
+    case 12: return F(12);                                                                 \
+    default: return 0;                                                                     \
+    }
+
+// depth 13..24: returns 1 when launched, 0 for other depths
+int fa_forward_fast(int depth, const fa_operand& oa, int np, double* A, uint64_t stride,
+                    const fa_prime_set& S, hipStream_t s)
+{
+#define FA_F(D) launch_forward_any<D>(oa, np, A, stride, S, s)
+    FA_D1_SWITCH(depth - FA_ROW_LG, FA_F)
+#undef FA_F
+}
+
+int fa_inverse_fast(int depth, int np, double* A, uint64_t stride, const fa_prime_set& S,
+                    hipStream_t s)
+{
+#define FA_F(D) launch_inverse<D>(np, A, stride, S, s)
+    FA_D1_SWITCH(depth - FA_ROW_LG, FA_F)
+#undef FA_F
+}
+
+// sqr: a == b; only a is transformed and the pointwise stage squares
+int fa_convolve_fast(int depth, const fa_operand& oa, const fa_operand& ob, int np, double* A,
+                     double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s, int sqr)
+{
+    const int nops = sqr ? 1 : 2;
     switch (depth - FA_ROW_LG) {
-    case 1: return launch_fast_any<1>(oa, ob, np, A, B, stride, S, s);
-    case 2: return launch_fast_any<2>(oa, ob, np, A, B, stride, S, s);
-    case 3: return launch_fast_any<3>(oa, ob, np, A, B, stride, S, s);
-    case 4: return launch_fast_any<4>(oa, ob, np, A, B, stride, S, s);
-    case 5: return launch_fast_any<5>(oa, ob, np, A, B, stride, S, s);
-    case 6: return launch_fast_any<6>(oa, ob, np, A, B, stride, S, s);
-    case 7: return launch_fast_any<7>(oa, ob, np, A, B, stride, S, s);
-    case 8: return launch_fast_any<8>(oa, ob, np, A, B, stride, S, s);
-    case 9: return launch_fast_any<9>(oa, ob, np, A, B, stride, S, s);
-    case 10: return launch_fast_any<10>(oa, ob, np, A, B, stride, S, s);
-    case 11: return launch_fast_any<11>(oa, ob, np, A, B, stride, S, s);
-    case 12: return launch_fast_any<12>(oa, ob, np, A, B, stride, S, s);
+    case 1: return launch_fast_any<1>(oa, ob, nops, np, A, B, stride, S, s);
+    case 2: return launch_fast_any<2>(oa, ob, nops, np, A, B, stride, S, s);
+    case 3: return launch_fast_any<3>(oa, ob, nops, np, A, B, stride, S, s);
+    case 4: return launch_fast_any<4>(oa, ob, nops, np, A, B, stride, S, s);
+    case 5: return launch_fast_any<5>(oa, ob, nops, np, A, B, stride, S, s);
+    case 6: return launch_fast_any<6>(oa, ob, nops, np, A, B, stride, S, s);
+    case 7: return launch_fast_any<7>(oa, ob, nops, np, A, B, stride, S, s);
+    case 8: return launch_fast_any<8>(oa, ob, nops, np, A, B, stride, S, s);
+    case 9: return launch_fast_any<9>(oa, ob, nops, np, A, B, stride, S, s);
+    case 10: return launch_fast_any<10>(oa, ob, nops, np, A, B, stride, S, s);
+    case 11: return launch_fast_any<11>(oa, ob, nops, np, A, B, stride, S, s);
+    case 12: return launch_fast_any<12>(oa, ob, nops, np, A, B, stride, S, s);
     default: return 0;
     }
 }

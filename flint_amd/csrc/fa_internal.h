@@ -212,6 +212,9 @@ int fa_batch_crt_mpn(int np, const fa_bmpn* M, const uint64_t* work, unsigned nw
                      const void* F, hipStream_t s);
 int fa_batch_carry(const fa_bmpn* M, const unsigned* jobs, unsigned njobs, hipStream_t s);
 int fa_nmod_primes(const fa_tables* T, uint64_t len, uint64_t n, int* np, int* offset);
+// transform depth of the window [zl, zh) of a zn-coefficient product, with
+// the wraparound of plan.c:47-89 when it leaves the window exact
+int fa_poly_window_depth(uint64_t zl, uint64_t zh, uint64_t zn, uint64_t xmax);
 uint64_t fa_ks_slot_bits(int64_t b1, int64_t b2, uint64_t an, uint64_t bn);
 // Kronecker route: variant 1 = signed slots, one integer product; 0 = three
 // products of non-negative parts (fa_set_ks_variant)

@@ -25,23 +25,24 @@
 
 // depth <= 12: one workgroup per prime does import, both forward
 // transforms, the pointwise product and the inverse transform in LDS.
+// sqr: a == b, one forward transform and a pointwise square
 __global__ void __launch_bounds__(256) fa_k_small_conv(fa_operand a, fa_operand b, int depth,
                                                        double* __restrict__ out, uint64_t stride,
-                                                       fa_prime_set S)
+                                                       fa_prime_set S, int sqr)
 {
     extern __shared__ double lds[];
     const int i = blockIdx.x;
     const double n = S.p[i], ninv = S.pinv[i];
     const int N = 1 << depth;
     double* sa = lds;
-    double* sb = lds + N;
+    double* sb = sqr ? lds : lds + N;
     for (int x = threadIdx.x; x < N; x += blockDim.x) {
         sa[x] = fa_operand_value(a, (uint64_t)x, S.pow2[i], n, ninv);
-        sb[x] = fa_operand_value(b, (uint64_t)x, S.pow2[i], n, ninv);
+        if (!sqr) sb[x] = fa_operand_value(b, (uint64_t)x, S.pow2[i], n, ninv);
     }
     __syncthreads();
     fa_lds_fwd(sa, depth, 0, 0, S.w2[i], n, ninv);
-    fa_lds_fwd(sb, depth, 0, 0, S.w2[i], n, ninv);
+    if (!sqr) fa_lds_fwd(sb, depth, 0, 0, S.w2[i], n, ninv);
     const double m = S.norm[i];
     for (int x = threadIdx.x; x < N; x += blockDim.x)
         sa[x] = fa_mulmod(fa_mulmod(sa[x], m, n, ninv), sb[x], n, ninv);
@@ -1036,11 +1037,16 @@ static fa_operand to_operand(const fa_pack_desc* d)
 }
 
 int fa_convolve_fast(int depth, const fa_operand& oa, const fa_operand& ob, int np, double* A,
-                     double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s);
+                     double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s, int sqr);
 
 int fa_fast_enabled = 1;   // switched off by tests comparing the two kernel sets
 
 extern "C" FA_API void fa_set_fast_kernels(int on) { fa_fast_enabled = on; }
+
+// squares take the one-transform route (switched off by tests comparing it
+// with the two-transform product)
+int fa_sqr_enabled = 1;
+extern "C" FA_API void fa_set_square_route(int on) { fa_sqr_enabled = on; }
 
 static int col_lg(int d1, int d2)
 {
@@ -1058,14 +1064,18 @@ int fa_convolve(fa_thread_state* st, const fa_tables* T, int np, int offset, int
     fa_prime_set_init(&S, T, np, depth, offset, 1);
     fa_operand oa = to_operand(a), ob = to_operand(b);
     const uint64_t stride = fa_res_stride(depth);
+    // a square (the same operand twice, mpn_mul.c:1665, 1513-1521): one forward
+    // transform per prime
+    const int sqr = fa_sqr_enabled && a->kind == b->kind && a->src == b->src && a->n == b->n &&
+                    a->bits == b->bits && a->width == b->width;
     if (depth <= FA_ROW_LG) {
-        size_t lds = 2 * stride * sizeof(double);
-        FA_LAUNCH("fa_k_small_conv", fa_k_small_conv, dim3(np), dim3(256), lds, stream, oa, ob, depth, A,
-                           stride, S);
+        size_t lds = (sqr ? 1 : 2) * stride * sizeof(double);
+        FA_LAUNCH("fa_k_small_conv", fa_k_small_conv, dim3(np), dim3(256), lds, stream, oa, ob,
+                  depth, A, stride, S, sqr);
         LAUNCH_CHECK();
         return FA_OK;
     }
-    if (fa_fast_enabled && fa_convolve_fast(depth, oa, ob, np, A, B, stride, S, stream)) {
+    if (fa_fast_enabled && fa_convolve_fast(depth, oa, ob, np, A, B, stride, S, stream, sqr)) {
         LAUNCH_CHECK();
         return FA_OK;
     }
@@ -1361,6 +1371,33 @@ static int mpn_mul_device(uint64_t* z, const uint64_t* a, uint64_t an, const uin
     return mpn_mul_plan_device(z, a, an, b, bn, stream, np_force, bits_force);
 }
 
+// CRT of zlen coefficients of `bits`-bit chunks into z[0, zn) (the low zn
+// limbs of the sum; zeros above it), carry flags in the thread's scratch
+int fa_crt_mpn_export(int np, const double* A, uint64_t stride, uint64_t zlen, int bits,
+                      uint64_t* z, uint64_t zn, hipStream_t stream)
+{
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    if (!stream) stream = st->stream;
+    const uint64_t own = (uint64_t)bits / 8;
+    const uint64_t ngroups = (zn + own - 1) / own;
+    const uint64_t nblk16 = (ngroups + (FA_CRT16_GROUPS - 2)) / (FA_CRT16_GROUPS - 1);
+    const size_t bytes = ((ngroups + 7) & ~(size_t)7) + 2 * (nblk16 + 8) + 4096;
+    void* buf;
+    int rc = fa_scratch_reserve(st, bytes, &buf, stream);
+    if (rc) return rc;
+    uint8_t* flags = (uint8_t*)buf;
+    uint8_t* agg = flags + ((ngroups + 7) & ~UINT64_C(7));
+    uint8_t* cin = agg + ((nblk16 + 7) & ~UINT64_C(7));
+    fa_crt_consts C;
+    fa_crt_consts_init(&C, np, 0);
+    rc = crt16(np, A, stride, C, zlen, bits, z, zn, ngroups, flags, agg, cin, nblk16, stream);
+    if (rc) return rc;
+    rc = fa_buffers_leave(st, stream);
+    if (rc) return rc;
+    return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "crt launch");
+}
+
 // the launches of one product on plan P, with scratch `buf` of at least
 // mpn_scratch_bytes(P, an + bn) bytes
 static int mpn_mul_run(uint64_t* z, const uint64_t* a, uint64_t an, const uint64_t* b,
@@ -1486,8 +1523,12 @@ static int mpn_mul_host(uint64_t* z, const uint64_t* a, int64_t an, const uint64
     uint64_t* da = (uint64_t*)hb;
     uint64_t* db = da + an;
     uint64_t* dz = db + bn;
+    // a square keeps one device copy, so the device path sees a == b
+    const bool sqr = a == b && an == bn;
+    if (sqr) db = da;
     if (hipMemcpyAsync(da, a, an * sizeof(uint64_t), hipMemcpyHostToDevice, st->stream) != hipSuccess ||
-        hipMemcpyAsync(db, b, bn * sizeof(uint64_t), hipMemcpyHostToDevice, st->stream) != hipSuccess)
+        (!sqr && hipMemcpyAsync(db, b, bn * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                st->stream) != hipSuccess))
         return fa_set_error(FA_E_HIP, "operand upload failed");
     rc = mpn_mul_device(dz, da, (uint64_t)an, db, (uint64_t)bn, st->stream, np, bits);
     if (rc) {
@@ -1506,6 +1547,215 @@ extern "C" int fa_mpn_mul(uint64_t* z, const uint64_t* a, int64_t an, const uint
                           int64_t bn)
 {
     return mpn_mul_host(z, a, an, b, bn, 0, 0);
+}
+
+// ------------------------------------------------------------------------
+// products mod 2^N + 1, N = 64 n (sd_fft_mpn_mulmod_2expp1, negacyclic.c:
+// 491-501): operands of n + 1 limbs with value <= 2^N, result normalized to
+// [0, 2^N].  The full product P = x y <= 2^(2N) is formed by the ordinary
+// path; with lo = P mod 2^N and hi = P >> N <= 2^N, x y = lo - hi (mod
+// 2^N + 1): D = lo - hi over n + 1 limbs, and D + 2^N + 1 when D < 0.
+// One workgroup, each thread a run of consecutive limbs, borrow and carry
+// chains resolved by workgroup scans of (generate, propagate) bits.
+// ------------------------------------------------------------------------
+
+// exclusive scan of per-thread (g, p) chains over the workgroup: the carry
+// into thread t given carry-in c0 at thread 0
+FA_DEV uint8_t fa_wg_chain_in(uint8_t g, uint8_t p, uint8_t c0, uint8_t* sg, uint8_t* sp)
+{
+    const int t = threadIdx.x, nt = blockDim.x;
+    sg[t] = g;
+    sp[t] = p;
+    __syncthreads();
+    for (int d = 1; d < nt; d <<= 1) {
+        uint8_t gg = sg[t], pp = sp[t];
+        if (t >= d) {
+            gg = (uint8_t)(gg | (pp & sg[t - d]));
+            pp = (uint8_t)(pp & sp[t - d]);
+        }
+        __syncthreads();
+        sg[t] = gg;
+        sp[t] = pp;
+        __syncthreads();
+    }
+    const uint8_t c = t ? (uint8_t)(sg[t - 1] | (sp[t - 1] & c0)) : c0;
+    __syncthreads();
+    return c;
+}
+
+__global__ void __launch_bounds__(1024) fa_k_fold_2expp1(uint64_t* __restrict__ z,
+                                                         const uint64_t* __restrict__ P,
+                                                         uint64_t n)
+{
+    __shared__ uint8_t sg[1024], sp[1024];
+    __shared__ uint8_t sneg;
+    const int t = threadIdx.x;
+    const uint64_t L = (n + 1 + 1023) / 1024;        // limbs of d = lo - hi per thread
+    const uint64_t i0 = (uint64_t)t * L < n + 1 ? (uint64_t)t * L : n + 1;
+    const uint64_t i1 = i0 + L < n + 1 ? i0 + L : n + 1;
+    // 1. d = lo - hi over n + 1 limbs (lo_n = 0): borrow generate / propagate
+    uint8_t G = 0, Q = 1;
+    for (uint64_t i = i0; i < i1; i++) {
+        const uint64_t x = i < n ? P[i] : 0, y = P[n + i];
+        const uint8_t g = x < y, q = x == y;
+        G = (uint8_t)(g | (q & G));
+        Q = (uint8_t)(Q & q);
+    }
+    uint8_t b = fa_wg_chain_in(G, Q, 0, sg, sp);
+    for (uint64_t i = i0; i < i1; i++) {
+        const uint64_t x = i < n ? P[i] : 0, y = P[n + i];
+        z[i] = x - y - b;
+        b = (uint8_t)(x < y || (x == y && b));
+    }
+    if (t == 1023) sneg = b;         // borrow out of the last limb: D < 0
+    __syncthreads();
+    const bool neg = sneg;
+    // 2. D < 0: r = D + 2^N + 1, i.e. the low n limbs (D mod 2^N) plus one,
+    // the carry out of them giving limb n; D >= 0: limb n is 0
+    if (!neg) {
+        if (t == 0) z[n] = 0;
+        return;
+    }
+    const uint64_t m0 = i0 < n ? i0 : n, m1 = i1 < n ? i1 : n;
+    uint8_t Pn = 1;
+    for (uint64_t i = m0; i < m1; i++) Pn = (uint8_t)(Pn & (z[i] == ~UINT64_C(0)));
+    uint8_t c = fa_wg_chain_in(0, Pn, 1, sg, sp);
+    for (uint64_t i = m0; i < m1; i++) {
+        const uint64_t v = z[i] + c;
+        c = (uint8_t)(c && v == 0);
+        z[i] = v;
+    }
+    if (m1 == n && m0 < m1) z[n] = c;   // the run holding limb n - 1
+}
+
+// z[0, n] = x y mod 2^(64 n) + 1 on device operands of n + 1 limbs with
+// values <= 2^(64 n), enqueued on `stream`
+extern "C" int fa_mpn_mulmod_2expp1_device(uint64_t* z, const uint64_t* x, const uint64_t* y,
+                                           int64_t n, void* stream)
+{
+    if (n <= 0 || !z || !x || !y)
+        return fa_set_error(FA_E_ARG, "fa_mpn_mulmod_2expp1_device: bad arguments");
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    hipStream_t s = stream ? (hipStream_t)stream : st->stream;
+    // one plan covers the product (block products would share the buffers)
+    fa_mpn_plan_t Q;
+    const uint64_t pn = 2 * (uint64_t)n + 2;
+    if (pn > fa_mpn_block_cap || fa_mpn_choose_plan((uint64_t)n + 1, (uint64_t)n + 1, &Q))
+        return fa_set_error(FA_E_RANGE, "mulmod 2^N+1: n = %lld above one transform",
+                            (long long)n);
+    void* hb;
+    int rc = fa_hbuf_reserve(st, pn * sizeof(uint64_t), &hb, s);
+    if (rc) return rc;
+    uint64_t* P = (uint64_t*)hb;
+    rc = mpn_mul_device(P, x, (uint64_t)n + 1, y, (uint64_t)n + 1, s);
+    if (rc) return rc;
+    FA_LAUNCH("fa_k_fold_2expp1", fa_k_fold_2expp1, dim3(1), dim3(1024), 0, s, z, P, (uint64_t)n);
+    if (hipGetLastError() != hipSuccess) return fa_set_error(FA_E_HIP, "fold launch failed");
+    return fa_buffers_leave(st, s);
+}
+
+// host operands: checked to be normalized (top limb 0, or 1 with the rest 0)
+extern "C" int fa_mpn_mulmod_2expp1(uint64_t* z, const uint64_t* x, const uint64_t* y, int64_t n)
+{
+    if (n <= 0 || !z || !x || !y)
+        return fa_set_error(FA_E_ARG, "fa_mpn_mulmod_2expp1: bad arguments");
+    for (const uint64_t* v : {x, y}) {
+        if (v[n] > 1) return fa_set_error(FA_E_ARG, "operand above 2^N: top limb %llu",
+                                          (unsigned long long)v[n]);
+        if (v[n] == 1)
+            for (int64_t i = 0; i < n; i++)
+                if (v[i]) return fa_set_error(FA_E_ARG, "operand above 2^N");
+    }
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    const uint64_t m = (uint64_t)n + 1;
+    void* tb;
+    int rc = fa_tbuf_reserve(st, 3 * m * sizeof(uint64_t), &tb, st->stream);
+    if (rc) return rc;
+    uint64_t* dx = (uint64_t*)tb;
+    uint64_t* dy = x == y ? dx : dx + m;
+    uint64_t* dz = dx + 2 * m;
+    if (hipMemcpyAsync(dx, x, m * 8, hipMemcpyHostToDevice, st->stream) != hipSuccess ||
+        (dy != dx && hipMemcpyAsync(dy, y, m * 8, hipMemcpyHostToDevice, st->stream) != hipSuccess))
+        return fa_set_error(FA_E_HIP, "operand upload failed");
+    rc = fa_mpn_mulmod_2expp1_device(dz, dx, dy, n, st->stream);
+    if (rc) {
+        (void)hipStreamSynchronize(st->stream);
+        return rc;
+    }
+    if (hipMemcpyAsync(z, dz, m * 8, hipMemcpyDeviceToHost, st->stream) != hipSuccess ||
+        hipStreamSynchronize(st->stream) != hipSuccess)
+        return fa_set_error(FA_E_HIP, "result download failed");
+    return FA_OK;
+}
+
+// _mpn_ctx_mpn_mul_range (mpn_mul.c:1617-1978) and flint_mpn_mulmid_fft_small
+// (mpn_extras/mulmid_fft_small.c:26-33): limbs [lo, hi) of a*b.  The
+// reference returns a lower approximation (partial products landing wholly
+// below lo dropped, deficit below 2^64 min(an, bn, lo) in units of limb lo,
+// doc/source/mpn_extras.rst:221-231); this window is exact, the tightest
+// approximation that contract admits.  The product is formed on the device
+// and only the window crosses PCIe.
+extern "C" int fa_mpn_mul_range(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a,
+                                int64_t an, const uint64_t* b, int64_t bn)
+{
+    if (an <= 0 || bn <= 0 || !a || !b || lo >= hi || hi > (uint64_t)(an + bn) || !z)
+        return fa_set_error(FA_E_ARG, "fa_mpn_mul_range: bad arguments");
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    const uint64_t zn = (uint64_t)(an + bn);
+    void* hb;
+    int rc = fa_hbuf_reserve(st, ((size_t)an + (size_t)bn + zn) * sizeof(uint64_t), &hb,
+                             st->stream);
+    if (rc) return rc;
+    uint64_t* da = (uint64_t*)hb;
+    uint64_t* db = da + an;
+    uint64_t* dz = db + bn;
+    const bool sqr = a == b && an == bn;
+    if (sqr) db = da;
+    if (hipMemcpyAsync(da, a, an * sizeof(uint64_t), hipMemcpyHostToDevice, st->stream) != hipSuccess ||
+        (!sqr && hipMemcpyAsync(db, b, bn * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                st->stream) != hipSuccess))
+        return fa_set_error(FA_E_HIP, "operand upload failed");
+    rc = mpn_mul_device(dz, da, (uint64_t)an, db, (uint64_t)bn, st->stream);
+    if (rc) {
+        (void)hipStreamSynchronize(st->stream);
+        return rc;
+    }
+    if (hipMemcpyAsync(z, dz + lo, (hi - lo) * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                       st->stream) != hipSuccess)
+        return fa_set_error(FA_E_HIP, "window download failed");
+    if (hipStreamSynchronize(st->stream) != hipSuccess)
+        return fa_set_error(FA_E_HIP, "stream synchronize failed: %s",
+                            hipGetErrorString(hipGetLastError()));
+    return FA_OK;
+}
+
+// the same on device operands: the product goes to the thread's block
+// buffer and the window is copied to z on `stream`
+extern "C" int fa_mpn_mul_range_device(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a,
+                                       int64_t an, const uint64_t* b, int64_t bn, void* stream)
+{
+    if (an <= 0 || bn <= 0 || !a || !b || lo >= hi || hi > (uint64_t)(an + bn) || !z)
+        return fa_set_error(FA_E_ARG, "fa_mpn_mul_range_device: bad arguments");
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    hipStream_t s = stream ? (hipStream_t)stream : st->stream;
+    const uint64_t zn = (uint64_t)(an + bn);
+    if (lo == 0 && hi == zn) return mpn_mul_device_call(z, a, an, b, bn, s, 0, 0);
+    // the window's own buffer: block products (mpn_mul_blocked) use tbuf,
+    // so the full product goes to hbuf here
+    void* hb;
+    int rc = fa_hbuf_reserve(st, zn * sizeof(uint64_t), &hb, s);
+    if (rc) return rc;
+    uint64_t* dz = (uint64_t*)hb;
+    rc = mpn_mul_device(dz, a, (uint64_t)an, b, (uint64_t)bn, s);
+    if (rc) return rc;
+    if (hipMemcpyAsync(z, dz + lo, (hi - lo) * sizeof(uint64_t), hipMemcpyDeviceToDevice, s) !=
+        hipSuccess)
+        return fa_set_error(FA_E_HIP, "window copy failed");
+    return fa_buffers_leave(st, s);
 }
 
 extern "C" int fa_mpn_mul_plan(uint64_t* z, const uint64_t* a, int64_t an, const uint64_t* b,

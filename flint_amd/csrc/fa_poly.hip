@@ -405,6 +405,20 @@ __global__ void __launch_bounds__(256) fa_k_add_coeffs(uint64_t* __restrict__ z,
 uint64_t fa_poly_block_cap = 0;
 extern "C" FA_API void fa_set_poly_block_cap(uint64_t n) { fa_poly_block_cap = n; }
 
+// transform depth of the window [zl, zh) of a product of zn coefficients
+// whose operands have at most xmax coefficients (plan.c:47-89): the power of
+// two 2^d = 2^floor(log2 zn) when the cyclic product mod X^(2^d) - 1 leaves
+// the window untouched -- operands no longer than 2^d, zh <= 2^d and
+// zn - 2^d <= zl, so the coefficients k + 2^d folded onto the window are
+// zero -- otherwise the smallest 2^d >= zn.  At least 16 points.
+int fa_poly_window_depth(uint64_t zl, uint64_t zh, uint64_t zn, uint64_t xmax)
+{
+    int d = 63 - __builtin_clzll(zn);
+    const uint64_t i = UINT64_C(1) << d;
+    if (!(xmax <= i && zh <= i && i <= zn && zn <= zl + i)) d = clog2_u64(zn);
+    return d < 4 ? 4 : d;
+}
+
 static uint64_t poly_block_limit(uint64_t zn)
 {
     const uint64_t full = UINT64_C(1) << FA_MAX_DEPTH;
@@ -536,8 +550,8 @@ extern "C" int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh, const u
     int rc0 = fa_nmod_primes(T, (uint64_t)(an < bn ? an : bn), n, &J.np, &J.offset);
     if (rc0) return rc0;
     fa_crt_consts_init(&J.C, J.np, J.offset);
-    J.depth = clog2_u64((uint64_t)zn);
-    if (J.depth < 4) J.depth = 4;
+    J.depth = fa_poly_window_depth((uint64_t)zl, (uint64_t)zh, (uint64_t)zn,
+                                   (uint64_t)(an > bn ? an : bn));
     if (const uint64_t lim = poly_block_limit((uint64_t)zn))
         return poly_mul_blocked(0, z, 1, zl, zh, a, 1, an, b, 1, bn, n, J.np, J.offset, J.C, lim);
 
@@ -547,8 +561,11 @@ extern "C" int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh, const u
     int rc = poly_buffers(st, an, bn, J.np, J.depth, cnt, &da, &db, &A, &B, &dz);
     if (rc) return rc;
     hipStream_t s = st->stream;
+    // a square keeps one device copy (nmod_poly_mul.c:939 squares when a == b)
+    if (a == b && an == bn) db = da;
     HIPTRY(hipMemcpyAsync(da, a, an * sizeof(uint64_t), hipMemcpyHostToDevice, s), "upload a");
-    HIPTRY(hipMemcpyAsync(db, b, bn * sizeof(uint64_t), hipMemcpyHostToDevice, s), "upload b");
+    if (db != da)
+        HIPTRY(hipMemcpyAsync(db, b, bn * sizeof(uint64_t), hipMemcpyHostToDevice, s), "upload b");
     fa_pack_desc pa = {FA_PACK_U64, da, (uint64_t)an, 0, 1};
     fa_pack_desc pb = {FA_PACK_U64, db, (uint64_t)bn, 0, 1};
     rc = fa_convolve(st, T, J.np, J.offset, J.depth, &pa, &pb, A, B, s);
@@ -633,6 +650,44 @@ int fa_nmod_primes(const fa_tables* T, uint64_t len, uint64_t n, int* np, int* o
     return *np ? FA_OK : fa_set_error(FA_E_RANGE, "no prime set for modulus");
 }
 
+// prime set of a plan whose output coefficients accumulate at most
+// len_bound products of prod_bits-bit values (plan.c:356-398): the modulus
+// itself when it is one of the FFT primes, otherwise the fewest primes with
+// prod >= len_bound * 2^prod_bits
+int fa_nmod_primes_bound(const fa_tables* T, uint64_t len_bound, uint64_t prod_bits, uint64_t n,
+                         int* np, int* offset)
+{
+    *np = 0;
+    *offset = 0;
+    if (nbits_u64(n) == 50)
+        for (int i = 0; i < FA_NPRIMES; i++)
+            if (T->pr[i].n == n) { *np = 1; *offset = i; }
+    for (int k = 1; k <= FA_NPRIMES && !*np; k++) {
+        fa_crt_consts C;
+        fa_crt_consts_init(&C, k, 0);
+        if (prod_ge(C, len_bound, prod_bits)) *np = k;
+    }
+    return *np ? FA_OK : fa_set_error(FA_E_RANGE, "no prime set for %llu products of %llu bits",
+                                      (unsigned long long)len_bound,
+                                      (unsigned long long)prod_bits);
+}
+
+// coefficients [zl, zl + cnt) mod n of inverse-transformed residue rows
+int fa_crt_nmod_launch(const double* A, uint64_t stride, const fa_crt_consts& C, uint64_t zl,
+                       uint64_t cnt, uint64_t n, uint64_t* dz, hipStream_t s)
+{
+    if (!cnt) return FA_OK;
+    const int norm = 64 - nbits_u64(n);
+    const uint64_t d = n << norm;
+    const uint64_t dinv = (uint64_t)(((u128)0 - 1) / d - ((u128)1 << 64));
+#define FA_L(NP)                                                                            \
+    FA_LAUNCH("fa_k_crt_nmod", fa_k_crt_nmod<NP>, dim3((unsigned)((cnt + 255) / 256)),      \
+              dim3(256), 0, s, A, stride, C, zl, cnt, n, dinv, norm, dz)
+    FA_CRT_NP_SWITCH(C.np, FA_L)
+#undef FA_L
+    return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "crt launch");
+}
+
 extern "C" int fa_fmpz_poly_mul_mid(uint64_t* z, uint64_t zw, int64_t zl, int64_t zh,
                                     const uint64_t* a, uint64_t aw, int64_t an, const uint64_t* b,
                                     uint64_t bw, int64_t bn)
@@ -652,8 +707,8 @@ extern "C" int fa_fmpz_poly_mul_mid(uint64_t* z, uint64_t zw, int64_t zl, int64_
     int64_t b2 = fa_vec_max_bits(b, bw, bn);
     int np = fa_fmpz_primes((uint64_t)bn, b1, b2);
     if (!np) return 0;
-    int depth = clog2_u64((uint64_t)zn);
-    if (depth < 4) depth = 4;
+    const int depth = fa_poly_window_depth((uint64_t)zl, (uint64_t)zh, (uint64_t)zn,
+                                           (uint64_t)(an > bn ? an : bn));
     fa_crt_consts C;
     fa_crt_consts_init(&C, np, 0);
     if (const uint64_t lim = poly_block_limit((uint64_t)zn)) {
@@ -671,10 +726,14 @@ extern "C" int fa_fmpz_poly_mul_mid(uint64_t* z, uint64_t zw, int64_t zl, int64_
                           &A, &B, &dz);
     if (rc) return rc;
     hipStream_t s = st->stream;
+    // a square keeps one device copy (fmpz_poly_mul.c:352 squares when a == b)
+    if (a == b && an == bn && aw == bw) db = da;
     HIPTRY(hipMemcpyAsync(da, a, (uint64_t)an * aw * sizeof(uint64_t), hipMemcpyHostToDevice, s),
            "upload a");
-    HIPTRY(hipMemcpyAsync(db, b, (uint64_t)bn * bw * sizeof(uint64_t), hipMemcpyHostToDevice, s),
-           "upload b");
+    if (db != da)
+        HIPTRY(hipMemcpyAsync(db, b, (uint64_t)bn * bw * sizeof(uint64_t), hipMemcpyHostToDevice,
+                              s),
+               "upload b");
     fa_pack_desc pa = {FA_PACK_S64W, da, (uint64_t)an, 0, (int)aw};
     fa_pack_desc pb = {FA_PACK_S64W, db, (uint64_t)bn, 0, (int)bw};
     rc = fa_convolve(st, T, np, 0, depth, &pa, &pb, A, B, s);

@@ -127,6 +127,26 @@ FA_API int fa_fmpz_poly_mul_mid_mpn_ctx(uint64_t* z, uint64_t zw, int64_t zl, in
 FA_API int fa_mpn_mul(uint64_t* z, const uint64_t* a, int64_t an,
                const uint64_t* b, int64_t bn);
 
+/* _mpn_ctx_mpn_mul_range, mpn_mul.c:1617-1978, as flint_mpn_mulmid_fft_small
+   (mpn_extras/mulmid_fft_small.c:26-33) calls it: z[0, hi-lo) = limbs
+   [lo, hi) of a*b, 0 <= lo < hi <= an+bn.  Exact: the reference's contract
+   is a lower approximation whose deficit is at most one carry from below
+   lo (doc/source/mpn_extras.rst:221-231); the exact window meets it. */
+FA_API int fa_mpn_mul_range(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a,
+                            int64_t an, const uint64_t* b, int64_t bn);
+/* sd_fft_mpn_mulmod_2expp1 (negacyclic.c:491-501), the pointwise stage of
+   FLINT's Schoenhage-Strassen convolutions (fmpz_poly/mul_SS.c):
+   z[0, n] = x y mod 2^(64 n) + 1 for x, y of n + 1 limbs with values at
+   most 2^(64 n), the result normalized to [0, 2^(64 n)].  Host operands are
+   checked; the device variant enqueues on `stream`.  n + 1 limbs must fit
+   one transform (2^24 points). */
+FA_API int fa_mpn_mulmod_2expp1(uint64_t* z, const uint64_t* x, const uint64_t* y, int64_t n);
+FA_API int fa_mpn_mulmod_2expp1_device(uint64_t* z, const uint64_t* x, const uint64_t* y,
+                                       int64_t n, void* stream);
+/* the same with device operands and output, enqueued on `stream` */
+FA_API int fa_mpn_mul_range_device(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a,
+                                   int64_t an, const uint64_t* b, int64_t bn, void* stream);
+
 /* same with operands and output in device memory (no PCIe copies) */
 FA_API int fa_mpn_mul_device(uint64_t* z, const uint64_t* a, int64_t an,
                       const uint64_t* b, int64_t bn, void* stream);
@@ -156,6 +176,91 @@ FA_API int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh,
 FA_API int fa_fmpz_poly_mul_mid(uint64_t* z, uint64_t zw, int64_t zl, int64_t zh,
                          const uint64_t* a, uint64_t aw, int64_t an,
                          const uint64_t* b, uint64_t bw, int64_t bn);
+
+/* ---------------------------------------------------------------------- */
+/* plans and transformed operands, fft_small.h:530-700 (plan.c, op.c)      */
+/* ---------------------------------------------------------------------- */
+
+/* A plan: prime set, chunk size and transform depth for products of at
+   most an_max x bn_max limbs (kind 0) or for the window [zl, zh) of a
+   length-zn convolution mod n (kind 1), with up to len_bound products
+   accumulated in transform space (fft_small_plan_struct, fft_small.h:530). */
+typedef struct {
+    int32_t kind;            /* 0: naturals (mpn), 1: polynomials mod n */
+    int32_t np, offset;      /* primes offset .. offset + np - 1 */
+    int32_t bits;            /* chunk bits (kind 0) */
+    int32_t depth, pad;      /* transform length 2^depth */
+    uint64_t stride;         /* doubles per prime in an op */
+    uint64_t zl, zh, zn;     /* output window; zn = coefficients of the convolution */
+    uint64_t an_max, bn_max, len_bound;
+    fa_nmod_t mod;
+} fa_plan_struct;
+
+#define FA_OP_PRIMAL 0       /* forward transforms (FFT_SMALL_OP_PRIMAL) */
+#define FA_OP_PRODUCT 1      /* pointwise products, one normalization (FFT_SMALL_OP_PRODUCT) */
+#define FA_OP_COEFF 2        /* after fa_ifft: ready for export */
+
+/* A transformed operand in device memory: np arrays of `stride` doubles in
+   this build's own layout (fft_small_op_struct, fft_small.h:620-640). */
+typedef struct {
+    double* data;
+    uint64_t stride, np, offset, depth;
+    int32_t domain;          /* FA_OP_* */
+    int32_t owns_data;
+} fa_op_struct;
+
+/* fft_small_plan_init_mpn, plan.c:443: exact for up to len_bound
+   accumulated products of an_max x bn_max limbs (is_signed must be 0:
+   signed exports are not built) */
+FA_API int fa_plan_init_mpn(fa_plan_struct* P, uint64_t an_max, uint64_t bn_max,
+                            uint64_t len_bound, int is_signed);
+/* fft_small_plan_init_nmod, plan.c:400: len_bound products of prod_bits-bit
+   values per coefficient (2 * bits of n for a plain product) */
+FA_API int fa_plan_init_nmod(fa_plan_struct* P, uint64_t zl, uint64_t zh, uint64_t zn,
+                             uint64_t xtrunc_max, uint64_t len_bound, uint64_t prod_bits,
+                             fa_nmod_t mod);
+FA_API void fa_plan_clear(fa_plan_struct* P);
+/* fft_small_op_init / _init_borrowed / _sizeof_data / _clear, op.c:18-80
+   (init allocates device memory; borrowed uses the caller's) */
+FA_API int fa_op_init(fa_op_struct* X, const fa_plan_struct* P);
+FA_API int fa_op_init_borrowed(fa_op_struct* X, const fa_plan_struct* P, void* data);
+FA_API uint64_t fa_op_sizeof_data(const fa_plan_struct* P);
+FA_API void fa_op_clear(fa_op_struct* X);
+/* fft_small_fft_mpn / fft_small_fft_nmod (mpn_mul.c:2040, nmod_poly_mul.c:752):
+   forward transform of a device operand into X.  An operand of the first
+   factor has at most an_max limbs, of the second at most bn_max. */
+FA_API int fa_fft_mpn(fa_op_struct* X, const uint64_t* a, int64_t an, const fa_plan_struct* P,
+                      void* stream);
+FA_API int fa_fft_nmod(fa_op_struct* X, const uint64_t* a, int64_t an, fa_nmod_t mod,
+                       const fa_plan_struct* P, void* stream);
+/* fft_small_op_mul/sqr/addmul/submul/add/sub/neg, op.c:272-418; Z may
+   alias A and B */
+FA_API int fa_op_mul(fa_op_struct* Z, const fa_op_struct* A, const fa_op_struct* B,
+                     const fa_plan_struct* P, void* stream);
+FA_API int fa_op_sqr(fa_op_struct* Z, const fa_op_struct* A, const fa_plan_struct* P,
+                     void* stream);
+FA_API int fa_op_addmul(fa_op_struct* Z, const fa_op_struct* A, const fa_op_struct* B,
+                        const fa_plan_struct* P, void* stream);
+FA_API int fa_op_submul(fa_op_struct* Z, const fa_op_struct* A, const fa_op_struct* B,
+                        const fa_plan_struct* P, void* stream);
+FA_API int fa_op_add(fa_op_struct* Z, const fa_op_struct* A, const fa_op_struct* B,
+                     const fa_plan_struct* P, void* stream);
+FA_API int fa_op_sub(fa_op_struct* Z, const fa_op_struct* A, const fa_op_struct* B,
+                     const fa_plan_struct* P, void* stream);
+FA_API int fa_op_neg(fa_op_struct* Z, const fa_op_struct* A, const fa_plan_struct* P,
+                     void* stream);
+/* fft_small_ifft, op.c:420: inverse transform in place */
+FA_API int fa_ifft(fa_op_struct* X, const fa_plan_struct* P, void* stream);
+/* fft_small_export_mpn (mpn_mul.c:2240): low zn limbs of the accumulated
+   integer into device z */
+FA_API int fa_export_mpn(uint64_t* z, int64_t zn, const fa_op_struct* X,
+                         const fa_plan_struct* P, void* stream);
+/* fft_small_export_nmod / _range (nmod_poly_mul.c:800-901): coefficients
+   of the plan's window, or of [zl, zh), mod n into device z */
+FA_API int fa_export_nmod(uint64_t* z, const fa_op_struct* X, fa_nmod_t mod,
+                          const fa_plan_struct* P, void* stream);
+FA_API int fa_export_nmod_range(uint64_t* z, const fa_op_struct* X, uint64_t zl, uint64_t zh,
+                                fa_nmod_t mod, const fa_plan_struct* P, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* sd_fft_ctx plugin surface, fft_small.h:206-281                         */
@@ -258,6 +363,10 @@ FA_API void fa_set_poly_block_cap(uint64_t n);
    stream is being captured into a graph or kernel timing is on.  Tuning
    switch, no counterpart in the reference. */
 FA_API void fa_set_prime_groups(int v);
+/* squares (a == b, an == bn; mpn_mul.c:1665, nmod_poly_mul.c:939,
+   fmpz_poly_mul.c:352): 1 = one forward transform per prime and a pointwise
+   square (default), 0 = the general product.  Tuning switch for tests. */
+FA_API void fa_set_square_route(int on);
 /* CRT of integer products: 1 = radix-2^24 digits on the FP64 lanes
    (default), 0 = 64-bit limb arithmetic */
 FA_API void fa_set_crt_variant(int v);

@@ -1,0 +1,61 @@
+"""GPU parity of products mod 2^N + 1 (sd_fft_mpn_mulmod_2expp1,
+negacyclic.c; the pointwise stage of FLINT's Schoenhage-Strassen
+convolutions): random normalized operands and the edge values 0, 1, 2^N - 1
+and 2^N (= -1), whose products hit both branches of the fold (D = lo - hi
+negative or not, the results 2^N and 1), against Python integers.  The
+device entry point on one pointer for both operands takes the squaring
+route."""
+import numpy as np
+import pytest
+
+from tests.helpers import from_int, to_int
+
+pytestmark = pytest.mark.gpu
+
+
+def _norm_rand(rng, n):
+    N = 64 * n
+    if rng.integers(0, 8) == 0:
+        return 1 << N
+    return int.from_bytes(rng.bytes(8 * n), "little")
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 1000, 33000, 300000])
+def test_mulmod_2expp1_matches_integers(n):
+    import flint_amd as fa
+    rng = np.random.default_rng(n)
+    N = 64 * n
+    F = (1 << N) + 1
+    edges = [0, 1, (1 << N) - 1, 1 << N, 2]
+    pairs = [(u, v) for u in edges for v in edges]
+    pairs += [(_norm_rand(rng, n), _norm_rand(rng, n)) for _ in range(6)]
+    if n > 1000:
+        pairs = pairs[::3]
+    for u, v in pairs:
+        x, y = from_int(u, n + 1), from_int(v, n + 1)
+        z = fa.mpn_mulmod_2expp1(x, y, n)
+        assert to_int(z) == (u * v) % F, (n, u.bit_length(), v.bit_length())
+    # squares (one operand array)
+    u = _norm_rand(rng, n)
+    x = from_int(u, n + 1)
+    assert to_int(fa.mpn_mulmod_2expp1(x, x, n)) == (u * u) % F
+
+
+def test_mulmod_2expp1_device_square_and_rejects():
+    import torch
+    import flint_amd as fa
+    n = 4096
+    rng = np.random.default_rng(3)
+    u = int.from_bytes(rng.bytes(8 * n), "little")
+    x = from_int(u, n + 1)
+    dx = torch.from_numpy(x.view(np.int64)).to("cuda:0")
+    dz = torch.zeros(n + 1, dtype=torch.int64, device="cuda:0")
+    s = torch.cuda.Stream()
+    assert fa.lib().fa_mpn_mulmod_2expp1_device(dz.data_ptr(), dx.data_ptr(), dx.data_ptr(), n,
+                                                s.cuda_stream) == 0
+    s.synchronize()
+    assert to_int(dz.cpu().numpy().view(np.uint64)) == (u * u) % ((1 << (64 * n)) + 1)
+    bad = x.copy()
+    bad[n] = 2
+    with pytest.raises(fa.FlintAmdError):
+        fa.mpn_mulmod_2expp1(bad, x, n)

@@ -1,0 +1,160 @@
+"""GPU parity of plans and transformed operands (fft_small.h:530-700,
+op.c, the reference's t-op_mpn.c / t-op_nmod.c pattern): sums of products
+accumulated in transform space, then one inverse transform and one export,
+checked against Python integers and the oracle.
+
+- Integers: sum_i a_i b_i with len_bound = K, including a square (op_sqr),
+  operand sums (op_add before the product), submul and neg/add in the
+  product domain, at transform depths on both sides of 12 (one LDS array, or
+  column and row passes).
+- nmod: a dot product of K polynomial pairs mod n, exported whole and as a
+  window, against K oracle products summed mod n.
+- The domain rules: a product of a product, addmul into a primal op and
+  add of mixed domains are refused with errors.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import from_int, rand_limbs, to_int
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int64)).to("cuda:0")
+
+
+@pytest.mark.parametrize("an,bn,K", [(40, 30, 3), (700, 500, 4), (20000, 9000, 5),
+                                     (150000, 150000, 3)])
+def test_mpn_dot_product_in_transform_space(an, bn, K):
+    import torch
+    import flint_amd as fa
+    rng = np.random.default_rng(an + K)
+    P = fa.Plan.mpn(an, bn, len_bound=4 * K)
+    print("plan np=%d bits=%d depth=%d" % (P.s.np, P.s.bits, P.s.depth), flush=True)
+    A = [rand_limbs(rng, an) for _ in range(K)]
+    B = [rand_limbs(rng, bn) for _ in range(K)]
+    dA = [_dev(torch, a) for a in A]
+    dB = [_dev(torch, b) for b in B]
+    opA = [fa.Op(P) for _ in range(K)]
+    opB = [fa.Op(P) for _ in range(K)]
+    for k in range(K):
+        opA[k].fft_mpn(dA[k].data_ptr(), an)
+        opB[k].fft_mpn(dB[k].data_ptr(), bn)
+    Z = fa.Op(P)
+    Z.mul(opA[0], opB[0])
+    for k in range(1, K):
+        Z.addmul(opA[k], opB[k])
+    assert Z.domain == fa.OP_PRODUCT
+    Z.ifft()
+    zn = an + bn + 1
+    dz = torch.zeros(zn, dtype=torch.int64, device="cuda:0")
+    Z.export_mpn(dz.data_ptr(), zn)
+    torch.cuda.synchronize()
+    want = sum(to_int(a) * to_int(b) for a, b in zip(A, B))
+    assert to_int(dz.cpu().numpy().view(np.uint64)) == want % (1 << (64 * zn))
+
+    # (a0 + a1) b0 - a1 b0 = a0 b0 (op_add before the product, submul)
+    S = fa.Op(P)
+    S.add(opA[0], opA[1])
+    W = fa.Op(P)
+    W.mul(S, opB[0])
+    W.submul(opA[1], opB[0])
+    W.ifft()
+    W.export_mpn(dz.data_ptr(), zn)
+    torch.cuda.synchronize()
+    assert to_int(dz.cpu().numpy().view(np.uint64)) == to_int(A[0]) * to_int(B[0])
+
+    # a0^2 via op_sqr, and a0 b0 + a1 b1 - a1 b1 through neg and add in the
+    # product domain; a short export keeps the low limbs
+    if an == bn:
+        Q = fa.Op(P)
+        Q.sqr(opA[0])
+        Q.ifft()
+        dq = torch.zeros(2 * an, dtype=torch.int64, device="cuda:0")
+        Q.export_mpn(dq.data_ptr(), 2 * an)
+        torch.cuda.synchronize()
+        assert to_int(dq.cpu().numpy().view(np.uint64)) == to_int(A[0]) ** 2
+    X = fa.Op(P)
+    X.mul(opA[0], opB[0])
+    X.addmul(opA[1], opB[1])
+    Y = fa.Op(P)
+    Y.mul(opA[1], opB[1])
+    Y.neg(Y)
+    X.add(X, Y)
+    X.ifft()
+    short = (an + bn) // 2
+    ds = torch.zeros(short, dtype=torch.int64, device="cuda:0")
+    X.export_mpn(ds.data_ptr(), short)
+    torch.cuda.synchronize()
+    assert to_int(ds.cpu().numpy().view(np.uint64)) == \
+        (to_int(A[0]) * to_int(B[0])) % (1 << (64 * short))
+
+
+@pytest.mark.parametrize("n,ln,K", [((1 << 61) - 1, 300, 4), (0x0003F00000000001, 5000, 3),
+                                    (1000003, 40000, 6), (2**64 - 59, 2000, 2)])
+def test_nmod_dot_product_in_transform_space(n, ln, K):
+    import torch
+    import flint_amd as fa
+    rng = np.random.default_rng(ln)
+
+    def draw():
+        if n < 2**63:
+            return rng.integers(0, n, ln, dtype=np.uint64)
+        return np.array([int.from_bytes(rng.bytes(8), "little") % n for _ in range(ln)],
+                        dtype=np.uint64)
+    A = [draw() for _ in range(K)]
+    B = [draw() for _ in range(K)]
+    zn = 2 * ln - 1
+    P = fa.Plan.nmod(0, zn, zn, ln, K * ln, 2 * int(n).bit_length(), n)
+    print("plan np=%d offset=%d depth=%d" % (P.s.np, P.s.offset, P.s.depth), flush=True)
+    Z = fa.Op(P)
+    ta, tb = fa.Op(P), fa.Op(P)
+    for k in range(K):
+        da, db = _dev(torch, A[k]), _dev(torch, B[k])
+        ta.fft_nmod(da.data_ptr(), ln, n)
+        tb.fft_nmod(db.data_ptr(), ln, n)
+        if k == 0:
+            Z.mul(ta, tb)
+        else:
+            Z.addmul(ta, tb)
+        torch.cuda.synchronize()
+    Z.ifft()
+    dz = torch.zeros(zn, dtype=torch.int64, device="cuda:0")
+    Z.export_nmod(dz.data_ptr(), n)
+    lo, hi = zn // 3, zn // 3 + 101
+    dw = torch.zeros(hi - lo, dtype=torch.int64, device="cuda:0")
+    Z.export_nmod(dw.data_ptr(), n, lo, hi)
+    torch.cuda.synchronize()
+    want = np.zeros(zn, dtype=object)
+    for k in range(K):
+        want += O.nmod_poly_mul_mid(A[k], B[k], 0, zn, n).astype(object)
+    want = np.array([int(v) % n for v in want], dtype=np.uint64)
+    got = dz.cpu().numpy().view(np.uint64)
+    assert np.array_equal(got, want)
+    assert np.array_equal(dw.cpu().numpy().view(np.uint64), want[lo:hi])
+
+
+def test_domain_rules_are_enforced():
+    import torch
+    import flint_amd as fa
+    P = fa.Plan.mpn(100, 100, 2)
+    a = _dev(torch, np.arange(1, 101, dtype=np.uint64))
+    A, B, Z = fa.Op(P), fa.Op(P), fa.Op(P)
+    A.fft_mpn(a.data_ptr(), 100)
+    B.fft_mpn(a.data_ptr(), 100)
+    Z.mul(A, B)
+    with pytest.raises(fa.FlintAmdError):
+        Z.mul(Z, A)                    # a product of a product
+    with pytest.raises(fa.FlintAmdError):
+        A.addmul(A, B)                 # addmul into a primal op
+    with pytest.raises(fa.FlintAmdError):
+        Z.add(Z, A)                    # mixed domains
+    with pytest.raises(fa.FlintAmdError):
+        A.ifft()                       # inverse of a primal op
+    other = fa.Plan.mpn(5000, 5000, 2)
+    C = fa.Op(other)
+    with pytest.raises(fa.FlintAmdError):
+        C.mul(A, B)                    # ops of another plan
+    torch.cuda.synchronize()

@@ -1,0 +1,124 @@
+"""GPU parity of windowed and middle products.
+
+- Integer windows (_mpn_ctx_mpn_mul_range, flint_mpn_mulmid_fft_small): the
+  checks of the reference's t-mulmid.c on random shapes and windows -- the
+  deficit (exact - computed) mod 2^(64 zn) has no limb at or above index 2
+  and its limb 1 is at most an + bn, and zlo = 0 windows are exact.  This
+  build's windows are exact, so the deficit is zero everywhere.
+- Polynomial windows with wraparound (plan.c:47-89): when the operands and
+  zh fit in 2^floor(log2 zn) and zn - 2^d <= zl, the product is taken mod
+  X^(2^d) - 1 at half the transform length.  Middle products of that shape
+  (nmod and fmpz) equal the oracle's window, and a batch request reports
+  the shorter transform.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import rand_limbs, to_int
+
+pytestmark = pytest.mark.gpu
+
+
+def _deficit_ok(full, z, zlo, zhi, an, bn):
+    zn = zhi - zlo
+    d = (to_int(full[zlo:zhi]) - to_int(z)) % (1 << (64 * zn))
+    if zlo == 0:
+        return d == 0
+    if zn >= 2:
+        return (d >> 128) == 0 and ((d >> 64) & ((1 << 64) - 1)) <= an + bn
+    return True
+
+
+def test_mpn_windows_meet_mulmid_contract():
+    import flint_amd as fa
+    rng = np.random.default_rng(2024)
+    shapes = [(int(rng.integers(1, 31)), int(rng.integers(1, 31))) for _ in range(60)]
+    shapes += [(600, 17), (5000, 4000), (40001, 39999), (70000, 65536), (1 << 20, 3 << 18)]
+    for an, bn in shapes:
+        a, b = rand_limbs(rng, an), rand_limbs(rng, bn)
+        full = O.mpn_mul(a, b) if an + bn < 200000 else fa.mpn_mul(a, b)
+        for _ in range(3):
+            zlo = int(rng.integers(0, an + bn))
+            zhi = zlo + 1 + int(rng.integers(0, an + bn - zlo))
+            z = fa.mpn_mul_range(a, b, zlo, zhi)
+            assert _deficit_ok(full, z, zlo, zhi, an, bn), (an, bn, zlo, zhi)
+            assert np.array_equal(z, full[zlo:zhi]), (an, bn, zlo, zhi)
+
+
+def test_mpn_middle_product_device():
+    """the balanced middle product of {a, 2n-1} and {b, n}: limbs
+    [n-1, 2n+1) (flint_mpn_mulmid_n's band) on device pointers"""
+    import torch
+    import flint_amd as fa
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(8)
+    for n in (100, 9000, 200000):
+        a, b = rand_limbs(rng, 2 * n - 1), rand_limbs(rng, n)
+        full = fa.mpn_mul(a, b)
+        da = torch.from_numpy(a.view(np.int64)).to(dev)
+        db = torch.from_numpy(b.view(np.int64)).to(dev)
+        dz = torch.zeros(n + 2, dtype=torch.int64, device=dev)
+        s = torch.cuda.Stream(device=dev)
+        fa.mpn_mul_range_device(dz.data_ptr(), n - 1, 2 * n + 1, da.data_ptr(), 2 * n - 1,
+                                db.data_ptr(), n, s.cuda_stream)
+        s.synchronize()
+        assert np.array_equal(dz.cpu().numpy().view(np.uint64), full[n - 1:2 * n + 1]), n
+
+
+@pytest.mark.parametrize("n", [(1 << 61) - 1, 65537, 2**64 - 59])
+@pytest.mark.parametrize("m", [50, 3000, 70000])
+def test_nmod_middle_products_wrap(n, m):
+    """a of 2m - 1 coefficients, b of m: window [m - 1, 2m - 1) of a
+    product of 3m - 2 coefficients, computed mod X^(2^d) - 1"""
+    import flint_amd as fa
+    rng = np.random.default_rng(m)
+    if n < 2**63:
+        a = rng.integers(0, n, 2 * m - 1, dtype=np.uint64)
+        b = rng.integers(0, n, m, dtype=np.uint64)
+    else:
+        a = rng.integers(0, 2**63, 2 * m - 1, dtype=np.uint64)
+        b = rng.integers(0, 2**63, m, dtype=np.uint64)
+    zn = 3 * m - 2
+    zl, zh = m - 1, 2 * m - 1
+    want = O.nmod_poly_mul_mid(a, b, zl, zh, n)
+    assert np.array_equal(fa.nmod_poly_mul_mid(a, b, zl, zh, n), want)
+    # a window that does not allow the wrap is computed at full length
+    assert np.array_equal(fa.nmod_poly_mul_mid(a, b, 0, zh, n),
+                          O.nmod_poly_mul_mid(a, b, 0, zh, n))
+    # the batch reports the depth it used: 2^floor(log2 zn) when the wrap applies
+    d = zn.bit_length() - 1
+    wraps = max(len(a), len(b)) <= (1 << d) and zh <= (1 << d) <= zn <= zl + (1 << d)
+    R = (fa.Request * 1)()
+    z = np.zeros(zh - zl, dtype=np.uint64)
+    R[0].kind = fa.REQ_NMOD
+    R[0].a, R[0].b, R[0].z = a.ctypes.data, b.ctypes.data, z.ctypes.data
+    R[0].an, R[0].bn, R[0].zl, R[0].zh = len(a), len(b), zl, zh
+    R[0].mod = fa.nmod_init(n)
+    assert fa.lib().fa_batch_mul(ctypes.cast(R, ctypes.c_void_p), 1, 0) == 0
+    assert R[0].status == 1 and np.array_equal(z, want)
+    want_depth = max(4, d if wraps else (zn - 1).bit_length())
+    assert R[0].depth == want_depth, (R[0].depth, want_depth, wraps)
+
+
+def test_fmpz_middle_products_wrap():
+    import flint_amd as fa
+    rng = np.random.default_rng(31)
+    w, zw = 2, 4
+    for m in (40, 2500):
+        av = [int(v) for v in rng.integers(-2**62, 2**62, 2 * m - 1)]
+        bv = [int(v) for v in rng.integers(-2**62, 2**62, m)]
+
+        def pack(vals):
+            out = np.zeros(len(vals) * w, dtype=np.uint64)
+            for i, v in enumerate(vals):
+                out[i * w:(i + 1) * w] = np.frombuffer((v % (1 << 128)).to_bytes(16, "little"),
+                                                       dtype=np.uint64)
+            return out
+        a, b = pack(av), pack(bv)
+        zl, zh = m - 1, 2 * m - 1
+        got = fa.fmpz_poly_mul_mid(a, w, b, w, zl, zh, zw)
+        ok, want = O.fmpz_poly_mul_mid(a, w, b, w, zl, zh, zw)
+        assert ok and got is not None and np.array_equal(got, want), m
