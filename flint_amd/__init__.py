@@ -95,6 +95,18 @@ SIGNATURES = {
     "fa_mpn_mul_device_plan": (ctypes.c_int, [_VP, _VP, _s64, _VP, _s64, ctypes.c_int,
                                               ctypes.c_int, _VP]),
     "fa_nmod_poly_mul_mid": (ctypes.c_int, [_P64, _s64, _s64, _P64, _s64, _P64, _s64, NmodT]),
+    "fa_nmod_poly_mul_mid_device": (ctypes.c_int, [_VP, _s64, _s64, _VP, _s64, _VP, _s64, NmodT,
+                                                   _VP]),
+    "fa_nmod_poly_inv_series": (ctypes.c_int, [_P64, _P64, _s64, _s64, NmodT]),
+    "fa_nmod_poly_mulmid_repack": (ctypes.c_int, [_P64, _P64, _s64, _P64, _s64, _s64, _s64,
+                                                  NmodT]),
+    "fa_nmod_poly_mulmid_fft_small": (ctypes.c_int, [_P64, _P64, _s64, _P64, _s64, _s64, _s64,
+                                                     NmodT]),
+    "fa_nmod_poly_divrem": (ctypes.c_int, [_P64, _P64, _P64, _s64, _P64, _s64, NmodT]),
+    "fa_nmod_poly_divrem_precomp_init": (ctypes.c_int, [ctypes.POINTER(_VP), _P64, _s64, _s64,
+                                                        NmodT]),
+    "fa_nmod_poly_divrem_precomp": (ctypes.c_int, [_P64, _P64, _P64, _s64, _VP, NmodT]),
+    "fa_nmod_poly_divrem_precomp_clear": (None, [_VP]),
     "fa_fmpz_poly_mul_mid": (ctypes.c_int, [_P64, _u64, _s64, _s64, _P64, _u64, _s64, _P64, _u64,
                                             _s64]),
     "fa_sd_fft_ctx_init_prime": (ctypes.c_int, [ctypes.POINTER(SdFftCtx), _u64]),
@@ -149,6 +161,7 @@ SIGNATURES = {
     "fa_op_neg": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
     "fa_ifft": (ctypes.c_int, [_VP, _VP, _VP]),
     "fa_export_mpn": (ctypes.c_int, [_VP, _s64, _VP, _VP, _VP]),
+    "fa_export_mpn_signed": (ctypes.c_int, [_VP, _s64, _VP, _VP, _u64, _VP, _VP]),
     "fa_export_nmod": (ctypes.c_int, [_VP, _VP, NmodT, _VP, _VP]),
     "fa_export_nmod_range": (ctypes.c_int, [_VP, _VP, _u64, _u64, NmodT, _VP, _VP]),
     "fa_set_device_memory_limit": (None, [_u64]),
@@ -448,9 +461,9 @@ class Plan:
         self.ptr = ctypes.addressof(self.s)
 
     @classmethod
-    def mpn(cls, an_max, bn_max, len_bound=1):
+    def mpn(cls, an_max, bn_max, len_bound=1, signed=False):
         P = cls()
-        _check(lib().fa_plan_init_mpn(P.ptr, an_max, bn_max, len_bound, 0))
+        _check(lib().fa_plan_init_mpn(P.ptr, an_max, bn_max, len_bound, 1 if signed else 0))
         return P
 
     @classmethod
@@ -517,6 +530,10 @@ class Op:
     def export_mpn(self, z_ptr, zn, stream=None):
         return _check(lib().fa_export_mpn(z_ptr, zn, self.ptr, self.plan.ptr, stream))
 
+    def export_mpn_signed(self, z_ptr, zn, sign_ptr, nslots, stream=None):
+        return _check(lib().fa_export_mpn_signed(z_ptr, zn, sign_ptr, self.ptr, nslots,
+                                                 self.plan.ptr, stream))
+
     def export_nmod(self, z_ptr, n, zl=None, zh=None, stream=None):
         if zl is None:
             return _check(lib().fa_export_nmod(z_ptr, self.ptr, nmod_init(n), self.plan.ptr,
@@ -540,6 +557,77 @@ def nmod_poly_mul_mid(a, b, zl, zh, n):
     z = np.zeros(max(zh - zl, 1), dtype=np.uint64)
     _check(lib().fa_nmod_poly_mul_mid(_p(z), zl, zh, _p(a), len(a), _p(b), len(b), nmod_init(n)))
     return z[: max(zh - zl, 0)]
+
+
+def nmod_poly_mulmid_repack(a, b, znlo, zn, n):
+    """the small-modulus packed route: (1, z) when it applied, (0, None) if not"""
+    a = _arr(a)
+    b = a if b is a else _arr(b)
+    z = np.zeros(max(zn - znlo, 1), dtype=np.uint64)
+    rc = _check(lib().fa_nmod_poly_mulmid_repack(_p(z), _p(a), len(a), _p(b), len(b), znlo, zn,
+                                                 nmod_init(n)))
+    return (1, z[: zn - znlo]) if rc == 1 else (0, None)
+
+
+def nmod_poly_mulmid_fft_small(a, b, znlo, zn, n):
+    """_nmod_poly_mulmid_fft_small: packed route for n <= 23 when it applies"""
+    a = _arr(a)
+    b = a if b is a else _arr(b)
+    z = np.zeros(max(zn - znlo, 1), dtype=np.uint64)
+    _check(lib().fa_nmod_poly_mulmid_fft_small(_p(z), _p(a), len(a), _p(b), len(b), znlo, zn,
+                                               nmod_init(n)))
+    return z[: zn - znlo]
+
+
+def nmod_poly_inv_series(f, m, n):
+    """f^-1 mod x^m over Z/nZ (f[0] invertible)"""
+    f = _arr(f)
+    g = np.empty(m, dtype=np.uint64)
+    _check(lib().fa_nmod_poly_inv_series(_p(g), _p(f), len(f), m, nmod_init(n)))
+    return g
+
+
+def nmod_poly_divrem(a, b, n):
+    """_nmod_poly_divrem_mpn_ctx: (q, r) with a = b q + r, len(r) = len(b) - 1"""
+    a = _arr(a)
+    b = _arr(b)
+    q = np.empty(len(a) - len(b) + 1, dtype=np.uint64)
+    r = np.empty(max(len(b) - 1, 1), dtype=np.uint64)
+    _check(lib().fa_nmod_poly_divrem(_p(q), _p(r), _p(a), len(a), _p(b), len(b), nmod_init(n)))
+    return q, r[: len(b) - 1]
+
+
+class DivremPrecomp:
+    """nmod_poly_divrem_precomp: divisor b and its inverse to precision Bn
+    kept on the device"""
+
+    def __init__(self, b, Bn, n):
+        b = _arr(b)
+        self.h = _VP()
+        self.n = n
+        self.bn = len(b)
+        _check(lib().fa_nmod_poly_divrem_precomp_init(ctypes.byref(self.h), _p(b), len(b), Bn,
+                                                      nmod_init(n)))
+
+    def divrem(self, a):
+        """(q, r), or None when len(a) - len(b) + 1 exceeds Bn"""
+        a = _arr(a)
+        q = np.empty(len(a) - self.bn + 1, dtype=np.uint64)
+        r = np.empty(max(self.bn - 1, 1), dtype=np.uint64)
+        rc = _check(lib().fa_nmod_poly_divrem_precomp(_p(q), _p(r), _p(a), len(a), self.h,
+                                                      nmod_init(self.n)))
+        return (q, r[: self.bn - 1]) if rc == 1 else None
+
+    def close(self):
+        if self.h:
+            lib().fa_nmod_poly_divrem_precomp_clear(self.h)
+            self.h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def fmpz_poly_mul_mid(a, aw, b, bw, zl, zh, zw):

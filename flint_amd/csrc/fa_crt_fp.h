@@ -34,6 +34,7 @@ struct fa_crt_fp_consts {
     double rp[FA_NPRIMES];
     double cof[FA_NPRIMES][16];   // 24-bit digits of M / p_i
     double M[18];                 // 24-bit digits of M
+    double Mh[18];                // 24-bit digits of floor(M / 2)
 };
 
 void fa_crt_fp_consts_init(fa_crt_fp_consts* F, const fa_crt_consts& C);
@@ -54,8 +55,10 @@ FA_DEV double fa_crt_fp_carry(double* col)
     return carry;
 }
 
-// coefficient j -> its NL little-endian 64-bit limbs
-template <int NP>
+// coefficient j -> its NL little-endian 64-bit limbs.  MODE 0: c in
+// [0, M); for the centred lift c~ = c - M when c > M/2 (the signed exports,
+// export_signed.c:60-80), MODE 1 gives max(c~, 0) and MODE 2 max(-c~, 0).
+template <int NP, int MODE = 0>
 FA_DEV void fa_crt_fp_value(const double* __restrict__ Q, uint64_t stride, uint64_t j,
                             const fa_crt_fp_consts& F, uint64_t* L)
 {
@@ -95,6 +98,27 @@ FA_DEV void fa_crt_fp_value(const double* __restrict__ Q, uint64_t stride, uint6
 #pragma unroll
         for (int m = 0; m < ND; m++) col[m] += F.M[m];
         fa_crt_fp_carry<ND>(col);
+    }
+    if (MODE) {
+        // c > floor(M/2): the first differing digit from the top decides
+        int gt = 0, decided = 0;
+#pragma unroll
+        for (int m = ND - 1; m >= 0; m--) {
+            if (!decided && col[m] != F.Mh[m]) {
+                gt = col[m] > F.Mh[m];
+                decided = 1;
+            }
+        }
+        if (MODE == 1) {
+            if (gt) {
+#pragma unroll
+                for (int m = 0; m < ND; m++) col[m] = 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < ND; m++) col[m] = gt ? F.M[m] - col[m] : 0.0;
+            fa_crt_fp_carry<ND>(col);
+        }
     }
 #pragma unroll
     for (int l = 0; l < NL; l++) L[l] = 0;

@@ -653,6 +653,10 @@ void fa_crt_fp_consts_init(fa_crt_fp_consts* F, const fa_crt_consts& C)
         ml_digits24(F->cof[i], 16, C.cof[i], FA_CRT_MAXL);
     }
     ml_digits24(F->M, 18, C.M, FA_CRT_MAXL);
+    uint64_t h[FA_CRT_MAXL];
+    for (int l = 0; l < FA_CRT_MAXL; l++)
+        h[l] = (C.M[l] >> 1) | (l + 1 < FA_CRT_MAXL ? C.M[l + 1] << 63 : 0);
+    ml_digits24(F->Mh, 18, h, FA_CRT_MAXL);
 }
 
 uint64_t fa_crt_cop(int np, int i)

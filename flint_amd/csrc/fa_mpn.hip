@@ -763,7 +763,7 @@ FA_DEV uint32_t fa_seg_ballot(bool pred)
     return (uint32_t)(m >> (threadIdx.x & 32));
 }
 
-template <int NP>
+template <int NP, int MODE = 0>
 FA_DEV void fa_crt_fp_block(const double* __restrict__ Q, uint64_t stride,
                             const fa_crt_fp_consts& F, uint64_t zlen, int bits,
                             uint64_t* __restrict__ z, uint64_t zn, uint64_t ngroups,
@@ -784,7 +784,7 @@ FA_DEV void fa_crt_fp_block(const double* __restrict__ Q, uint64_t stride,
         const int64_t j = g0 * 8 + t;
         uint64_t L[NL];
         if (j >= 0 && (uint64_t)j < zlen) {
-            fa_crt_fp_value<NP>(Q, stride, (uint64_t)j, F, L);
+            fa_crt_fp_value<NP, MODE>(Q, stride, (uint64_t)j, F, L);
         } else {
 #pragma unroll
             for (int l = 0; l < NL; l++) L[l] = 0;
@@ -890,14 +890,14 @@ FA_DEV void fa_crt_fp_block(const double* __restrict__ Q, uint64_t stride,
     }
 }
 
-template <int NP>
+template <int NP, int MODE = 0>
 __global__ void __launch_bounds__(256) fa_k_crt_fp(const double* __restrict__ Q, uint64_t stride,
                                                    fa_crt_fp_consts F, uint64_t zlen, int bits,
                                                    uint64_t* __restrict__ z, uint64_t zn,
                                                    uint64_t ngroups, uint8_t* __restrict__ flags,
                                                    uint8_t* __restrict__ agg)
 {
-    fa_crt_fp_block<NP>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, blockIdx.x);
+    fa_crt_fp_block<NP, MODE>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, blockIdx.x);
 }
 
 // batched CRT of integer products: work item = job | block << 32
@@ -962,10 +962,18 @@ extern "C" FA_API void fa_set_crt_variant(int v) { fa_crt_fp_enabled = v; }
 template <int NP>
 static void launch_crt_fp(const double* Q, uint64_t stride, const fa_crt_fp_consts& F,
                           uint64_t zlen, int bits, uint64_t* z, uint64_t zn, uint64_t ngroups,
-                          uint8_t* flags, uint8_t* agg, uint64_t nblk, hipStream_t s)
+                          uint8_t* flags, uint8_t* agg, uint64_t nblk, hipStream_t s,
+                          int mode = 0)
 {
-    FA_LAUNCH("fa_k_crt_fp", fa_k_crt_fp<NP>, dim3((unsigned)nblk), dim3(256), 0, s, Q, stride, F,
-              zlen, bits, z, zn, ngroups, flags, agg);
+    if (mode == 1)
+        FA_LAUNCH("fa_k_crt_fp", (fa_k_crt_fp<NP, 1>), dim3((unsigned)nblk), dim3(256), 0, s, Q,
+                  stride, F, zlen, bits, z, zn, ngroups, flags, agg);
+    else if (mode == 2)
+        FA_LAUNCH("fa_k_crt_fp", (fa_k_crt_fp<NP, 2>), dim3((unsigned)nblk), dim3(256), 0, s, Q,
+                  stride, F, zlen, bits, z, zn, ngroups, flags, agg);
+    else
+        FA_LAUNCH("fa_k_crt_fp", fa_k_crt_fp<NP>, dim3((unsigned)nblk), dim3(256), 0, s, Q,
+                  stride, F, zlen, bits, z, zn, ngroups, flags, agg);
 }
 
 template <int NP>
@@ -977,21 +985,23 @@ static void launch_crt16(const double* Q, uint64_t stride, const fa_crt_consts& 
               stride, C, zlen, bits, z, zn, ngroups, flags, agg);
 }
 
+// mode: 0 the CRT value c in [0, M); 1, 2 the positive and negated
+// negative parts of the centred value (FP64 route only)
 static int crt16(int np, const double* Q, uint64_t stride, const fa_crt_consts& C, uint64_t zlen,
                  int bits, uint64_t* z, uint64_t zn, uint64_t ngroups, uint8_t* flags, uint8_t* agg,
-                 uint8_t* cin, uint64_t nblk, hipStream_t s)
+                 uint8_t* cin, uint64_t nblk, hipStream_t s, int mode = 0)
 {
-    if (fa_crt_fp_enabled) {
+    if (fa_crt_fp_enabled || mode) {
         fa_crt_fp_consts F;
         fa_crt_fp_consts_init(&F, C);
         switch (np) {
-        case 2: launch_crt_fp<2>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
-        case 3: launch_crt_fp<3>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
-        case 4: launch_crt_fp<4>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
-        case 5: launch_crt_fp<5>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
-        case 6: launch_crt_fp<6>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
-        case 7: launch_crt_fp<7>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
-        case 8: launch_crt_fp<8>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s); break;
+        case 2: launch_crt_fp<2>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode); break;
+        case 3: launch_crt_fp<3>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode); break;
+        case 4: launch_crt_fp<4>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode); break;
+        case 5: launch_crt_fp<5>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode); break;
+        case 6: launch_crt_fp<6>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode); break;
+        case 7: launch_crt_fp<7>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode); break;
+        case 8: launch_crt_fp<8>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode); break;
         default: return fa_set_error(FA_E_ARG, "crt: np %d", np);
         }
         FA_LAUNCH("fa_k_carry_fused", fa_k_carry_fused, dim3(1), dim3(1024), 0, s, flags, agg, z,
@@ -1374,7 +1384,7 @@ static int mpn_mul_device(uint64_t* z, const uint64_t* a, uint64_t an, const uin
 // CRT of zlen coefficients of `bits`-bit chunks into z[0, zn) (the low zn
 // limbs of the sum; zeros above it), carry flags in the thread's scratch
 int fa_crt_mpn_export(int np, const double* A, uint64_t stride, uint64_t zlen, int bits,
-                      uint64_t* z, uint64_t zn, hipStream_t stream)
+                      uint64_t* z, uint64_t zn, hipStream_t stream, int mode)
 {
     fa_thread_state* st = fa_state();
     if (!st) return FA_E_NODEV;
@@ -1391,7 +1401,8 @@ int fa_crt_mpn_export(int np, const double* A, uint64_t stride, uint64_t zlen, i
     uint8_t* cin = agg + ((nblk16 + 7) & ~UINT64_C(7));
     fa_crt_consts C;
     fa_crt_consts_init(&C, np, 0);
-    rc = crt16(np, A, stride, C, zlen, bits, z, zn, ngroups, flags, agg, cin, nblk16, stream);
+    rc = crt16(np, A, stride, C, zlen, bits, z, zn, ngroups, flags, agg, cin, nblk16, stream,
+               mode);
     if (rc) return rc;
     rc = fa_buffers_leave(st, stream);
     if (rc) return rc;
@@ -1626,6 +1637,62 @@ __global__ void __launch_bounds__(1024) fa_k_fold_2expp1(uint64_t* __restrict__ 
         z[i] = v;
     }
     if (m1 == n && m0 < m1) z[n] = c;   // the run holding limb n - 1
+}
+
+// z[0, zn) -= w[0, zn) as two's complement; then |z| and *sign = 1 when the
+// difference is negative (the top bit; the caller's zn leaves room for it),
+// 0 otherwise, and 0 for a zero difference (export_signed.c:588-599).  One
+// workgroup, runs of consecutive limbs per thread, chains by workgroup scans.
+__global__ void __launch_bounds__(1024) fa_k_signed_diff(uint64_t* __restrict__ z,
+                                                         const uint64_t* __restrict__ w,
+                                                         uint64_t zn, int32_t* __restrict__ sign)
+{
+    __shared__ uint8_t sg[1024], sp[1024];
+    __shared__ uint8_t sneg, snz;
+    const int t = threadIdx.x;
+    const uint64_t L = (zn + 1023) / 1024;
+    const uint64_t i0 = (uint64_t)t * L < zn ? (uint64_t)t * L : zn;
+    const uint64_t i1 = i0 + L < zn ? i0 + L : zn;
+    uint8_t G = 0, Q = 1;
+    for (uint64_t i = i0; i < i1; i++) {
+        const uint64_t x = z[i], y = w[i];
+        G = (uint8_t)((x < y) | ((x == y) & G));
+        Q = (uint8_t)(Q & (x == y));
+    }
+    uint8_t b = fa_wg_chain_in(G, Q, 0, sg, sp);
+    for (uint64_t i = i0; i < i1; i++) {
+        const uint64_t x = z[i], y = w[i];
+        z[i] = x - y - b;
+        b = (uint8_t)(x < y || (x == y && b));
+    }
+    if (t == 0) snz = 0;
+    __syncthreads();
+    if (i1 == zn && i0 < i1) sneg = (uint8_t)(z[zn - 1] >> 63);
+    __syncthreads();
+    const bool neg = sneg;
+    if (neg) {
+        // |D| = ~D + 1
+        uint8_t P = 1;
+        for (uint64_t i = i0; i < i1; i++) P = (uint8_t)(P & (z[i] == 0));   // ~z all ones
+        uint8_t c = fa_wg_chain_in(0, P, 1, sg, sp);
+        for (uint64_t i = i0; i < i1; i++) {
+            const uint64_t v = ~z[i] + c;
+            c = (uint8_t)(c && v == 0);
+            z[i] = v;
+        }
+    }
+    uint8_t nz = 0;
+    for (uint64_t i = i0; i < i1; i++) nz |= z[i] != 0;
+    if (nz) snz = 1;
+    __syncthreads();
+    if (t == 0) *sign = (neg && snz) ? 1 : 0;
+}
+
+int fa_signed_diff_launch(uint64_t* z, const uint64_t* w, uint64_t zn, int32_t* sign,
+                          hipStream_t s)
+{
+    FA_LAUNCH("fa_k_signed_diff", fa_k_signed_diff, dim3(1), dim3(1024), 0, s, z, w, zn, sign);
+    return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "signed diff launch");
 }
 
 // z[0, n] = x y mod 2^(64 n) + 1 on device operands of n + 1 limbs with

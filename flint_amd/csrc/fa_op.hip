@@ -27,7 +27,9 @@ int fa_forward_fast(int depth, const fa_operand& oa, int np, double* A, uint64_t
 int fa_inverse_fast(int depth, int np, double* A, uint64_t stride, const fa_prime_set& S,
                     hipStream_t s);
 int fa_crt_mpn_export(int np, const double* A, uint64_t stride, uint64_t zlen, int bits,
-                      uint64_t* z, uint64_t zn, hipStream_t stream);
+                      uint64_t* z, uint64_t zn, hipStream_t stream, int mode);
+int fa_signed_diff_launch(uint64_t* z, const uint64_t* w, uint64_t zn, int32_t* sign,
+                          hipStream_t s);
 int fa_crt_nmod_launch(const double* A, uint64_t stride, const fa_crt_consts& C, uint64_t zl,
                        uint64_t cnt, uint64_t n, uint64_t* dz, hipStream_t s);
 int fa_nmod_primes_bound(const fa_tables* T, uint64_t len_bound, uint64_t prod_bits, uint64_t n,
@@ -173,8 +175,13 @@ extern "C" int fa_plan_init_mpn(fa_plan_struct* P, uint64_t an_max, uint64_t bn_
 {
     if (!P || !an_max || !bn_max || !len_bound)
         return fa_set_error(FA_E_ARG, "fa_plan_init_mpn: bad arguments");
-    if (is_signed)
-        return fa_set_error(FA_E_ARG, "fa_plan_init_mpn: signed exports are not built");
+    // the centred lift of the signed exports needs one spare factor of two
+    // (plan.c:443-446)
+    if (is_signed) {
+        if (len_bound > UINT64_MAX / 2)
+            return fa_set_error(FA_E_RANGE, "fa_plan_init_mpn: len_bound too large");
+        len_bound *= 2;
+    }
     memset(P, 0, sizeof(*P));
     uint64_t best_cost = UINT64_MAX;
     for (int np = 2; np <= FA_NPRIMES; np++) {
@@ -212,6 +219,7 @@ extern "C" int fa_plan_init_mpn(fa_plan_struct* P, uint64_t an_max, uint64_t bn_
     P->an_max = an_max;
     P->bn_max = bn_max;
     P->len_bound = len_bound;
+    P->pad = is_signed ? 1 : 0;         // signed: exports centre the coefficients
     return FA_OK;
 }
 
@@ -452,7 +460,40 @@ extern "C" int fa_export_mpn(uint64_t* z, int64_t zn, const fa_op_struct* X,
     if (!compatible(X, P) || P->kind != 0 || X->domain != FA_OP_COEFF || !z || zn <= 0)
         return fa_set_error(FA_E_ARG, "fa_export_mpn: op not inverse-transformed, or bad output");
     return fa_crt_mpn_export(P->np, X->data, X->stride, P->zn, P->bits, z, (uint64_t)zn,
-                             stream_of(stream));
+                             stream_of(stream), 0);
+}
+
+// fft_small_export_mpn_signed (export_signed.c:367-606): the first nslots
+// coefficients, each centred to (-M/2, M/2], summed at their bit offsets:
+// z[0, zn) = |value| (zn must leave a spare top bit, 64 zn >= nslots bits
+// + 64 (limbs of M) + 2 as in the reference), *sign (device int32) = 1 when
+// the value is negative.  The positive and the negated negative parts are
+// two CRT passes of the product path; one workgroup subtracts them.
+extern "C" int fa_export_mpn_signed(uint64_t* z, int64_t zn, int32_t* sign,
+                                    const fa_op_struct* X, uint64_t nslots,
+                                    const fa_plan_struct* P, void* stream)
+{
+    if (!compatible(X, P) || P->kind != 0 || X->domain != FA_OP_COEFF || !z || zn <= 0 ||
+        !sign || nslots > P->zn)
+        return fa_set_error(FA_E_ARG, "fa_export_mpn_signed: bad arguments");
+    if (!P->pad)
+        return fa_set_error(FA_E_ARG, "fa_export_mpn_signed: the plan is not signed");
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    hipStream_t s = stream_of(stream);
+    // the negative part goes to the thread's block buffer, kept apart from
+    // the CRT's scratch
+    void* tb;
+    int rc = fa_tbuf_reserve(st, (size_t)zn * 8, &tb, s);
+    if (rc) return rc;
+    uint64_t* w = (uint64_t*)tb;
+    rc = fa_crt_mpn_export(P->np, X->data, X->stride, nslots, P->bits, z, (uint64_t)zn, s, 1);
+    if (rc) return rc;
+    rc = fa_crt_mpn_export(P->np, X->data, X->stride, nslots, P->bits, w, (uint64_t)zn, s, 2);
+    if (rc) return rc;
+    rc = fa_signed_diff_launch(z, w, (uint64_t)zn, sign, s);
+    if (rc) return rc;
+    return fa_buffers_leave(st, s);
 }
 
 // fft_small_export_nmod_range (nmod_poly_mul.c:860-901): coefficients

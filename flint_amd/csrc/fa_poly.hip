@@ -585,6 +585,68 @@ extern "C" int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh, const u
     return FA_OK;
 }
 
+int fa_crt_nmod_launch(const double* A, uint64_t stride, const fa_crt_consts& C, uint64_t zl,
+                       uint64_t cnt, uint64_t n, uint64_t* dz, hipStream_t s);
+
+__global__ void __launch_bounds__(256) fa_k_fill_zero(uint64_t* __restrict__ z, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = 0;
+}
+
+// _nmod_poly_mul_mid on device pointers, enqueued on `stream` (NULL: the
+// thread's): z[0, zh - zl) = coefficients [zl, zh) of a*b mod n, zeros past
+// the product.  Products of more than one transform's length are refused.
+extern "C" int fa_nmod_poly_mul_mid_device(uint64_t* z, int64_t zl, int64_t zh,
+                                           const uint64_t* a, int64_t an, const uint64_t* b,
+                                           int64_t bn, fa_nmod_t mod, void* stream)
+{
+    if (an <= 0 || bn <= 0 || !a || !b || mod.n == 0 || zl < 0)
+        return fa_set_error(FA_E_ARG, "fa_nmod_poly_mul_mid_device: bad arguments");
+    if (zl >= zh) return FA_OK;
+    if (!z) return fa_set_error(FA_E_ARG, "fa_nmod_poly_mul_mid_device: null output");
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    hipStream_t s = stream ? (hipStream_t)stream : st->stream;
+    const int64_t zn = an + bn - 1;
+    const uint64_t n = mod.n;
+    // zeros past the product, and everything when n = 1
+    const int64_t zero_from = n == 1 ? zl : (zh > zn ? (zl > zn ? zl : zn) : zh);
+    if (zero_from < zh) {
+        const uint64_t cnt0 = (uint64_t)(zh - zero_from);
+        FA_LAUNCH("fa_k_fill_zero", fa_k_fill_zero, dim3((unsigned)((cnt0 + 255) / 256)),
+                  dim3(256), 0, s, z + (zero_from - zl), cnt0);
+        zh = zero_from;
+    }
+    if (zl >= zh) return fa_buffers_leave(st, s);
+    const int depth = fa_poly_window_depth((uint64_t)zl, (uint64_t)zh, (uint64_t)zn,
+                                           (uint64_t)(an > bn ? an : bn));
+    if (depth > FA_MAX_DEPTH)
+        return fa_set_error(FA_E_RANGE, "fa_nmod_poly_mul_mid_device: %lld coefficients",
+                            (long long)zn);
+    fa_tables* T = fa_get_tables(depth < 20 ? 20 : depth);
+    if (!T) return FA_E_NODEV;
+    int np, offset;
+    int rc = fa_nmod_primes(T, (uint64_t)(an < bn ? an : bn), n, &np, &offset);
+    if (rc) return rc;
+    fa_crt_consts C;
+    fa_crt_consts_init(&C, np, offset);
+    const size_t tr = (size_t)np * fa_res_stride(depth) * sizeof(double);
+    void* buf;
+    rc = fa_scratch_reserve(st, 2 * tr + 256, &buf, s);
+    if (rc) return rc;
+    double* A = (double*)buf;
+    double* B = (double*)((char*)buf + tr);
+    fa_pack_desc pa = {FA_PACK_U64, a, (uint64_t)an, 0, 1};
+    fa_pack_desc pb = {FA_PACK_U64, b, (uint64_t)bn, 0, 1};
+    rc = fa_convolve(st, T, np, offset, depth, &pa, &pb, A, B, s);
+    if (rc) return rc;
+    rc = fa_crt_nmod_launch(A, fa_res_stride(depth), C, (uint64_t)zl, (uint64_t)(zh - zl), n, z,
+                            s);
+    if (rc) return rc;
+    return fa_buffers_leave(st, s);
+}
+
 // _fmpz_vec_max_bits: max bit size of |c|, negated if some c < 0
 int64_t fa_vec_max_bits(const uint64_t* v, uint64_t w, int64_t n)
 {

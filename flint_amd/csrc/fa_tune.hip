@@ -11,6 +11,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "fa_internal.h"
 #include "../../include/flint_amd.h"
 
@@ -71,8 +74,11 @@ static thread_local size_t tune_words = 0;
 
 // times every candidate of an x bn on the device alone: after a warm-up
 // call, K back-to-back products (K = 2^21 / (an + bn), between 1 and 64)
-// are captured into one hipGraph, and each of `reps` replays is timed with
-// HIP events; ms[j] = the mean time per product, -1 for invalid candidates.
+// are captured into one hipGraph, replayed once untimed, then `reps` times
+// with HIP events around each replay; ms[j] = the median replay time per
+// product (robust to the single slow replays a shared device shows), -1 for
+// invalid candidates.  The first call of a process first keeps the device
+// busy for about 50 ms so that its clocks have risen before any timing.
 // Host launch overhead is outside the measurement.
 extern "C" int fa_autotune_mpn(int64_t an, int64_t bn, int reps, float* ms)
 {
@@ -107,6 +113,24 @@ extern "C" int fa_autotune_mpn(int64_t an, int64_t bn, int reps, float* ms)
         (void)hipStreamDestroy(cs);
         return fa_set_error(FA_E_HIP, "fa_autotune_mpn: events");
     }
+    static thread_local bool warmed = false;
+    if (!warmed) {
+        // clocks up: repeated products of the first valid candidate
+        for (int j = 0; j < FA_NCAND; j++) {
+            if (!cand[2 * j]) continue;
+            float t = 0;
+            (void)hipEventRecord(e0, cs);
+            for (int r = 0; r < 1000 && t < 50.f && !rc; r++) {
+                rc = fa_mpn_mul_device_plan(z, a, an, b, bn, cand[2 * j], cand[2 * j + 1], cs);
+                (void)hipEventRecord(e1, cs);
+                if (hipEventSynchronize(e1) != hipSuccess) break;
+                (void)hipEventElapsedTime(&t, e0, e1);
+            }
+            break;
+        }
+        warmed = true;
+    }
+    std::vector<float> times((size_t)reps);
     for (int j = 0; j < FA_NCAND && !rc; j++) {
         ms[j] = -1.f;
         if (!cand[2 * j]) continue;
@@ -129,7 +153,6 @@ extern "C" int fa_autotune_mpn(int64_t an, int64_t bn, int reps, float* ms)
         if (!rc && (ce != hipSuccess || !g ||
                     hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess))
             rc = fa_set_error(FA_E_HIP, "tune graph");
-        float tot = 0.f;
         if (!rc && hipGraphLaunch(ex, cs) != hipSuccess) rc = fa_set_error(FA_E_HIP, "tune launch");
         for (int r = 0; r < reps && !rc; r++) {
             (void)hipEventRecord(e0, cs);
@@ -144,11 +167,17 @@ extern "C" int fa_autotune_mpn(int64_t an, int64_t bn, int reps, float* ms)
             }
             float t = 0;
             (void)hipEventElapsedTime(&t, e0, e1);
-            tot += t;
+            times[(size_t)r] = t;
         }
         if (ex) (void)hipGraphExecDestroy(ex);
         if (g) (void)hipGraphDestroy(g);
-        if (!rc) ms[j] = tot / (float)(reps * K);
+        if (!rc) {
+            std::sort(times.begin(), times.end());
+            const float med = (reps & 1) ? times[(size_t)reps / 2]
+                                         : 0.5f * (times[(size_t)reps / 2 - 1] +
+                                                   times[(size_t)reps / 2]);
+            ms[j] = med / (float)K;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -167,6 +167,48 @@ FA_API int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh,
                          const uint64_t* a, int64_t an,
                          const uint64_t* b, int64_t bn, fa_nmod_t mod);
 
+/* the same on device arrays, enqueued on `stream` (NULL: the thread's);
+   products of at most one transform (2^24 coefficients) */
+FA_API int fa_nmod_poly_mul_mid_device(uint64_t* z, int64_t zl, int64_t zh,
+                                       const uint64_t* a, int64_t an,
+                                       const uint64_t* b, int64_t bn, fa_nmod_t mod,
+                                       void* stream);
+
+/* _nmod_poly_mulmid_fft_small_repack (nmod_poly/mullow_fft_small.c:
+   340-411): for moduli n <= 23, coefficient pairs packed into one
+   coefficient mod the FFT prime 1108307720798209 and multiplied at half
+   length with that one prime.  z[0, zn-znlo) = coefficients [znlo, zn) of
+   a*b mod n; returns 1 when the packed route applied, 0 when its bounds do
+   not hold (z untouched), negative on error. */
+FA_API int fa_nmod_poly_mulmid_repack(uint64_t* z, const uint64_t* a, int64_t an,
+                                      const uint64_t* b, int64_t bn, int64_t znlo, int64_t zn,
+                                      fa_nmod_t mod);
+/* _nmod_poly_mulmid_fft_small (mullow_fft_small.c:419-424): the packed
+   route when it applies, else _nmod_poly_mul_mid_default_mpn_ctx */
+FA_API int fa_nmod_poly_mulmid_fft_small(uint64_t* z, const uint64_t* a, int64_t an,
+                                         const uint64_t* b, int64_t bn, int64_t znlo,
+                                         int64_t zn, fa_nmod_t mod);
+
+/* _nmod_poly_inv_series as used by _nmod_poly_divrem_mpn_ctx
+   (nmod_poly_mul.c:1246): g[0, m) = f^-1 mod x^m, f[0] invertible mod n;
+   Newton iteration on the device products */
+FA_API int fa_nmod_poly_inv_series(uint64_t* g, const uint64_t* f, int64_t fn, int64_t m,
+                                   fa_nmod_t mod);
+/* _nmod_poly_divrem_mpn_ctx, fft_small.h:776-782, nmod_poly_mul.c:1234-1262:
+   q[0, an-bn+1) and r[0, bn-1) with a = b q + r, an >= bn >= 2, the leading
+   coefficient of b invertible mod n (FA_E_ARG otherwise) */
+FA_API int fa_nmod_poly_divrem(uint64_t* q, uint64_t* r, const uint64_t* a, int64_t an,
+                               const uint64_t* b, int64_t bn, fa_nmod_t mod);
+/* nmod_poly_divrem_precomp (fft_small.h:800-835, nmod_poly_mul.c:1265-1313):
+   the divisor and its inverse to precision Bn kept on the device; divrem
+   returns 1, or 0 when an - bn + 1 > Bn */
+typedef struct fa_divrem_precomp_s* fa_divrem_precomp;
+FA_API int fa_nmod_poly_divrem_precomp_init(fa_divrem_precomp* M, const uint64_t* b, int64_t bn,
+                                            int64_t Bn, fa_nmod_t mod);
+FA_API int fa_nmod_poly_divrem_precomp(uint64_t* q, uint64_t* r, const uint64_t* a, int64_t an,
+                                       fa_divrem_precomp M, fa_nmod_t mod);
+FA_API void fa_nmod_poly_divrem_precomp_clear(fa_divrem_precomp M);
+
 /* _fmpz_poly_mul_mid_mpn_ctx, fmpz_poly_mul.c:319-401.  Coefficients are
    passed as fixed-width two's-complement limb vectors (aw, bw, zw limbs per
    coefficient), the form fmpz_get_signed_ui_array / fmpz_set_signed_ui_array
@@ -189,7 +231,8 @@ typedef struct {
     int32_t kind;            /* 0: naturals (mpn), 1: polynomials mod n */
     int32_t np, offset;      /* primes offset .. offset + np - 1 */
     int32_t bits;            /* chunk bits (kind 0) */
-    int32_t depth, pad;      /* transform length 2^depth */
+    int32_t depth;           /* transform length 2^depth */
+    int32_t pad;             /* kind 0: 1 when made for signed exports */
     uint64_t stride;         /* doubles per prime in an op */
     uint64_t zl, zh, zn;     /* output window; zn = coefficients of the convolution */
     uint64_t an_max, bn_max, len_bound;
@@ -210,8 +253,9 @@ typedef struct {
 } fa_op_struct;
 
 /* fft_small_plan_init_mpn, plan.c:443: exact for up to len_bound
-   accumulated products of an_max x bn_max limbs (is_signed must be 0:
-   signed exports are not built) */
+   accumulated products of an_max x bn_max limbs; is_signed leaves the spare
+   factor of two the centred signed export needs (the plan's pad field
+   records it) */
 FA_API int fa_plan_init_mpn(fa_plan_struct* P, uint64_t an_max, uint64_t bn_max,
                             uint64_t len_bound, int is_signed);
 /* fft_small_plan_init_nmod, plan.c:400: len_bound products of prod_bits-bit
@@ -255,6 +299,12 @@ FA_API int fa_ifft(fa_op_struct* X, const fa_plan_struct* P, void* stream);
    integer into device z */
 FA_API int fa_export_mpn(uint64_t* z, int64_t zn, const fa_op_struct* X,
                          const fa_plan_struct* P, void* stream);
+/* fft_small_export_mpn_signed (export_signed.c:367-606), signed plans:
+   the first nslots coefficients centred to (-M/2, M/2] and summed at their
+   bit offsets; z = zn limbs of |value| (64 zn >= nslots bits + 64 (limbs of
+   M) + 2), *sign (a device int32) = 1 when the value is negative */
+FA_API int fa_export_mpn_signed(uint64_t* z, int64_t zn, int32_t* sign, const fa_op_struct* X,
+                                uint64_t nslots, const fa_plan_struct* P, void* stream);
 /* fft_small_export_nmod / _range (nmod_poly_mul.c:800-901): coefficients
    of the plan's window, or of [zl, zh), mod n into device z */
 FA_API int fa_export_nmod(uint64_t* z, const fa_op_struct* X, fa_nmod_t mod,

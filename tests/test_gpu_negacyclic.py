@@ -13,6 +13,12 @@ from tests.helpers import from_int, to_int
 pytestmark = pytest.mark.gpu
 
 
+def _mod_fermat(x, N):
+    """x mod 2^N + 1 for 0 <= x <= 2^(2N) by 2^N = -1 (no long division)"""
+    r = (x & ((1 << N) - 1)) - (x >> N)
+    return r + (1 << N) + 1 if r < 0 else r
+
+
 def _norm_rand(rng, n):
     N = 64 * n
     if rng.integers(0, 8) == 0:
@@ -20,12 +26,11 @@ def _norm_rand(rng, n):
     return int.from_bytes(rng.bytes(8 * n), "little")
 
 
-@pytest.mark.parametrize("n", [1, 2, 7, 64, 1000, 33000, 300000])
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 1000, 33000, 100000])
 def test_mulmod_2expp1_matches_integers(n):
     import flint_amd as fa
     rng = np.random.default_rng(n)
     N = 64 * n
-    F = (1 << N) + 1
     edges = [0, 1, (1 << N) - 1, 1 << N, 2]
     pairs = [(u, v) for u in edges for v in edges]
     pairs += [(_norm_rand(rng, n), _norm_rand(rng, n)) for _ in range(6)]
@@ -34,11 +39,11 @@ def test_mulmod_2expp1_matches_integers(n):
     for u, v in pairs:
         x, y = from_int(u, n + 1), from_int(v, n + 1)
         z = fa.mpn_mulmod_2expp1(x, y, n)
-        assert to_int(z) == (u * v) % F, (n, u.bit_length(), v.bit_length())
+        assert to_int(z) == _mod_fermat(u * v, N), (n, u.bit_length(), v.bit_length())
     # squares (one operand array)
     u = _norm_rand(rng, n)
     x = from_int(u, n + 1)
-    assert to_int(fa.mpn_mulmod_2expp1(x, x, n)) == (u * u) % F
+    assert to_int(fa.mpn_mulmod_2expp1(x, x, n)) == _mod_fermat(u * u, N)
 
 
 def test_mulmod_2expp1_device_square_and_rejects():
@@ -54,7 +59,7 @@ def test_mulmod_2expp1_device_square_and_rejects():
     assert fa.lib().fa_mpn_mulmod_2expp1_device(dz.data_ptr(), dx.data_ptr(), dx.data_ptr(), n,
                                                 s.cuda_stream) == 0
     s.synchronize()
-    assert to_int(dz.cpu().numpy().view(np.uint64)) == (u * u) % ((1 << (64 * n)) + 1)
+    assert to_int(dz.cpu().numpy().view(np.uint64)) == _mod_fermat(u * u, 64 * n)
     bad = x.copy()
     bad[n] = 2
     with pytest.raises(fa.FlintAmdError):

@@ -158,3 +158,47 @@ def test_domain_rules_are_enforced():
     with pytest.raises(fa.FlintAmdError):
         C.mul(A, B)                    # ops of another plan
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("an,bn", [(30, 20), (3000, 2500), (70000, 40000)])
+def test_signed_export(an, bn):
+    """a0 b0 - a1 b1 and its negation through submul and the signed export
+    (export_signed.c): |value| and the sign, against Python integers; a zero
+    difference has sign 0"""
+    import torch
+    import flint_amd as fa
+    rng = np.random.default_rng(an)
+    P = fa.Plan.mpn(an, bn, len_bound=2, signed=True)
+    assert P.s.pad == 1
+    A = [rand_limbs(rng, an) for _ in range(2)]
+    B = [rand_limbs(rng, bn) for _ in range(2)]
+    ops = []
+    for k in range(2):
+        oa, ob = fa.Op(P), fa.Op(P)
+        oa.fft_mpn(_dev(torch, A[k]).data_ptr(), an)
+        ob.fft_mpn(_dev(torch, B[k]).data_ptr(), bn)
+        ops.append((oa, ob))
+    nl = -(-50 * P.s.np // 64)
+    zn = (P.s.zn * P.s.bits + 64 * nl + 2 + 63) // 64 + 1
+    dz = torch.zeros(zn, dtype=torch.int64, device="cuda:0")
+    ds = torch.full((1,), 7, dtype=torch.int32, device="cuda:0")
+    for first, second in ((0, 1), (1, 0), (0, 0)):
+        Z = fa.Op(P)
+        Z.mul(ops[first][0], ops[first][1])
+        Z.submul(ops[second][0], ops[second][1])
+        Z.ifft()
+        Z.export_mpn_signed(dz.data_ptr(), zn, ds.data_ptr(), P.s.zn)
+        torch.cuda.synchronize()
+        want = to_int(A[first]) * to_int(B[first]) - to_int(A[second]) * to_int(B[second])
+        got = to_int(dz.cpu().numpy().view(np.uint64))
+        sign = int(ds.cpu()[0])
+        assert (-got if sign else got) == want, (first, second)
+        assert sign == (1 if want < 0 else 0)
+    # an unsigned plan refuses the signed export
+    Pu = fa.Plan.mpn(an, bn, len_bound=2)
+    U = fa.Op(Pu)
+    U.fft_mpn(_dev(torch, A[0]).data_ptr(), an)
+    U.sqr(U)
+    U.ifft()
+    with pytest.raises(fa.FlintAmdError):
+        U.export_mpn_signed(dz.data_ptr(), zn, ds.data_ptr(), Pu.s.zn)

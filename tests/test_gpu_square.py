@@ -39,8 +39,8 @@ def test_mpn_square_all_ones_and_device_pointers():
     dev = torch.device("cuda:0")
     for n in (4096, 70000, 1 << 21):
         a = np.full(n, np.uint64(2**64 - 1))
-        want = (1 << (64 * n)) - 1
-        want = want * want
+        N = 64 * n
+        want = (1 << (2 * N)) - (1 << (N + 1)) + 1       # (2^N - 1)^2
         got = fa.mpn_mul(a, a)
         assert to_int(got) == want, n
         # the device entry point with one pointer for both operands
