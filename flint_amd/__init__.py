@@ -98,6 +98,8 @@ SIGNATURES = {
     "fa_nmod_poly_mul_mid_device": (ctypes.c_int, [_VP, _s64, _s64, _VP, _s64, _VP, _s64, NmodT,
                                                    _VP]),
     "fa_nmod_poly_inv_series": (ctypes.c_int, [_P64, _P64, _s64, _s64, NmodT]),
+    "fa_fmpz_poly_mul_mid_device": (ctypes.c_int, [_VP, _u64, _s64, _s64, _VP, _u64, _s64, _VP,
+                                                   _u64, _s64, _s64, _s64, ctypes.c_int, _VP]),
     "fa_nmod_poly_mulmid_repack": (ctypes.c_int, [_P64, _P64, _s64, _P64, _s64, _s64, _s64,
                                                   NmodT]),
     "fa_nmod_poly_mulmid_fft_small": (ctypes.c_int, [_P64, _P64, _s64, _P64, _s64, _s64, _s64,

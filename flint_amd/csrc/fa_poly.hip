@@ -647,6 +647,75 @@ extern "C" int fa_nmod_poly_mul_mid_device(uint64_t* z, int64_t zl, int64_t zh,
     return fa_buffers_leave(st, s);
 }
 
+// _fmpz_poly_mul_mid on device arrays (w-limb two's complement
+// coefficients, abits/bbits their _fmpz_vec_max_bits), enqueued on `stream`:
+// 1 computed, 0 declined (more than eight primes, fmpz_poly_mul.c:377-379)
+// unless `ks` asks for the Kronecker route then (fmpz_poly/mul.c:160)
+extern "C" int fa_fmpz_poly_mul_mid_device(uint64_t* z, uint64_t zw, int64_t zl, int64_t zh,
+                                           const uint64_t* a, uint64_t aw, int64_t an,
+                                           const uint64_t* b, uint64_t bw, int64_t bn,
+                                           int64_t abits, int64_t bbits, int ks, void* stream)
+{
+    if (an <= 0 || bn <= 0 || !a || !b || aw == 0 || bw == 0 || zw == 0 || zl < 0)
+        return fa_set_error(FA_E_ARG, "fa_fmpz_poly_mul_mid_device: bad arguments");
+    if (zl >= zh) return 1;
+    if (!z) return fa_set_error(FA_E_ARG, "fa_fmpz_poly_mul_mid_device: null output");
+    fa_thread_state* st = fa_state();
+    if (!st) return FA_E_NODEV;
+    hipStream_t s = stream ? (hipStream_t)stream : st->stream;
+    const int64_t zn = an + bn - 1;
+    if (zh > zn) {
+        const int64_t s0 = zl > zn ? zl : zn;
+        const uint64_t w = (uint64_t)(zh - s0) * zw;
+        FA_LAUNCH("fa_k_fill_zero", fa_k_fill_zero, dim3((unsigned)((w + 255) / 256)), dim3(256),
+                  0, s, z + (uint64_t)(s0 - zl) * zw, w);
+        if (zl >= zn) return fa_buffers_leave(st, s) ? FA_E_HIP : 1;
+        zh = zn;
+    }
+    const int np = fa_fmpz_primes((uint64_t)bn, abits, bbits);
+    const uint64_t cnt = (uint64_t)(zh - zl);
+    int rc;
+    if (!np) {
+        if (!ks) return 0;
+        const uint64_t ws = fa_ks_scratch_words(abits, bbits, (uint64_t)an, (uint64_t)bn);
+        void* hb;
+        rc = fa_hbuf_reserve(st, ws * 8, &hb, s);
+        if (rc) return rc;
+        rc = fa_ks_device(z, zw, (uint64_t)zl, cnt, a, aw, (uint64_t)an, b, bw, (uint64_t)bn,
+                          abits, bbits, (uint64_t*)hb, s);
+        if (rc) return rc;
+        rc = fa_buffers_leave(st, s);
+        return rc ? rc : 1;
+    }
+    const int depth = fa_poly_window_depth((uint64_t)zl, (uint64_t)zh, (uint64_t)zn,
+                                           (uint64_t)(an > bn ? an : bn));
+    if (depth > FA_MAX_DEPTH)
+        return fa_set_error(FA_E_RANGE, "fa_fmpz_poly_mul_mid_device: %lld coefficients",
+                            (long long)zn);
+    fa_tables* T = fa_get_tables(depth < 20 ? 20 : depth);
+    if (!T) return FA_E_NODEV;
+    fa_crt_consts C;
+    fa_crt_consts_init(&C, np, 0);
+    const size_t tr = (size_t)np * fa_res_stride(depth) * sizeof(double);
+    void* buf;
+    rc = fa_scratch_reserve(st, 2 * tr + 256, &buf, s);
+    if (rc) return rc;
+    double* A = (double*)buf;
+    double* B = (double*)((char*)buf + tr);
+    fa_pack_desc pa = {FA_PACK_S64W, a, (uint64_t)an, 0, (int)aw};
+    fa_pack_desc pb = {FA_PACK_S64W, b, (uint64_t)bn, 0, (int)bw};
+    rc = fa_convolve(st, T, np, 0, depth, &pa, &pb, A, B, s);
+    if (rc) return rc;
+#define FA_L(NL)                                                                            \
+    FA_LAUNCH("fa_k_crt_fmpz", fa_k_crt_fmpz<NL>, dim3((unsigned)((cnt + 255) / 256)),      \
+              dim3(256), 0, s, A, fa_res_stride(depth), C, (uint64_t)zl, cnt, zw, z)
+    FA_CRT_NL_SWITCH(fa_crt_poly_limbs(C), FA_L)
+#undef FA_L
+    if (hipGetLastError() != hipSuccess) return fa_set_error(FA_E_HIP, "crt launch");
+    rc = fa_buffers_leave(st, s);
+    return rc ? rc : 1;
+}
+
 // _fmpz_vec_max_bits: max bit size of |c|, negated if some c < 0
 int64_t fa_vec_max_bits(const uint64_t* v, uint64_t w, int64_t n)
 {

@@ -254,3 +254,38 @@ def time_device(B, torch, steps, warmup):
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / steps
+
+
+def device_per_call(reqs, R, stream_ptr):
+    """one device entry-point call per request of a recorded batch's request
+    array R (device operands and outputs already in place), enqueued on one
+    stream: the per-call counterpart of the batch replay"""
+    L = fa.lib()
+    for q, r in zip(reqs, R):
+        if q["kind"] == "mpn":
+            rc = L.fa_mpn_mul_device(r.z, r.a, r.an, r.b, r.bn, stream_ptr)
+        elif q["kind"] == "nmod":
+            rc = L.fa_nmod_poly_mul_mid_device(r.z, r.zl, r.zh, r.a, r.an, r.b, r.bn, r.mod,
+                                               stream_ptr)
+        else:
+            rc = L.fa_fmpz_poly_mul_mid_device(r.z, r.zw, r.zl, r.zh, r.a, r.aw, r.an, r.b, r.bw,
+                                               r.bn, r.abits, r.bbits, 1, stream_ptr)
+        if rc < 0:
+            raise fa.FlintAmdError("device call failed: %s" % L.fa_last_error().decode())
+
+
+def time_device_per_call(reqs, R, torch, steps, warmup):
+    """ms per pass of device_per_call, HIP events on its stream"""
+    stream = torch.cuda.Stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    for _ in range(warmup):
+        device_per_call(reqs, R, sp)
+    stream.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        device_per_call(reqs, R, sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps

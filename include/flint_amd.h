@@ -189,6 +189,17 @@ FA_API int fa_nmod_poly_mulmid_fft_small(uint64_t* z, const uint64_t* a, int64_t
                                          const uint64_t* b, int64_t bn, int64_t znlo,
                                          int64_t zn, fa_nmod_t mod);
 
+/* _fmpz_poly_mul_mid on device arrays (two's complement coefficients as
+   fa_fmpz_poly_mul_mid; abits, bbits = their _fmpz_vec_max_bits, which the
+   caller knows or computes with fa_fmpz_vec_max_bits), enqueued on
+   `stream`.  Returns 1 computed, 0 declined as the reference declines
+   (fmpz_poly_mul.c:377-379); with ks != 0 the declined products are
+   computed by Kronecker substitution instead (fmpz_poly/mul.c:160). */
+FA_API int fa_fmpz_poly_mul_mid_device(uint64_t* z, uint64_t zw, int64_t zl, int64_t zh,
+                                       const uint64_t* a, uint64_t aw, int64_t an,
+                                       const uint64_t* b, uint64_t bw, int64_t bn,
+                                       int64_t abits, int64_t bbits, int ks, void* stream);
+
 /* _nmod_poly_inv_series as used by _nmod_poly_divrem_mpn_ctx
    (nmod_poly_mul.c:1246): g[0, m) = f^-1 mod x^m, f[0] invertible mod n;
    Newton iteration on the device products */
