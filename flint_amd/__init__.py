@@ -54,6 +54,24 @@ class SdFftCtx(ctypes.Structure):
                 ("slot", ctypes.c_int32)]
 
 
+class StreamReq(ctypes.Structure):
+    """fa_stream_req (include/flint_amd.h): one header of the full stream."""
+    _fields_ = [("kind", ctypes.c_int32), ("bits", ctypes.c_int32), ("an", _s64), ("bn", _s64),
+                ("n", _u64), ("key", _u64 * 2)]
+
+
+class FillSpec(ctypes.Structure):
+    """fa_fill_spec (include/flint_amd.h): one operand written on the device."""
+    _fields_ = [("dst", _VP), ("count", _u64), ("key", _u64), ("n", _u64), ("w", _u64),
+                ("kind", ctypes.c_int32), ("bits", ctypes.c_int32)]
+
+
+class EvalSpec(ctypes.Structure):
+    """fa_eval_spec (include/flint_amd.h): one evaluation of device words."""
+    _fields_ = [("src", _VP), ("count", _u64), ("w", _u64), ("x", _u64), ("m", _u64),
+                ("is_signed", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
 class MpnCtxStruct(ctypes.Structure):
     """fa_mpn_ctx_struct (include/flint_amd.h)."""
     _fields_ = [("p", _u64), ("primes", _u64 * 8), ("device", ctypes.c_int32),
@@ -132,6 +150,11 @@ SIGNATURES = {
     "fa_rng_fill_below": (None, [_VP, _VP, _u64, _u64]),
     "fa_rng_fill_limbs": (None, [_VP, _VP, _u64, ctypes.c_int]),
     "fa_rng_fill_signed": (None, [_VP, _VP, _u64, ctypes.c_int, _u64]),
+    "fa_stream_headers": (ctypes.c_int, [_u64, _s64, _VP]),
+    "fa_stream_word": (_u64, [_u64, _u64]),
+    "fa_stream_fill_device": (ctypes.c_int, [_VP, _s64, _VP]),
+    "fa_eval_device": (ctypes.c_int, [_VP, _s64, _VP, _VP]),
+    "fa_set_truncation": (None, [ctypes.c_int]),
     "fa_prof_enable": (None, [ctypes.c_int]),
     "fa_prof_reset": (None, []),
     "fa_prof_read": (ctypes.c_int, [ctypes.c_char_p, _PD, _P64, ctypes.c_int]),
@@ -312,6 +335,11 @@ def set_poly_block_cap(n):
 def set_col_variant(v):
     """column-pass tiles at 2^24 points: 1 = 8 columns per workgroup, 0 = 4"""
     lib().fa_set_col_variant(int(v))
+
+
+def set_truncation(on=True):
+    """row-granularity truncation of 2^13..2^23-point transforms (outputs identical)"""
+    lib().fa_set_truncation(1 if on else 0)
 
 
 def set_square_route(on=True):
@@ -744,6 +772,30 @@ class DeviceBatch:
             self.close()
         except Exception:
             pass
+
+
+def stream_headers(count, seed=0xF1A70005):
+    """the first `count` headers of the device-generated mixed stream"""
+    H = (StreamReq * count)()
+    _check(lib().fa_stream_headers(seed, count, ctypes.cast(H, _VP)))
+    return H
+
+
+def stream_word(key, j):
+    """word j of the SplitMix64 sequence seeded with key"""
+    return int(lib().fa_stream_word(key, j))
+
+
+def stream_fill_device(specs, stream=None):
+    """writes the operands of a FillSpec array on the device and waits"""
+    _check(lib().fa_stream_fill_device(ctypes.cast(specs, _VP), len(specs), stream))
+
+
+def eval_device(specs, stream=None):
+    """values of an EvalSpec array (numpy uint64)"""
+    out = np.zeros(max(len(specs), 1), dtype=np.uint64)
+    _check(lib().fa_eval_device(ctypes.cast(specs, _VP), len(specs), out.ctypes.data, stream))
+    return out[:len(specs)]
 
 
 def mpn_candidates(an, bn):

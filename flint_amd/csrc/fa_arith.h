@@ -42,6 +42,32 @@ FA_DEV uint64_t fa_reduce_0n_u64(double a, double n, double ninv)
 // 64x64 -> 128 helpers
 FA_DEV uint64_t fa_umulhi(uint64_t a, uint64_t b) { return __umul64hi(a, b); }
 
+// (hi*2^64 + lo) mod n with hi < n, by the Moller-Granlund preinverse
+// (udiv_qrnnd_preinv; dinv = floor((2^128-1)/d) - 2^64 for d = n << norm,
+// the ninv of FLINT's nmod_t, nmod.h:284)
+FA_DEV uint64_t fa_ll_mod_preinv(uint64_t hi, uint64_t lo, uint64_t n, uint64_t dinv, int norm)
+{
+    uint64_t d = n << norm;
+    if (norm) {
+        hi = (hi << norm) | (lo >> (64 - norm));
+        lo <<= norm;
+    }
+    uint64_t p_lo = dinv * hi;
+    uint64_t p_hi = __umul64hi(dinv, hi);
+    uint64_t q0 = p_lo + lo;
+    uint64_t q1 = p_hi + hi + 1 + (q0 < p_lo);
+    uint64_t r = lo - q1 * d;
+    if (r > q0) r += d;
+    if (r >= d) r -= d;
+    return r >> norm;
+}
+
+// a b mod n for a, b < n (n < 2^64 with its nmod_t preinverse)
+FA_DEV uint64_t fa_mulmod_preinv(uint64_t a, uint64_t b, uint64_t n, uint64_t dinv, int norm)
+{
+    return fa_ll_mod_preinv(__umul64hi(a, b), a * b, n, dinv, norm);
+}
+
 struct fa_prime_dev {
     double p;
     double pinv;

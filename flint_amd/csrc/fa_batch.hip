@@ -25,6 +25,8 @@
 #include <vector>
 
 #include "fa_internal.h"
+
+extern int fa_trunc_enabled;
 #include "fa_crt_fp.h"
 #include "../../include/flint_amd.h"
 
@@ -449,6 +451,15 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
             j.np = p.np;
             j.offset = p.offset;
             j.depth = p.depth;
+            // truncation at row granularity below 2^24 points, for products
+            // that end inside the transform (not wrapped ones)
+            j.trows = p.depth > FA_ROW_LG ? 1 << (p.depth - FA_ROW_LG) : 0;
+            {
+                const uint64_t plen = q.kind == FA_REQ_MPN ? p.zlen : p.zn;
+                if (fa_trunc_enabled && p.depth > FA_ROW_LG && p.depth < 2 * FA_ROW_LG &&
+                    plen && plen <= ((uint64_t)1 << p.depth))
+                    j.trows = (int)((plen + (1u << FA_ROW_LG) - 1) >> FA_ROW_LG);
+            }
             // normalizers depend on (np, depth, offset) only
             const int nkey = (p.np * 32 + p.depth) * 16 + p.offset;
             auto it = norms.find(nkey);
@@ -476,7 +487,8 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
                             for (int lo = 0; lo < 2; lo++)
                                 tb.fwd.push_back(FA_WORK(jid, i, z, 2 * h + lo));
                 for (int i = 0; i < p.np; i++)
-                    for (int row = 0; row < nrow; row++) tb.row.push_back(FA_WORK(jid, i, 0, row));
+                    for (int row = 0; row < nrow && row < j.trows; row++)
+                        tb.row.push_back(FA_WORK(jid, i, 0, row));
                 for (int h = 0; h < ng / 2; h++)
                     for (int i = 0; i < p.np; i++)
                         for (int lo = 0; lo < 2; lo++)

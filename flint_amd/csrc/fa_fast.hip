@@ -604,7 +604,7 @@ FA_DEV unsigned fa_xcd_linear()
 template <int D1, int KIND>
 FA_DEV void fa_col_fwd_tile(const fa_operand& op, double* __restrict__ dst, int d2, int g,
                             double n, double ninv, const double* __restrict__ w2,
-                            const double* __restrict__ pow2, double* lds)
+                            const double* __restrict__ pow2, double* lds, int trows)
 {
     typedef col_geom<D1> G;
     const int c = threadIdx.x % G::CPW, tc = threadIdx.x / G::CPW;
@@ -637,8 +637,125 @@ FA_DEV void fa_col_fwd_tile(const fa_operand& op, double* __restrict__ dst, int 
         }
     }
     const int jl = (D1 < 4) ? 0 : G::NSTAGE - 1;
+    // rows at or past trows are not read by a truncated product
 #pragma unroll
-    for (int k = 0; k < G::NEL; k++) dst[(uint64_t)G::pos(jl, tc, k) * FA_ROW_PITCH + col] = x[k];
+    for (int k = 0; k < G::NEL; k++) {
+        const int row = G::pos(jl, tc, k);
+        if (row < trows) dst[(uint64_t)row * FA_ROW_PITCH + col] = x[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Truncated inverse column transform (van der Hoeven's inverse TFT in the
+// block-twisted form; sd_ifft.c:1320-1493 is the reference's version).  A
+// column of 2^D1 points holds, at rows [0, t), the evaluations computed by
+// the row pass; its coefficients at rows >= t are zero (the product is
+// shorter than t rows).  Block Q of size 2L splits X^2L - w into
+// (X^L - r)(X^L + r), r = w2[Q], children 2Q (a = lo + r hi) and 2Q + 1
+// (b = lo - r hi).  Values are carried at the scale of their block (a block
+// of size M holds M times its coefficients, the scale of the full inverse):
+//  - t >= L: a is known; invert it (scale L), turn the known tail of hi into
+//    the tail of b, v_j = u_j - r x_hi[j] for j >= t - L, solve b's block
+//    with t - L evaluations, then (u + v, (u - v) r^-1) gives lo and hi at
+//    scale 2L;
+//  - t < L: b is unknown, lo's tail and all of hi are known; a's tail at
+//    scale L is (x_lo + r x_hi) / 2; solve a's block with t evaluations, then
+//    lo = 2 u - r x_hi.
+// The descent is one chain of blocks (at most D1 of them); the way back
+// applies each block's combination.  The tile sits in LDS as element e of
+// column c at s[(e << LG) + c]; every step runs over all CPW columns.
+// ---------------------------------------------------------------------------
+template <int D1>
+struct trunc_geom {
+    static constexpr int CPW = (D1 >= 4) ? (1024 >> (D1 - 4)) : 1024;
+    static constexpr int LG = (D1 >= 4) ? (14 - D1) : 10;   // log2 CPW
+    static constexpr int LDS = CPW << D1;
+};
+
+FA_DEV void fa_trunc_step(double* s, int LG, int A, int Bo, int j0, int j1, int mode, double r,
+                          double rinv, double n, double ninv)
+{
+    // mode 0: x_A[j] = (x_A[j] + r x_B[j]) / 2       (descent, t < L)
+    // mode 1: x_B[j] = x_A[j] - r x_B[j]             (descent, t >= L)
+    // mode 2: (x_A, x_B) = (x_A + x_B, (x_A - x_B) r^-1)   (way back, t >= L)
+    // mode 3: x_A[j] = 2 x_A[j] - r x_B[j]           (way back, t < L)
+    const int C = 1 << LG;
+    const int total = (j1 - j0) << LG;
+    const double half = 0.5 * (n + 1.0);
+    for (int u = threadIdx.x; u < total; u += blockDim.x) {
+        const int j = j0 + (u >> LG), c = u & (C - 1);
+        double* pa = s + ((A + j) << LG) + c;
+        double* pb = s + ((Bo + j) << LG) + c;
+        const double xa = *pa, xb = *pb;
+        if (mode == 0) {
+            *pa = fa_reduce_pm1n(fa_mulmod(xa + fa_mulmod(xb, r, n, ninv), half, n, ninv), n, ninv);
+        } else if (mode == 1) {
+            *pb = fa_reduce_pm1n(xa - fa_mulmod(xb, r, n, ninv), n, ninv);
+        } else if (mode == 2) {
+            *pa = fa_reduce_pm1n(xa + xb, n, ninv);
+            *pb = fa_reduce_pm1n(fa_mulmod(xa - xb, rinv, n, ninv), n, ninv);
+        } else {
+            *pa = fa_reduce_pm1n(2.0 * xa - fa_mulmod(xb, r, n, ninv), n, ninv);
+        }
+    }
+    __syncthreads();
+}
+
+// evaluations at rows [0, t) of the tile's columns -> the coefficients of
+// rows [0, t) (times 2^D1); rows >= t of s must hold zeros on entry
+template <int D1>
+FA_DEV void fa_trunc_inverse(double* s, int t, double n, double ninv,
+                             const double* __restrict__ w2, const double* __restrict__ wi)
+{
+    constexpr int LG = trunc_geom<D1>::LG;
+    int o = 0, k = D1, tt = t;
+    uint64_t Q = 0;
+    int po[D1 + 1], pk[D1 + 1], pt[D1 + 1];
+    uint64_t pq[D1 + 1];
+    int depth = 0;
+    while (k > 0 && tt > 0 && tt < (1 << k)) {
+        const int L = 1 << (k - 1);
+        const double r = w2[Q];
+        po[depth] = o; pk[depth] = k; pt[depth] = tt; pq[depth] = Q;
+        depth++;
+        if (tt >= L) {
+            fa_lds_inv(s + ((size_t)o << LG), k - 1, LG, 2 * Q, wi, n, ninv);
+            fa_trunc_step(s, LG, o, o + L, tt - L, L, 1, r, 0.0, n, ninv);
+            o += L;
+            tt -= L;
+            Q = 2 * Q + 1;
+        } else {
+            fa_trunc_step(s, LG, o, o + L, tt, L, 0, r, 0.0, n, ninv);
+            Q = 2 * Q;
+        }
+        k--;
+    }
+    if (k > 0 && tt == (1 << k)) fa_lds_inv(s + ((size_t)o << LG), k, LG, Q, wi, n, ninv);
+    while (depth-- > 0) {
+        const int L = 1 << (pk[depth] - 1);
+        const uint64_t Qd = pq[depth];
+        fa_trunc_step(s, LG, po[depth], po[depth] + L, 0, L, pt[depth] >= L ? 2 : 3, w2[Qd],
+                      wi[Qd], n, ninv);
+    }
+}
+
+template <int D1>
+FA_DEV void fa_col_inv_trunc_tile(double* __restrict__ dst, int g, int t, double n, double ninv,
+                                  const double* __restrict__ w2, const double* __restrict__ wi,
+                                  double* lds)
+{
+    typedef trunc_geom<D1> G;
+    const uint64_t col0 = (uint64_t)g * G::CPW;
+    for (int u = threadIdx.x; u < (G::CPW << D1); u += blockDim.x) {
+        const int e = u >> G::LG, c = u & (G::CPW - 1);
+        lds[u] = e < t ? dst[(uint64_t)e * FA_ROW_PITCH + col0 + c] : 0.0;
+    }
+    __syncthreads();
+    fa_trunc_inverse<D1>(lds, t, n, ninv, w2, wi);
+    for (int u = threadIdx.x; u < (t << G::LG); u += blockDim.x) {
+        const int e = u >> G::LG, c = u & (G::CPW - 1);
+        dst[(uint64_t)e * FA_ROW_PITCH + col0 + c] = lds[u];
+    }
 }
 
 template <int D1>
@@ -698,26 +815,38 @@ __global__ void __launch_bounds__(1024) fa_k_col_fwd16(fa_operand a, fa_operand 
                                                        double* __restrict__ A,
                                                        double* __restrict__ B, uint64_t stride,
                                                        fa_prime_set S, int np, int ngroups,
-                                                       int nops)
+                                                       int nops, int trows)
 {
     __shared__ double lds[col_geom<D1>::LDS];
     int i, g, z;
     fa_col_split(fa_xcd_linear(), np, ngroups, &i, &g, &z);
     if (z >= nops) return;
     fa_col_fwd_tile<D1, KIND>(z ? b : a, (z ? B : A) + (uint64_t)i * stride, d2, g, S.p[i],
-                              S.pinv[i], S.w2[i], S.pow2[i], lds);
+                              S.pinv[i], S.w2[i], S.pow2[i], lds, trows);
 }
 
 template <int D1>
+constexpr int fa_col_inv_lds()
+{
+    return col_geom<D1>::LDS > trunc_geom<D1>::LDS ? col_geom<D1>::LDS : trunc_geom<D1>::LDS;
+}
+
+// trows < 2^D1: the truncated inverse (the tile groups of both kernels
+// cover the same 4096 columns: CPW columns per group)
+template <int D1>
 __global__ void __launch_bounds__(1024) fa_k_col_inv16(double* __restrict__ A, int d2,
                                                        uint64_t stride, fa_prime_set S, int np,
-                                                       int ngroups)
+                                                       int ngroups, int trows)
 {
-    __shared__ double lds[col_geom<D1>::LDS];
+    __shared__ double lds[fa_col_inv_lds<D1>()];
     int i, g, z;
     fa_col_split(fa_xcd_linear(), np, ngroups, &i, &g, &z);
     if (z >= 1) return;
-    fa_col_inv_tile<D1>(A + (uint64_t)i * stride, d2, g, S.p[i], S.pinv[i], S.w2inv[i], lds);
+    if (trows < (1 << D1))
+        fa_col_inv_trunc_tile<D1>(A + (uint64_t)i * stride, g, trows, S.p[i], S.pinv[i], S.w2[i],
+                                  S.w2inv[i], lds);
+    else
+        fa_col_inv_tile<D1>(A + (uint64_t)i * stride, d2, g, S.p[i], S.pinv[i], S.w2inv[i], lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -808,7 +937,9 @@ FA_DEV void fa_col_fwdw_rest(double* x0, double* x1, double* __restrict__ dst, i
     }
 }
 
-template <int C, int KIND>
+// OWN > 0: the chunk width in bytes of a byte-aligned import, fixed at
+// compile time (the import's word masks and partial sums fold away)
+template <int C, int KIND, int OWN = 0>
 FA_DEV void fa_col_fwdw_tile(const fa_operand& op, double* __restrict__ dst, int d2, int g,
                              double n, double ninv, const double* __restrict__ w2,
                              const double* __restrict__ pow2, double* lds)
@@ -827,7 +958,7 @@ FA_DEV void fa_col_fwdw_tile(const fa_operand& op, double* __restrict__ dst, int
         // without per-chunk checks, at offsets stepping by 256 rows.  Each
         // import path runs the rest of the tile on its own (no join of
         // differently allocated registers).
-        const int own = op.bits >> 3;
+        const int own = OWN ? OWN : op.bits >> 3;
         const uint64_t last = ((uint64_t)(u + 128 + 256 * (hz ? 7 : 15)) << d2) + col;
         if (__all(((last * (uint64_t)own) & ~UINT64_C(3)) + 32 <= op.n * 8)) {
             const char* base = (const char*)op.src;
@@ -977,7 +1108,7 @@ FA_DEV void fa_colw_split(unsigned L, int np, int ngroups, int* i, int* g, int* 
     *z = (int)(rest / (unsigned)lines);
 }
 
-template <int C, int KIND>
+template <int C, int KIND, int OWN = 0>
 __global__ void __launch_bounds__(128 * C) fa_k_col_fwd16w(fa_operand a, fa_operand b, int d2,
                                                            double* __restrict__ A,
                                                            double* __restrict__ B,
@@ -988,7 +1119,7 @@ __global__ void __launch_bounds__(128 * C) fa_k_col_fwd16w(fa_operand a, fa_oper
     int i, g, z;
     fa_colw_split<C>(fa_xcd_linear(), np, ngroups, &i, &g, &z);
     if (z >= nops) return;
-    fa_col_fwdw_tile<C, KIND>(z ? b : a, (z ? B : A) + (uint64_t)i * stride, d2, g, S.p[i],
+    fa_col_fwdw_tile<C, KIND, OWN>(z ? b : a, (z ? B : A) + (uint64_t)i * stride, d2, g, S.p[i],
                               S.pinv[i], S.w2[i], S.pow2[i], lds);
 }
 
@@ -1051,7 +1182,7 @@ __global__ void __launch_bounds__(1024) fa_kb_col_fwd(const fa_bjob* __restrict_
                                S.pow2[pi], lds);
     else
         fa_col_fwd_tile<D1, KIND>(op, dst, FA_ROW_LG, w.tile, S.p[pi], S.pinv[pi], S.w2[pi],
-                                  S.pow2[pi], lds);
+                                  S.pow2[pi], lds, j.trows);
 }
 
 template <int D1>
@@ -1059,7 +1190,7 @@ __global__ void __launch_bounds__(1024) fa_kb_col_inv(const fa_bjob* __restrict_
                                                       const uint64_t* __restrict__ work,
                                                       unsigned nwork, fa_prime_set S)
 {
-    __shared__ double lds[(D1 == 12) ? FA_C8_LDS : col_geom<D1>::LDS];
+    __shared__ double lds[(D1 == 12) ? FA_C8_LDS : fa_col_inv_lds<D1>()];
     const unsigned L = fa_xcd_linear();
     if (L >= nwork) return;
     const fa_witem w = fa_decode(work[L]);
@@ -1068,6 +1199,9 @@ __global__ void __launch_bounds__(1024) fa_kb_col_inv(const fa_bjob* __restrict_
     double* dst = j.A + (uint64_t)w.prime * j.stride;
     if constexpr (D1 == 12)
         fa_col_inv8_tile(dst, FA_ROW_LG, w.tile, S.p[pi], S.pinv[pi], S.w2inv[pi], lds);
+    else if (j.trows < (1 << D1))
+        fa_col_inv_trunc_tile<D1>(dst, w.tile, j.trows, S.p[pi], S.pinv[pi], S.w2[pi],
+                                  S.w2inv[pi], lds);
     else
         fa_col_inv_tile<D1>(dst, FA_ROW_LG, w.tile, S.p[pi], S.pinv[pi], S.w2inv[pi], lds);
 }
@@ -1210,6 +1344,11 @@ int fa_batch_transform_bin(int depth, int kind, const fa_bjob* J, const uint64_t
     }
 }
 
+// truncated transforms (rows past the product's length skipped); switched
+// off by tests comparing them with the full transforms
+int fa_trunc_enabled = 1;
+extern "C" FA_API void fa_set_truncation(int on) { fa_trunc_enabled = on; }
+
 // column tiles at d1 = 12: 1 = 8 columns, 1024 threads; 2 = 4 columns, 512
 // threads, half-column LDS; 0 = 4 columns, 1024 threads, whole-column LDS
 int fa_col_variant = 1;
@@ -1300,8 +1439,12 @@ static int launch_fast_pipe(const fa_operand& oa, const fa_operand& ob, int nops
                   hipStreamWaitEvent(t, X->fwd[g - 1], 0) != hipSuccess))
             return fa_set_error(FA_E_HIP, "pipeline fork");
         const unsigned gf = ((unsigned)(m * ngroups * nops) + 7) & ~7u;
-        FA_LAUNCH("fa_k_col_fwd16w", (fa_k_col_fwd16w<8, KIND>), dim3(gf), dim3(1024), 0, t, oa,
-                  ob, d2, Ag, Bg, stride, Sg, m, ngroups, nops);
+        if (KIND == FA_PACK_MPN8 && oa.bits == 136 && ob.bits == 136)
+            FA_LAUNCH("fa_k_col_fwd16w", (fa_k_col_fwd16w<8, KIND, 17>), dim3(gf), dim3(1024), 0,
+                      t, oa, ob, d2, Ag, Bg, stride, Sg, m, ngroups, nops);
+        else
+            FA_LAUNCH("fa_k_col_fwd16w", (fa_k_col_fwd16w<8, KIND>), dim3(gf), dim3(1024), 0, t,
+                      oa, ob, d2, Ag, Bg, stride, Sg, m, ngroups, nops);
         if (hipEventRecord(X->fwd[g], t) != hipSuccess)
             return fa_set_error(FA_E_HIP, "pipeline event");
         if (nops == 1)
@@ -1322,10 +1465,14 @@ static int launch_fast_pipe(const fa_operand& oa, const fa_operand& ob, int nops
     return 1;
 }
 
+// trows: rows of the product's coefficients (its length / 4096, rounded
+// up); the row pass and the inverse column pass stop there (D1 < 12)
 template <int D1, int KIND>
 static int launch_fast(const fa_operand& oa, const fa_operand& ob, int nops, int np, double* A,
-                       double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s)
+                       double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s,
+                       int trows)
 {
+    if (D1 == 12 || trows < 1 || trows > (1 << D1)) trows = 1 << D1;
     if constexpr (D1 == 12) {
         if (fa_pipe_groups >= 2 && np >= 2 && fa_col_variant == 1 && fa_row_variant == 1 &&
             !fa_prof_on()) {
@@ -1346,7 +1493,10 @@ static int launch_fast(const fa_operand& oa, const fa_operand& ob, int nops, int
     const int ngroups = (1 << d2) / (wide ? wide : G::CPW);
     unsigned total = (unsigned)(np * ngroups * nops);
     unsigned grid = (total + 7) & ~7u;
-    if (wide == 8)
+    if (wide == 8 && KIND == FA_PACK_MPN8 && oa.bits == 136 && ob.bits == 136)
+        FA_LAUNCH("fa_k_col_fwd16w", (fa_k_col_fwd16w<8, KIND, 17>), dim3(grid), dim3(1024), 0, s,
+                  oa, ob, d2, A, B, stride, S, np, ngroups, nops);
+    else if (wide == 8)
         FA_LAUNCH("fa_k_col_fwd16w", (fa_k_col_fwd16w<8, KIND>), dim3(grid), dim3(1024), 0, s, oa,
                   ob, d2, A, B, stride, S, np, ngroups, nops);
     else if (wide == 4)
@@ -1354,15 +1504,16 @@ static int launch_fast(const fa_operand& oa, const fa_operand& ob, int nops, int
                   ob, d2, A, B, stride, S, np, ngroups, nops);
     else
         FA_LAUNCH("fa_k_col_fwd16", (fa_k_col_fwd16<D1, KIND>), dim3(grid), dim3(1024), 0, s, oa, ob,
-                  d2, A, B, stride, S, np, ngroups, nops);
+                  d2, A, B, stride, S, np, ngroups, nops, trows);
+    const unsigned nrow = (unsigned)trows;
     if (fa_row_variant == 1 && nops == 1)
-        FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16lr<true>, dim3(1u << D1, np), dim3(256), 0, s,
+        FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16lr<true>, dim3(nrow, np), dim3(256), 0, s,
                   A, A, stride, S);
     else if (fa_row_variant == 1)
-        FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16lr<false>, dim3(1u << D1, np), dim3(256), 0, s,
+        FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16lr<false>, dim3(nrow, np), dim3(256), 0, s,
                   A, B, stride, S);
     else
-        FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16, dim3(1u << D1, np), dim3(256), 0, s, A,
+        FA_LAUNCH("fa_k_row_conv16", fa_k_row_conv16, dim3(nrow, np), dim3(256), 0, s, A,
                   nops == 1 ? A : B, stride, S);
     unsigned total2 = (unsigned)(np * ngroups);
     unsigned grid2 = (total2 + 7) & ~7u;
@@ -1374,20 +1525,23 @@ static int launch_fast(const fa_operand& oa, const fa_operand& ob, int nops, int
                   stride, S, np, ngroups);
     else
         FA_LAUNCH("fa_k_col_inv16", fa_k_col_inv16<D1>, dim3(grid2), dim3(1024), 0, s, A, d2, stride,
-                  S, np, ngroups);
+                  S, np, ngroups, trows);
     return 1;
 }
 
 template <int D1>
 static int launch_fast_any(const fa_operand& oa, const fa_operand& ob, int nops, int np,
                            double* A, double* B, uint64_t stride, const fa_prime_set& S,
-                           hipStream_t s)
+                           hipStream_t s, int trows)
 {
     switch (fa_pack_kind(oa, ob)) {
-    case FA_PACK_MPN8: return launch_fast<D1, FA_PACK_MPN8>(oa, ob, nops, np, A, B, stride, S, s);
-    case FA_PACK_U64: return launch_fast<D1, FA_PACK_U64>(oa, ob, nops, np, A, B, stride, S, s);
-    case FA_PACK_S64W: return launch_fast<D1, FA_PACK_S64W>(oa, ob, nops, np, A, B, stride, S, s);
-    default: return launch_fast<D1, FA_PACK_ANY>(oa, ob, nops, np, A, B, stride, S, s);
+    case FA_PACK_MPN8:
+        return launch_fast<D1, FA_PACK_MPN8>(oa, ob, nops, np, A, B, stride, S, s, trows);
+    case FA_PACK_U64:
+        return launch_fast<D1, FA_PACK_U64>(oa, ob, nops, np, A, B, stride, S, s, trows);
+    case FA_PACK_S64W:
+        return launch_fast<D1, FA_PACK_S64W>(oa, ob, nops, np, A, B, stride, S, s, trows);
+    default: return launch_fast<D1, FA_PACK_ANY>(oa, ob, nops, np, A, B, stride, S, s, trows);
     }
 }
 
@@ -1407,7 +1561,7 @@ static int launch_forward(const fa_operand& oa, int np, double* A, uint64_t stri
                   oa, d2, A, A, stride, S, np, ngroups, 1);
     else
         FA_LAUNCH("fa_k_col_fwd16", (fa_k_col_fwd16<D1, KIND>), dim3(grid), dim3(1024), 0, s, oa,
-                  oa, d2, A, A, stride, S, np, ngroups, 1);
+                  oa, d2, A, A, stride, S, np, ngroups, 1, 1 << D1);
     FA_LAUNCH("fa_k_row_fwd16", fa_k_row_fwd16, dim3(1u << D1, np), dim3(256), 0, s, A, stride, S);
     return 1;
 }
@@ -1427,7 +1581,7 @@ static int launch_inverse(int np, double* A, uint64_t stride, const fa_prime_set
                   stride, S, np, ngroups);
     else
         FA_LAUNCH("fa_k_col_inv16", fa_k_col_inv16<D1>, dim3(grid), dim3(1024), 0, s, A, d2, stride,
-                  S, np, ngroups);
+                  S, np, ngroups, 1 << D1);
     return 1;
 }
 
@@ -1469,24 +1623,20 @@ int fa_inverse_fast(int depth, int np, double* A, uint64_t stride, const fa_prim
 #undef FA_F
 }
 
-// sqr: a == b; only a is transformed and the pointwise stage squares
+// sqr: a == b; only a is transformed and the pointwise stage squares.
+// zlen: coefficients of the product (0: unknown, no truncation); the product
+// must vanish from coefficient zlen on (a product shorter than the
+// transform, not a wrapped one)
 int fa_convolve_fast(int depth, const fa_operand& oa, const fa_operand& ob, int np, double* A,
-                     double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s, int sqr)
+                     double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s, int sqr,
+                     uint64_t zlen)
 {
     const int nops = sqr ? 1 : 2;
-    switch (depth - FA_ROW_LG) {
-    case 1: return launch_fast_any<1>(oa, ob, nops, np, A, B, stride, S, s);
-    case 2: return launch_fast_any<2>(oa, ob, nops, np, A, B, stride, S, s);
-    case 3: return launch_fast_any<3>(oa, ob, nops, np, A, B, stride, S, s);
-    case 4: return launch_fast_any<4>(oa, ob, nops, np, A, B, stride, S, s);
-    case 5: return launch_fast_any<5>(oa, ob, nops, np, A, B, stride, S, s);
-    case 6: return launch_fast_any<6>(oa, ob, nops, np, A, B, stride, S, s);
-    case 7: return launch_fast_any<7>(oa, ob, nops, np, A, B, stride, S, s);
-    case 8: return launch_fast_any<8>(oa, ob, nops, np, A, B, stride, S, s);
-    case 9: return launch_fast_any<9>(oa, ob, nops, np, A, B, stride, S, s);
-    case 10: return launch_fast_any<10>(oa, ob, nops, np, A, B, stride, S, s);
-    case 11: return launch_fast_any<11>(oa, ob, nops, np, A, B, stride, S, s);
-    case 12: return launch_fast_any<12>(oa, ob, nops, np, A, B, stride, S, s);
-    default: return 0;
-    }
+    const int d1 = depth - FA_ROW_LG;
+    int trows = d1 >= 1 && d1 <= 12 ? 1 << d1 : 0;
+    if (fa_trunc_enabled && zlen && zlen <= ((uint64_t)1 << depth))
+        trows = (int)((zlen + (1u << FA_ROW_LG) - 1) >> FA_ROW_LG);
+#define FA_F(D) launch_fast_any<D>(oa, ob, nops, np, A, B, stride, S, s, trows)
+    FA_D1_SWITCH(d1, FA_F)
+#undef FA_F
 }

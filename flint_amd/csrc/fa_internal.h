@@ -178,6 +178,8 @@ struct fa_bjob {
     double* B;
     uint64_t stride;           // 2^depth
     int np, offset, depth, pad;
+    int trows, pad2;           // rows of the product's coefficients (the row pass and the
+                               // inverse column pass stop there)
     double norm[FA_NPRIMES];   // pointwise normalizer of prime offset + i
 };
 

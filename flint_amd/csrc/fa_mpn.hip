@@ -1047,7 +1047,15 @@ static fa_operand to_operand(const fa_pack_desc* d)
 }
 
 int fa_convolve_fast(int depth, const fa_operand& oa, const fa_operand& ob, int np, double* A,
-                     double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s, int sqr);
+                     double* B, uint64_t stride, const fa_prime_set& S, hipStream_t s, int sqr,
+                     uint64_t zlen);
+
+// coefficients of an operand descriptor (chunks of an integer)
+static uint64_t desc_len(const fa_pack_desc* d)
+{
+    if (d->kind == FA_PACK_MPN) return (64 * d->n + (uint64_t)d->bits - 1) / (uint64_t)d->bits;
+    return d->n;
+}
 
 int fa_fast_enabled = 1;   // switched off by tests comparing the two kernel sets
 
@@ -1085,7 +1093,9 @@ int fa_convolve(fa_thread_state* st, const fa_tables* T, int np, int offset, int
         LAUNCH_CHECK();
         return FA_OK;
     }
-    if (fa_fast_enabled && fa_convolve_fast(depth, oa, ob, np, A, B, stride, S, stream, sqr)) {
+    const uint64_t la = desc_len(a), lb = desc_len(b);
+    const uint64_t zlen = la && lb ? la + lb - 1 : 0;
+    if (fa_fast_enabled && fa_convolve_fast(depth, oa, ob, np, A, B, stride, S, stream, sqr, zlen)) {
         LAUNCH_CHECK();
         return FA_OK;
     }

@@ -101,26 +101,6 @@ FA_DEV void fa_crt_coeff_poly(const double* __restrict__ Q, uint64_t stride, uin
     }
 }
 
-// (hi*2^64 + lo) mod n with hi < n, by the Moller-Granlund preinverse
-// (udiv_qrnnd_preinv; dinv = floor((2^128-1)/d) - 2^64 for d = n << norm,
-// the ninv of FLINT's nmod_t, nmod.h:284)
-FA_DEV uint64_t fa_ll_mod_preinv(uint64_t hi, uint64_t lo, uint64_t n, uint64_t dinv, int norm)
-{
-    uint64_t d = n << norm;
-    if (norm) {
-        hi = (hi << norm) | (lo >> (64 - norm));
-        lo <<= norm;
-    }
-    uint64_t p_lo = dinv * hi;
-    uint64_t p_hi = __umul64hi(dinv, hi);
-    uint64_t q0 = p_lo + lo;
-    uint64_t q1 = p_hi + hi + 1 + (q0 < p_lo);
-    uint64_t r = lo - q1 * d;
-    if (r > q0) r += d;
-    if (r >= d) r -= d;
-    return r >> norm;
-}
-
 // coefficient j modulo n from its NP residues, through the mixed-radix
 // digits of c (Garner): t_0 = c mod p_0 and
 //     t_k = (c - (t_0 + p_0 (t_1 + ... p_(k-2) t_(k-1)))) (p_0 ... p_(k-1))^-1 mod p_k,

@@ -72,14 +72,18 @@ extern "C" int fa_plan_features(int64_t an, int64_t bn, float* f)
 static thread_local uint64_t* tune_buf = nullptr;
 static thread_local size_t tune_words = 0;
 
-// times every candidate of an x bn on the device alone: after a warm-up
-// call, K back-to-back products (K = 2^21 / (an + bn), between 1 and 64)
-// are captured into one hipGraph, replayed once untimed, then `reps` times
-// with HIP events around each replay; ms[j] = the median replay time per
-// product (robust to the single slow replays a shared device shows), -1 for
-// invalid candidates.  The first call of a process first keeps the device
-// busy for about 50 ms so that its clocks have risen before any timing.
-// Host launch overhead is outside the measurement.
+// times every candidate of an x bn on the device alone: each candidate runs
+// once outside any capture (twiddle tables and scratch reach the largest
+// size any candidate needs), then K back-to-back products (K = 2^21 /
+// (an + bn), between 1 and 64) of each are captured into a hipGraph of its
+// own and replayed once untimed.  The timed replays are interleaved: replay
+// r of every candidate runs before replay r + 1 of any, so that a slow
+// stretch of the device (clocks, another process) lands on one sample of
+// many candidates rather than on every sample of one.  ms[j] = the median
+// replay time per product (HIP events around each replay), -1 for invalid
+// candidates.  The first call of a process first keeps the device busy for
+// about 50 ms so that its clocks have risen before any timing.  Host launch
+// overhead is outside the measurement.
 extern "C" int fa_autotune_mpn(int64_t an, int64_t bn, int reps, float* ms)
 {
     if (an <= 0 || bn <= 0 || reps <= 0 || !ms)
@@ -98,6 +102,7 @@ extern "C" int fa_autotune_mpn(int64_t an, int64_t bn, int reps, float* ms)
             return fa_set_error(FA_E_NOMEM, "fa_autotune_mpn: %zu words", words);
         tune_words = words;
         (void)hipMemsetAsync(tune_buf, 0xA5, words * 8, st->stream);
+        (void)hipStreamSynchronize(st->stream);
     }
     uint64_t* a = tune_buf;
     uint64_t* b = a + an;
@@ -130,33 +135,39 @@ extern "C" int fa_autotune_mpn(int64_t an, int64_t bn, int reps, float* ms)
         }
         warmed = true;
     }
-    std::vector<float> times((size_t)reps);
+    // every candidate once outside the capture
     for (int j = 0; j < FA_NCAND && !rc; j++) {
         ms[j] = -1.f;
         if (!cand[2 * j]) continue;
-        const int np = cand[2 * j], bits = cand[2 * j + 1];
-        // warm-up outside the capture: twiddle tables and scratch reach size
-        rc = fa_mpn_mul_device_plan(z, a, an, b, bn, np, bits, cs);
-        if (rc || hipStreamSynchronize(cs) != hipSuccess) {
-            rc = rc ? rc : fa_set_error(FA_E_HIP, "tune warm-up");
-            break;
-        }
+        rc = fa_mpn_mul_device_plan(z, a, an, b, bn, cand[2 * j], cand[2 * j + 1], cs);
+        if (!rc && hipStreamSynchronize(cs) != hipSuccess)
+            rc = fa_set_error(FA_E_HIP, "tune warm-up");
+    }
+    std::vector<hipGraph_t> g((size_t)FA_NCAND, nullptr);
+    std::vector<hipGraphExec_t> ex((size_t)FA_NCAND, nullptr);
+    for (int j = 0; j < FA_NCAND && !rc; j++) {
+        if (!cand[2 * j]) continue;
         if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             rc = fa_set_error(FA_E_HIP, "tune capture");
             break;
         }
         for (int64_t k = 0; k < K && !rc; k++)
-            rc = fa_mpn_mul_device_plan(z, a, an, b, bn, np, bits, cs);
-        hipGraph_t g = nullptr;
-        hipError_t ce = hipStreamEndCapture(cs, &g);
-        hipGraphExec_t ex = nullptr;
-        if (!rc && (ce != hipSuccess || !g ||
-                    hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess))
+            rc = fa_mpn_mul_device_plan(z, a, an, b, bn, cand[2 * j], cand[2 * j + 1], cs);
+        const hipError_t ce = hipStreamEndCapture(cs, &g[(size_t)j]);
+        if (!rc && (ce != hipSuccess || !g[(size_t)j] ||
+                    hipGraphInstantiate(&ex[(size_t)j], g[(size_t)j], nullptr, nullptr, 0) !=
+                        hipSuccess))
             rc = fa_set_error(FA_E_HIP, "tune graph");
-        if (!rc && hipGraphLaunch(ex, cs) != hipSuccess) rc = fa_set_error(FA_E_HIP, "tune launch");
-        for (int r = 0; r < reps && !rc; r++) {
+        if (!rc && (hipGraphLaunch(ex[(size_t)j], cs) != hipSuccess ||
+                    hipStreamSynchronize(cs) != hipSuccess))
+            rc = fa_set_error(FA_E_HIP, "tune launch");
+    }
+    std::vector<float> times((size_t)FA_NCAND * (size_t)reps, 0.f);
+    for (int r = 0; r < reps && !rc; r++) {
+        for (int j = 0; j < FA_NCAND && !rc; j++) {
+            if (!ex[(size_t)j]) continue;
             (void)hipEventRecord(e0, cs);
-            if (hipGraphLaunch(ex, cs) != hipSuccess) {
+            if (hipGraphLaunch(ex[(size_t)j], cs) != hipSuccess) {
                 rc = fa_set_error(FA_E_HIP, "tune launch");
                 break;
             }
@@ -167,17 +178,18 @@ extern "C" int fa_autotune_mpn(int64_t an, int64_t bn, int reps, float* ms)
             }
             float t = 0;
             (void)hipEventElapsedTime(&t, e0, e1);
-            times[(size_t)r] = t;
+            times[(size_t)j * (size_t)reps + (size_t)r] = t;
         }
-        if (ex) (void)hipGraphExecDestroy(ex);
-        if (g) (void)hipGraphDestroy(g);
-        if (!rc) {
-            std::sort(times.begin(), times.end());
-            const float med = (reps & 1) ? times[(size_t)reps / 2]
-                                         : 0.5f * (times[(size_t)reps / 2 - 1] +
-                                                   times[(size_t)reps / 2]);
+    }
+    for (int j = 0; j < FA_NCAND; j++) {
+        if (!rc && ex[(size_t)j]) {
+            float* tj = &times[(size_t)j * (size_t)reps];
+            std::sort(tj, tj + reps);
+            const float med = (reps & 1) ? tj[reps / 2] : 0.5f * (tj[reps / 2 - 1] + tj[reps / 2]);
             ms[j] = med / (float)K;
         }
+        if (ex[(size_t)j]) (void)hipGraphExecDestroy(ex[(size_t)j]);
+        if (g[(size_t)j]) (void)hipGraphDestroy(g[(size_t)j]);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -235,6 +235,8 @@ def device_batch(reqs, torch, dev, ks=True):
         r.a, r.b, r.z = ta.data_ptr(), tb.data_ptr(), tz.data_ptr()
         keep += [ta, tb, tz]
         outs.append(tz)
+    # the uploads and fills ran on torch's stream; the batch runs on others
+    torch.cuda.synchronize()
     B = fa.DeviceBatch(R, graph=True, ks=ks)
     keep.append(R)
     return B, outs, keep
@@ -289,3 +291,198 @@ def time_device_per_call(reqs, R, torch, steps, warmup):
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / steps
+
+
+# ----------------------------------------------------------------------
+# the mixed stream at its full size (200,000 requests, about 0.5 TB of
+# operands), generated, multiplied and checked on the device chunk by chunk
+# ----------------------------------------------------------------------
+
+# 61-bit prime of the integer and fmpz checks
+CHECK_Q = 2305843009213693951
+
+
+def stream_words(h):
+    """(a words, b words, z words, a width, z width) of a stream header"""
+    an, bn = h.an, h.bn
+    if h.kind == fa.REQ_MPN:
+        return an, bn, an + bn, 1, 1
+    if h.kind == fa.REQ_NMOD:
+        return an, bn, an + bn - 1, 1, 1
+    w = h.bits // 64 + 1
+    zw = (2 * h.bits + 64) // 64 + 1
+    return an * w, bn * w, (an + bn - 1) * zw, w, zw
+
+
+def stream_header_workspace(h):
+    a, b, z, w, zw = stream_words(h)
+    io = 8 * (a + b + z)
+    if h.kind == fa.REQ_MPN:
+        return io + mpn_workspace(h.an, h.bn)
+    depth = max(4, int(np.ceil(np.log2(h.an + h.bn))))
+    ws = io + 16 * 8 * res_words(depth)
+    if h.kind == fa.REQ_FMPZ:
+        sb = 2 * h.bits + int(min(h.an, h.bn)).bit_length() + 1
+        la, lb = -(-h.an * sb // 64), -(-h.bn * sb // 64)
+        ws += mpn_workspace(la, lb) + 8 * 2 * (la + lb)
+    return ws
+
+
+def stream_chunks(H, cap):
+    """header indices in consecutive groups of at most `cap` bytes of
+    operands, outputs and workspace"""
+    out, cur, ws = [], [], 0
+    for k in range(len(H)):
+        w = stream_header_workspace(H[k])
+        if cur and ws + w > cap:
+            out.append(cur)
+            cur, ws = [], 0
+        cur.append(k)
+        ws += w
+    if cur:
+        out.append(cur)
+    return out
+
+
+def stream_limb_products(h):
+    if h.kind == fa.REQ_FMPZ:
+        wa = -(-(h.bits + 1) // 64)
+        return h.an * h.bn * wa * wa
+    return h.an * h.bn
+
+
+class StreamChunk:
+    """one chunk of the full stream in HBM: operands written on the device,
+    requests recorded into a graph-captured batch, outputs checked by
+    evaluation"""
+
+    def __init__(self, H, idx, torch, dev, net=None):
+        self.H, self.idx, self.torch = H, idx, torch
+        sizes = [stream_words(H[k]) for k in idx]
+        total = sum(a + b + z for a, b, z, _, _ in sizes)
+        self.arena = torch.empty(max(total, 1), dtype=torch.int64, device=dev)
+        base = self.arena.data_ptr()
+        R = (fa.Request * len(idx))()
+        F = (fa.FillSpec * (2 * len(idx)))()
+        off = 0
+        self.ptrs = []
+        for j, (k, (aw_, bw_, zw_, w, zw)) in enumerate(zip(idx, sizes)):
+            h = H[k]
+            pa, pb, pz = base + 8 * off, base + 8 * (off + aw_), base + 8 * (off + aw_ + bw_)
+            off += aw_ + bw_ + zw_
+            self.ptrs.append((pa, pb, pz))
+            for s_, (p, cnt) in enumerate(((pa, h.an), (pb, h.bn))):
+                f = F[2 * j + s_]
+                f.dst, f.count, f.key, f.kind = p, cnt, h.key[s_], h.kind
+                f.n, f.w, f.bits = h.n, w, h.bits
+            r = R[j]
+            r.kind, r.an, r.bn = h.kind, h.an, h.bn
+            r.a, r.b, r.z = pa, pb, pz
+            if h.kind == fa.REQ_NMOD:
+                r.zl, r.zh = 0, h.an + h.bn - 1
+                r.mod = fa.nmod_init(h.n)
+            elif h.kind == fa.REQ_FMPZ:
+                r.aw = r.bw = w
+                r.zw = zw
+                r.zl, r.zh = 0, h.an + h.bn - 1
+                # the generator's bound: |c| < 2^b with both signs present
+                r.abits = r.bbits = -h.bits
+        if net is not None:
+            mi = [j for j, k in enumerate(idx) if H[k].kind == fa.REQ_MPN]
+            if mi:
+                shapes = np.array([[H[idx[j]].an, H[idx[j]].bn] for j in mi], dtype=np.int64)
+                for j, (np_, bits) in zip(mi, net.plan_mpn(shapes)):
+                    R[j].np, R[j].bits = int(np_), int(bits)
+        self.R, self.F = R, F
+        torch.cuda.synchronize()            # the arena comes from torch's stream
+        fa.stream_fill_device(F)
+        self.batch = fa.DeviceBatch(R, graph=True, ks=True)
+        self.limb_products = sum(stream_limb_products(H[k]) for k in idx)
+
+    def run(self, stream):
+        self.batch.launch(stream)
+
+    def check(self):
+        """the requests whose product fails A(x) B(x) = Z(x) mod m"""
+        H, idx = self.H, self.idx
+        E = (fa.EvalSpec * (3 * len(idx)))()
+        mods = []
+        for j, k in enumerate(idx):
+            h = H[k]
+            pa, pb, pz = self.ptrs[j]
+            _, _, _, w, zw = stream_words(h)
+            if h.kind == fa.REQ_MPN:
+                m, x, sg, cnt = CHECK_Q, (1 << 64) % CHECK_Q, 0, (h.an, h.bn, h.an + h.bn)
+                ws = (1, 1, 1)
+            elif h.kind == fa.REQ_NMOD:
+                m, x, sg, cnt = h.n, h.key[0] % h.n, 0, (h.an, h.bn, h.an + h.bn - 1)
+                ws = (1, 1, 1)
+            else:
+                m, x, sg, cnt = CHECK_Q, h.key[1] % CHECK_Q, 1, (h.an, h.bn, h.an + h.bn - 1)
+                ws = (w, w, zw)
+            mods.append(m)
+            for t, p in enumerate((pa, pb, pz)):
+                e = E[3 * j + t]
+                e.src, e.count, e.w, e.x, e.m, e.is_signed = p, cnt[t], ws[t], x, m, sg
+        v = fa.eval_device(E)
+        bad = []
+        for j, k in enumerate(idx):
+            a, b, z = (int(t) for t in v[3 * j:3 * j + 3])
+            if a * b % mods[j] != z or self.R[j].status != 1:
+                bad.append(k)
+        return bad
+
+    def close(self):
+        self.batch.close()
+        self.arena = None
+
+
+def run_full_stream(torch, dev, count=None, cap=48 << 30, net=None, per_call=False,
+                    progress=None, on_chunk=None):
+    """every request of the full stream, chunk by chunk: batch replay timed
+    with HIP events (one untimed replay first), products checked; optionally
+    the same requests one device call each; on_chunk(chunk) runs before a
+    chunk is released.  Returns a summary dict."""
+    H = fa.stream_headers(COUNTS["stream"] if count is None else count)
+    groups = stream_chunks(H, cap)
+    stream = torch.cuda.Stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    tot = {"requests": len(H), "chunks": len(groups), "limb_products": 0, "batch_ms": 0.0,
+           "per_call_ms": 0.0 if per_call else None, "failed": [], "wall_s": 0.0}
+    t0 = time.perf_counter()
+    for c, idx in enumerate(groups):
+        ch = StreamChunk(H, idx, torch, dev, net)
+        ch.run(sp)
+        stream.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ch.run(sp)
+        e1.record(stream)
+        stream.synchronize()
+        ms = e0.elapsed_time(e1)
+        bad = ch.check()
+        pc = None
+        if per_call:
+            reqs = [{"kind": {0: "mpn", 1: "nmod", 2: "fmpz"}[H[k].kind]} for k in idx]
+            e0.record(stream)
+            device_per_call(reqs, ch.R, sp)
+            e1.record(stream)
+            stream.synchronize()
+            pc = e0.elapsed_time(e1)
+            tot["per_call_ms"] += pc
+            bad = sorted(set(bad) | set(ch.check()))
+        if on_chunk:
+            on_chunk(ch)
+        tot["limb_products"] += ch.limb_products
+        tot["batch_ms"] += ms
+        tot["failed"] += bad
+        ch.close()
+        del ch
+        torch.cuda.empty_cache()            # chunk arenas differ in size
+        if progress:
+            progress("chunk %d/%d: %d requests, batch %.1f ms%s, failed %d" %
+                     (c + 1, len(groups), len(idx), ms,
+                      "" if pc is None else ", per call %.1f ms" % pc, len(bad)))
+    tot["wall_s"] = time.perf_counter() - t0
+    return tot

@@ -386,6 +386,59 @@ FA_API void fa_rng_fill_limbs(fa_rng* r, uint64_t* out, uint64_t n, int top_nonz
 /* n coefficients uniform in (-2^bits, 2^bits), w-limb two's complement */
 FA_API void fa_rng_fill_signed(fa_rng* r, uint64_t* out, uint64_t n, int bits, uint64_t w);
 
+/* The mixed stream at its full size, generated and checked on the device (no
+   counterpart in the reference).  Request headers come from the stream above
+   (seed 0xF1A70005), per request in this order: a double picks the kind
+   (< 0.5 integer, < 0.8 nmod, else fmpz); integer: an, bn log-uniform in
+   [2^4, 2^22]; nmod: an, bn log-uniform in [2^6, 2^16], modulus bits uniform
+   in [20, 63], the prime; fmpz: an, bn uniform in [1024, 16384], b uniform in
+   [64, 4096]; then key[0], key[1].  Operand s holds the SplitMix64 sequence
+   seeded with key[s] (output j = fa_stream_word(key, j)):
+     integer limb j       = word j (the top limb 1 if its word is 0)
+     nmod coefficient j   = floor(word j * n / 2^64), in [0, n)
+     fmpz coefficient j   = t - (2^b - 1), t the (b+1)-bit number of words
+                            j tw .. j tw + tw - 1 (tw = ceil((b+1)/64), the
+                            last cut to its low bits), 0 when all ones; in
+                            (-2^b, 2^b) as b / 64 + 1 limbs of two's complement */
+typedef struct {
+    int32_t kind;          /* FA_REQ_* */
+    int32_t bits;          /* nmod: modulus bits; fmpz: coefficient bits b */
+    int64_t an, bn;        /* limbs (integers) or coefficients */
+    uint64_t n;            /* nmod modulus */
+    uint64_t key[2];       /* seeds of the operands' contents */
+} fa_stream_req;
+FA_API int fa_stream_headers(uint64_t seed, int64_t count, fa_stream_req* out);
+FA_API uint64_t fa_stream_word(uint64_t key, uint64_t j);
+
+/* one operand to write on the device: count limbs (integer, nmod) or
+   coefficients of w limbs (fmpz) at dst, contents as above */
+typedef struct {
+    uint64_t* dst;
+    uint64_t count;
+    uint64_t key;
+    uint64_t n;            /* nmod modulus */
+    uint64_t w;            /* fmpz limbs per coefficient (>= bits / 64 + 1) */
+    int32_t kind;          /* FA_REQ_* */
+    int32_t bits;          /* fmpz coefficient bits */
+} fa_fill_spec;
+/* writes n operands in one launch on `stream` and waits for it */
+FA_API int fa_stream_fill_device(const fa_fill_spec* spec, int64_t n, void* stream);
+
+/* value at x mod m (2 <= m < 2^64) of count coefficients of w limbs each at
+   the device address src (two's complement when is_signed): sum_i c_i x^i;
+   an integer of count limbs is count coefficients of one limb at x = 2^64 */
+typedef struct {
+    const uint64_t* src;
+    uint64_t count;
+    uint64_t w;
+    uint64_t x;
+    uint64_t m;
+    int32_t is_signed;
+    int32_t pad;
+} fa_eval_spec;
+/* n evaluations on `stream`; out (host) receives the n values when it returns */
+FA_API int fa_eval_device(const fa_eval_spec* spec, int64_t n, uint64_t* out, void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* measurement support                                                    */
 /* ---------------------------------------------------------------------- */
@@ -401,6 +454,11 @@ FA_API int fa_prof_read(char* names, double* ms, int64_t* counts, int max);
 FA_API void fa_set_fast_kernels(int on);
 /* column-pass tiles at 2^24 points: 1 = 8 columns (default), 0 = 4 columns */
 FA_API void fa_set_col_variant(int v);
+/* 1 (default): transforms of 2^13 .. 2^23 points stop at the rows of the
+   product's length (row pass skipped past it, truncated inverse column
+   transform); 0: every transform runs all 2^depth points.  Tuning and test
+   switch; outputs are identical. */
+FA_API void fa_set_truncation(int on);
 /* row pass variant of the two-level transforms: 1 = last-stage twiddles
    formed per radix-4 layer, three workgroups per CU (default); 0 = all 15
    held in registers, two workgroups per CU.  Tuning switch, no counterpart
@@ -515,9 +573,11 @@ FA_API int fa_batch_mul(fa_request* reqs, int64_t n, int flags);
 FA_API int fa_mpn_candidates(int64_t an, int64_t bn, int32_t* out /* 64 x {np, bits} */);
 /* 16 planner features of a shape (log sizes, the automatic plan, bmax) */
 FA_API int fa_plan_features(int64_t an, int64_t bn, float* f16);
-/* exhaustive autotuner: device time (ms) per product of every candidate,
-   K back-to-back products captured in one hipGraph and replayed `reps`
-   times (host launch overhead excluded); -1 for invalid candidates */
+/* exhaustive autotuner: device time (ms) per product of every candidate:
+   K back-to-back products captured in one hipGraph per candidate, the
+   graphs replayed `reps` times in turn (replay r of every candidate before
+   replay r + 1 of any), the median replay per product (host launch
+   overhead excluded); -1 for invalid candidates */
 FA_API int fa_autotune_mpn(int64_t an, int64_t bn, int reps, float* ms64);
 
 /* FP32 MLP 16 -> 64 -> 64 -> 64 predicting log(ms) of the 64 candidates */

@@ -36,7 +36,8 @@ static u64 round_up(u64 a, u64 b) { return cdiv(a, b) * b; }
 /* ------------------------------------------------------------------ */
 
 static u64 mulm(u64 a, u64 b, u64 n) { return (u64)(((u128)a * b) % n); }
-static u64 addm(u64 a, u64 b, u64 n) { u64 s = a + b; return s >= n ? s - n : s; }
+/* a + b mod n for a, b < n < 2^64 (the sum may wrap past 2^64) */
+static u64 addm(u64 a, u64 b, u64 n) { u64 s = a + b; return (s < a || s >= n) ? s - n : s; }
 static u64 subm(u64 a, u64 b, u64 n) { return a >= b ? a - b : a + n - b; }
 
 static u64 powm(u64 a, u64 e, u64 n)
@@ -1063,6 +1064,89 @@ void or_rng_fill_signed(or_rng* r, u64* out, u64 n, int bits, u64 w)
             c[k] = x - sub;
             sub = (x < sub) ? 1 : 0;
             if (!sub) break;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* the full mixed stream: headers from the xoshiro256** stream, operand */
+/* contents from SplitMix64 sequences keyed per operand                 */
+/* ------------------------------------------------------------------ */
+
+/* word j of the SplitMix64 sequence seeded with key: the state after j + 1
+   increments, through the output function */
+u64 or_stream_word(u64 key, u64 j)
+{
+    u64 x = key + j * UINT64_C(0x9E3779B97F4A7C15);
+    return splitmix64(&x);
+}
+
+/* count headers, 7 words each: kind (0 integer, 1 nmod, 2 fmpz), bits, an,
+   bn, n, key0, key1 */
+void or_stream_headers(u64 seed, u64 count, u64* out)
+{
+    or_rng r;
+    or_rng_seed(&r, seed);
+    for (u64 k = 0; k < count; k++) {
+        u64* h = out + 7 * k;
+        memset(h, 0, 7 * sizeof(u64));
+        double u = or_rng_double(&r);
+        if (u < 0.5) {
+            h[0] = 0;
+            h[2] = or_rng_log_uniform(&r, 4, 22);
+            h[3] = or_rng_log_uniform(&r, 4, 22);
+        } else if (u < 0.8) {
+            h[0] = 1;
+            h[2] = or_rng_log_uniform(&r, 6, 16);
+            h[3] = or_rng_log_uniform(&r, 6, 16);
+            h[1] = or_rng_uniform(&r, 20, 63);
+            h[4] = or_rng_random_prime(&r, h[1]);
+        } else {
+            h[0] = 2;
+            h[2] = or_rng_uniform(&r, 1024, 16384);
+            h[3] = or_rng_uniform(&r, 1024, 16384);
+            h[1] = or_rng_uniform(&r, 64, 4096);
+        }
+        h[5] = or_rng_next(&r);
+        h[6] = or_rng_next(&r);
+    }
+}
+
+/* one operand of the stream: count limbs (kind 0: top limb 1 when its word
+   is 0; kind 1: word * n / 2^64) or count coefficients of w limbs (kind 2:
+   t of the words j tw .. j tw + tw - 1 cut to bits + 1 bits, the all-ones t
+   replaced by 0, then t + 1 - 2^bits) */
+void or_stream_fill(int kind, u64 key, u64 count, u64 n, u64 w, int bits, u64* out)
+{
+    if (kind == 0) {
+        for (u64 j = 0; j < count; j++) out[j] = or_stream_word(key, j);
+        if (count && out[count - 1] == 0) out[count - 1] = 1;
+        return;
+    }
+    if (kind == 1) {
+        for (u64 j = 0; j < count; j++) out[j] = (u64)(((u128)or_stream_word(key, j) * n) >> 64);
+        return;
+    }
+    u64 tw = (u64)(bits + 64) / 64;
+    int topb = bits + 1 - 64 * (int)(tw - 1);
+    u64 mask = (topb == 64) ? ~(u64)0 : (((u64)1 << topb) - 1);
+    for (u64 j = 0; j < count; j++) {
+        u64* c = out + j * w;
+        int ones = 1;
+        for (u64 k = 0; k < w; k++) c[k] = 0;
+        for (u64 k = 0; k < tw; k++) {
+            c[k] = or_stream_word(key, j * tw + k);
+            if (k + 1 == tw) c[k] &= mask;
+            if (c[k] != ((k + 1 == tw) ? mask : ~(u64)0)) ones = 0;
+        }
+        if (ones)
+            for (u64 k = 0; k < tw; k++) c[k] = 0;
+        for (u64 k = 0; k < w; k++) if (++c[k] != 0) break;
+        u64 sub = (u64)1 << (bits % 64);
+        for (u64 k = (u64)bits / 64; k < w && sub; k++) {
+            u64 x = c[k];
+            c[k] = x - sub;
+            sub = (x < sub) ? 1 : 0;
         }
     }
 }

@@ -85,6 +85,12 @@ uint64_t or_rng_log_uniform(or_rng* r, double x, double y);
 void     or_rng_fill_below(or_rng* r, uint64_t* out, uint64_t n, uint64_t bound);
 void     or_rng_fill_signed(or_rng* r, uint64_t* out, uint64_t n, int bits, uint64_t w);
 
+/* the full mixed stream (headers: 7 words each; operand contents) */
+uint64_t or_stream_word(uint64_t key, uint64_t j);
+void     or_stream_headers(uint64_t seed, uint64_t count, uint64_t* out);
+void     or_stream_fill(int kind, uint64_t key, uint64_t count, uint64_t n, uint64_t w, int bits,
+                        uint64_t* out);
+
 /* full-size checkers: residues mod a word-size q */
 uint64_t or_mpn_mod_q(const uint64_t* a, uint64_t n, uint64_t q);
 uint64_t or_nmod_poly_eval(const uint64_t* c, uint64_t n, uint64_t x, uint64_t q);

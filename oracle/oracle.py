@@ -99,6 +99,10 @@ def lib():
         L.or_nmod_poly_eval.restype = u64
         L.or_fmpz_poly_eval_mod_q.argtypes = [P64, u64, u64, u64, u64]
         L.or_fmpz_poly_eval_mod_q.restype = u64
+        L.or_stream_word.argtypes = [u64, u64]
+        L.or_stream_word.restype = u64
+        L.or_stream_headers.argtypes = [u64, u64, P64]
+        L.or_stream_fill.argtypes = [ctypes.c_int, u64, u64, u64, u64, ctypes.c_int, P64]
         _lib = L
     return _lib
 
@@ -289,4 +293,26 @@ def workload_requests(workload, count):
         else:
             u = r.double()
             out.append(mpn() if u < 0.5 else nmod() if u < 0.8 else fmpz())
+    return out
+
+
+# ----------------------------------------------------------------------
+# the full mixed stream: headers, and operand contents keyed per operand
+# ----------------------------------------------------------------------
+
+def stream_word(key, j):
+    return int(lib().or_stream_word(key, j))
+
+
+def stream_headers(count, seed=0xF1A70005):
+    """(count, 7) uint64: kind (0 integer, 1 nmod, 2 fmpz), bits, an, bn, n,
+    key0, key1"""
+    out = np.zeros((count, 7), dtype=np.uint64)
+    lib().or_stream_headers(seed, count, _ptr(out))
+    return out
+
+
+def stream_operand(kind, key, count, n=0, w=1, bits=0):
+    out = np.zeros(count * (w if kind == 2 else 1), dtype=np.uint64)
+    lib().or_stream_fill(kind, key, count, n, w, bits, _ptr(out))
     return out

@@ -192,6 +192,7 @@ def test_prime_groups_agree(an, bn):
         ta = torch.from_numpy(a.view(np.int64)).to(dev)
         tb = torch.from_numpy(b.view(np.int64)).to(dev)
         tz = torch.zeros(an + bn, dtype=torch.int64, device=dev)
+        torch.cuda.synchronize()   # the fill runs on torch's stream, the product on another
         stream = torch.cuda.Stream()
         fa.mpn_mul_device(tz.data_ptr(), ta.data_ptr(), an, tb.data_ptr(), bn, stream.cuda_stream)
         stream.synchronize()
@@ -278,6 +279,7 @@ def test_blocked_matches_single_plan_on_device_stream():
     ta = torch.from_numpy(a.view(np.int64)).to(dev)
     tb = torch.from_numpy(b.view(np.int64)).to(dev)
     tz = torch.zeros(an + bn, dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()   # the fill runs on torch's stream, the product on another
     stream = torch.cuda.Stream()
     try:
         fa.set_mpn_block_cap(120000)

@@ -55,6 +55,7 @@ def test_mulmod_2expp1_device_square_and_rejects():
     x = from_int(u, n + 1)
     dx = torch.from_numpy(x.view(np.int64)).to("cuda:0")
     dz = torch.zeros(n + 1, dtype=torch.int64, device="cuda:0")
+    torch.cuda.synchronize()   # the fill runs on torch's stream, the call on s
     s = torch.cuda.Stream()
     assert fa.lib().fa_mpn_mulmod_2expp1_device(dz.data_ptr(), dx.data_ptr(), dx.data_ptr(), n,
                                                 s.cuda_stream) == 0

@@ -22,7 +22,11 @@ pytestmark = pytest.mark.gpu
 
 
 def _dev(torch, a):
-    return torch.from_numpy(np.ascontiguousarray(a).view(np.int64)).to("cuda:0")
+    t = torch.from_numpy(np.ascontiguousarray(a).view(np.int64)).to("cuda:0")
+    # the upload runs on torch's stream, the library's kernels on their own:
+    # a pageable copy may still be in flight when .to() returns
+    torch.cuda.synchronize()
+    return t
 
 
 @pytest.mark.parametrize("an,bn,K", [(40, 30, 3), (700, 500, 4), (20000, 9000, 5),
@@ -50,6 +54,7 @@ def test_mpn_dot_product_in_transform_space(an, bn, K):
     Z.ifft()
     zn = an + bn + 1
     dz = torch.zeros(zn, dtype=torch.int64, device="cuda:0")
+    torch.cuda.synchronize()   # the fill runs on torch's stream, the library on its own
     Z.export_mpn(dz.data_ptr(), zn)
     torch.cuda.synchronize()
     want = sum(to_int(a) * to_int(b) for a, b in zip(A, B))
@@ -73,6 +78,7 @@ def test_mpn_dot_product_in_transform_space(an, bn, K):
         Q.sqr(opA[0])
         Q.ifft()
         dq = torch.zeros(2 * an, dtype=torch.int64, device="cuda:0")
+        torch.cuda.synchronize()
         Q.export_mpn(dq.data_ptr(), 2 * an)
         torch.cuda.synchronize()
         assert to_int(dq.cpu().numpy().view(np.uint64)) == to_int(A[0]) ** 2
@@ -86,6 +92,7 @@ def test_mpn_dot_product_in_transform_space(an, bn, K):
     X.ifft()
     short = (an + bn) // 2
     ds = torch.zeros(short, dtype=torch.int64, device="cuda:0")
+    torch.cuda.synchronize()
     X.export_mpn(ds.data_ptr(), short)
     torch.cuda.synchronize()
     assert to_int(ds.cpu().numpy().view(np.uint64)) == \
@@ -122,9 +129,11 @@ def test_nmod_dot_product_in_transform_space(n, ln, K):
         torch.cuda.synchronize()
     Z.ifft()
     dz = torch.zeros(zn, dtype=torch.int64, device="cuda:0")
+    torch.cuda.synchronize()
     Z.export_nmod(dz.data_ptr(), n)
     lo, hi = zn // 3, zn // 3 + 101
     dw = torch.zeros(hi - lo, dtype=torch.int64, device="cuda:0")
+    torch.cuda.synchronize()
     Z.export_nmod(dw.data_ptr(), n, lo, hi)
     torch.cuda.synchronize()
     want = np.zeros(zn, dtype=object)
@@ -173,15 +182,20 @@ def test_signed_export(an, bn):
     A = [rand_limbs(rng, an) for _ in range(2)]
     B = [rand_limbs(rng, bn) for _ in range(2)]
     ops = []
+    # the device operands stay referenced until the transforms (enqueued on
+    # the library's stream) have read them
+    dA = [_dev(torch, A[k]) for k in range(2)]
+    dB = [_dev(torch, B[k]) for k in range(2)]
     for k in range(2):
         oa, ob = fa.Op(P), fa.Op(P)
-        oa.fft_mpn(_dev(torch, A[k]).data_ptr(), an)
-        ob.fft_mpn(_dev(torch, B[k]).data_ptr(), bn)
+        oa.fft_mpn(dA[k].data_ptr(), an)
+        ob.fft_mpn(dB[k].data_ptr(), bn)
         ops.append((oa, ob))
     nl = -(-50 * P.s.np // 64)
     zn = (P.s.zn * P.s.bits + 64 * nl + 2 + 63) // 64 + 1
     dz = torch.zeros(zn, dtype=torch.int64, device="cuda:0")
     ds = torch.full((1,), 7, dtype=torch.int32, device="cuda:0")
+    torch.cuda.synchronize()
     for first, second in ((0, 1), (1, 0), (0, 0)):
         Z = fa.Op(P)
         Z.mul(ops[first][0], ops[first][1])
@@ -197,7 +211,7 @@ def test_signed_export(an, bn):
     # an unsigned plan refuses the signed export
     Pu = fa.Plan.mpn(an, bn, len_bound=2)
     U = fa.Op(Pu)
-    U.fft_mpn(_dev(torch, A[0]).data_ptr(), an)
+    U.fft_mpn(dA[0].data_ptr(), an)
     U.sqr(U)
     U.ifft()
     with pytest.raises(fa.FlintAmdError):

@@ -57,6 +57,7 @@ def test_mlp_training_lowers_loss():
 def test_planner_regret_and_gain():
     """train on the stored MI355X autotuner dataset with the library's own
     kernels; check regret on its held-out split and on shapes autotuned now"""
+    import flint_amd as fa
     from flint_amd import planner as PL
     trs, trm, tes, tem = PL.load_dataset()
     net, losses = PL.train(PL.features(trs), trm, epochs=800)
@@ -75,9 +76,15 @@ def test_planner_regret_and_gain():
     # per process and takes the median of its replays
     fresh = PL.sample_shapes(24, 0xF1A70005)
     _, mf = PL.collect(fresh, reps=7)
-    rf = PL.regret(mf, PL.learned_index(net, fresh))
+    lf = PL.learned_index(net, fresh)
+    rf = PL.regret(mf, lf)
     rfh = PL.regret(mf, np.array([PL.heuristic_index(a, b) for a, b in fresh]))
     print("fresh: regret learned %.4f fixed rule %.4f" % (rf.mean(), rfh.mean()))
+    for k in np.argsort(-rf)[:4]:
+        c = fa.mpn_candidates(int(fresh[k][0]), int(fresh[k][1]))
+        jb = int(np.where(mf[k] > 0, mf[k], np.inf).argmin())
+        print("  shape %s: learned %s %.4f ms, best %s %.4f ms" %
+              (tuple(fresh[k]), tuple(c[lf[k]]), mf[k][lf[k]], tuple(c[jb]), mf[k][jb]))
     assert rf.mean() <= 0.10
     net.close()
 

@@ -46,6 +46,7 @@ def test_mpn_square_all_ones_and_device_pointers():
         # the device entry point with one pointer for both operands
         da = torch.from_numpy(a.view(np.int64)).to(dev)
         dz = torch.empty(2 * n, dtype=torch.int64, device=dev)
+        torch.cuda.synchronize()   # the upload runs on torch's stream, the product on s
         s = torch.cuda.Stream(device=dev)
         fa.mpn_mul_device(dz.data_ptr(), da.data_ptr(), n, da.data_ptr(), n, s.cuda_stream)
         s.synchronize()

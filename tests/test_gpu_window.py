@@ -61,6 +61,7 @@ def test_mpn_middle_product_device():
         da = torch.from_numpy(a.view(np.int64)).to(dev)
         db = torch.from_numpy(b.view(np.int64)).to(dev)
         dz = torch.zeros(n + 2, dtype=torch.int64, device=dev)
+        torch.cuda.synchronize()   # the fill runs on torch's stream, the call on s
         s = torch.cuda.Stream(device=dev)
         fa.mpn_mul_range_device(dz.data_ptr(), n - 1, 2 * n + 1, da.data_ptr(), 2 * n - 1,
                                 db.data_ptr(), n, s.cuda_stream)

@@ -83,7 +83,7 @@ def test_signed_fill_edges():
 
 def test_checkers_match_python_integers():
     rng = np.random.default_rng(4)
-    qs = [(1 << 61) - 1, 1152921504606846883, 3, 2]
+    qs = [(1 << 61) - 1, 1152921504606846883, 3, 2, 2**63 - 25, 2**64 - 59]
     for q in qs:
         for n in (1, 2, 7, 50):
             a = rng.integers(0, 2**64, n, dtype=np.uint64, endpoint=False)
@@ -101,3 +101,58 @@ def test_checkers_match_python_integers():
                     (v % (1 << (64 * w))).to_bytes(8 * w, "little"), dtype=np.uint64)
             want = sum(v * pow(x, i, q) for i, v in enumerate(vals)) % q
             assert O.fmpz_poly_eval_mod_q(flat, w, x, q) == want
+
+
+# ----------------------------------------------------------------------
+# the full stream: headers and operand contents keyed per operand
+# ----------------------------------------------------------------------
+
+def test_stream_word_is_splitmix64():
+    """SplitMix64's published outputs for seed 1234567, and the library's
+    word function against the oracle's at large indices"""
+    import flint_amd as fa
+    want = [6457827717110365317, 3203168211198807973, 9817491932198370423,
+            4593380528125082431, 16408922859458223821]
+    assert [O.stream_word(1234567, j) for j in range(5)] == want
+    assert [fa.stream_word(1234567, j) for j in range(5)] == want
+    for key, j in ((0, 0), (2**64 - 1, 10**12), (0xF1A70005, 2**40 + 3)):
+        assert fa.stream_word(key, j) == O.stream_word(key, j)
+
+
+def test_stream_headers_agree_and_follow_their_distributions():
+    import flint_amd as fa
+    H = fa.stream_headers(3000)
+    R = O.stream_headers(3000)
+    kinds = [0, 0, 0]
+    for h, r in zip(H, R):
+        assert [h.kind, h.bits, h.an, h.bn, h.n, h.key[0], h.key[1]] == [int(v) for v in r]
+        kinds[h.kind] += 1
+        if h.kind == 0:
+            assert 16 <= h.an < (1 << 22) and 16 <= h.bn < (1 << 22)
+        elif h.kind == 1:
+            assert 64 <= h.an < 65536 and 20 <= h.bits <= 63
+            assert h.n.bit_length() == h.bits and O.lib().or_is_prime(h.n)
+        else:
+            assert 1024 <= h.an <= 16384 and 64 <= h.bits <= 4096
+    assert abs(kinds[0] / 3000 - 0.5) < 0.04 and abs(kinds[1] / 3000 - 0.3) < 0.04
+
+
+def test_stream_operands_from_their_words():
+    """each kind's element mapping, recomputed from the words with Python
+    integers"""
+    key = 0x5EED
+    a = O.stream_operand(0, key, 300)
+    assert [int(v) for v in a] == [O.stream_word(key, j) for j in range(300)]
+    n = 1000003
+    c = O.stream_operand(1, key, 300, n=n)
+    assert [int(v) for v in c] == [O.stream_word(key, j) * n >> 64 for j in range(300)]
+    for bits in (1, 64, 100, 191):
+        w = bits // 64 + 2
+        tw = (bits + 64) // 64
+        f = O.stream_operand(2, key, 50, w=w, bits=bits)
+        for j, v in enumerate(_signed_values(f, w)):
+            t = sum(O.stream_word(key, j * tw + k) << (64 * k) for k in range(tw))
+            t &= (1 << (bits + 1)) - 1
+            if t == (1 << (bits + 1)) - 1:
+                t = 0
+            assert v == t - ((1 << bits) - 1), (bits, j)
