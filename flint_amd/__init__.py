@@ -105,6 +105,8 @@ SIGNATURES = {
     "fa_mpn_mul": (ctypes.c_int, [_P64, _P64, _s64, _P64, _s64]),
     "fa_mpn_mul_device": (ctypes.c_int, [_VP, _VP, _s64, _VP, _s64, _VP]),
     "fa_mpn_mul_range": (ctypes.c_int, [_P64, _u64, _u64, _P64, _s64, _P64, _s64]),
+    "fa_mpn_mul_device_noalloc": (ctypes.c_int, [_VP, _VP, _s64, _VP, _s64, _VP]),
+    "fa_noalloc_scratch_bytes": (_u64, []),
     "fa_mpn_mulmod_2expp1": (ctypes.c_int, [_P64, _P64, _P64, _s64]),
     "fa_mpn_mulmod_2expp1_device": (ctypes.c_int, [_VP, _VP, _VP, _s64, _VP]),
     "fa_mpn_mul_range_device": (ctypes.c_int, [_VP, _u64, _u64, _VP, _s64, _VP, _s64, _VP]),

@@ -376,9 +376,14 @@ const fa_tables* fa_tables_ready(int depth)
     return (T && T->depth >= depth) ? T : NULL;
 }
 
+int fa_noalloc_setup();
+
+// builds the current device's tables and resolves the device address of
+// the no-allocation entry point's scratch array
 extern "C" int fa_init(void)
 {
-    return fa_get_tables(20) ? FA_OK : FA_E_NODEV;
+    if (!fa_get_tables(20)) return FA_E_NODEV;
+    return fa_noalloc_setup();
 }
 
 // ------------------------------------------------------------------------

@@ -151,6 +151,20 @@ FA_API int fa_mpn_mul_range_device(uint64_t* z, uint64_t lo, uint64_t hi, const 
 FA_API int fa_mpn_mul_device(uint64_t* z, const uint64_t* a, int64_t an,
                       const uint64_t* b, int64_t bn, void* stream);
 
+/* fa_mpn_mul_device for callers that cannot allocate (code under HIP graph
+   capture, serving loops, harnesses that forbid hipMalloc).  No counterpart
+   in the reference.  Same kernels and output as fa_mpn_mul_device, with the
+   scratch in a static 64 MiB device array per GPU instead of allocated
+   buffers: it calls no hipMalloc, hipFree or allocating library routine.
+   Products whose scratch exceeds the array (fa_noalloc_scratch_bytes();
+   about 2^20 limbs in total) return FA_E_RANGE.  fa_init() must have run on
+   the device first (it builds the tables and resolves the array's address);
+   otherwise FA_E_ARG.  All calls on one GPU share the array: calls that
+   overlap on one GPU (other streams or threads) race. */
+FA_API int fa_mpn_mul_device_noalloc(uint64_t* z, const uint64_t* a, int64_t an,
+                                     const uint64_t* b, int64_t bn, void* stream);
+FA_API uint64_t fa_noalloc_scratch_bytes(void);
+
 /* the same with a caller-chosen plan: np primes, bits-bit chunks (checked
    against the exactness bound of mpn_mul.c:79-95); the autotuner's knob */
 FA_API int fa_mpn_mul_plan(uint64_t* z, const uint64_t* a, int64_t an,
