@@ -40,10 +40,11 @@ class Request(ctypes.Structure):
                 ("a", _VP), ("b", _VP), ("an", _s64), ("bn", _s64), ("aw", _s64), ("bw", _s64),
                 ("z", _VP), ("zl", _s64), ("zh", _s64), ("zw", _s64), ("mod", NmodT),
                 ("abits", _s64), ("bbits", _s64), ("np", ctypes.c_int32), ("bits", ctypes.c_int32),
-                ("depth", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("depth", ctypes.c_int32), ("trunc", ctypes.c_int32)]
 
 
 REQ_MPN, REQ_NMOD, REQ_FMPZ = 0, 1, 2
+TRUNC_AUTO, TRUNC_OFF, TRUNC_ON = 0, 1, 2
 BATCH_DEVICE, BATCH_GRAPH, BATCH_FMPZ_KS = 1, 2, 4
 
 
@@ -339,9 +340,10 @@ def set_col_variant(v):
     lib().fa_set_col_variant(int(v))
 
 
-def set_truncation(on=True):
-    """row-granularity truncation of 2^13..2^23-point transforms (outputs identical)"""
-    lib().fa_set_truncation(1 if on else 0)
+def set_truncation(mode=1):
+    """row-granularity truncation of 2^13..2^23-point transforms (outputs
+    identical): 0 never, 1 automatic (default), 2 wherever it applies"""
+    lib().fa_set_truncation(int(mode))
 
 
 def set_square_route(on=True):
@@ -716,6 +718,7 @@ def batch_mul(requests, ks=False):
             z = np.zeros(len(a) + len(b), dtype=np.uint64)
             r.np = q.get("np", 0)
             r.bits = q.get("bits", 0)
+            r.trunc = q.get("trunc", TRUNC_AUTO)
         elif kind == "nmod":
             r.kind = REQ_NMOD
             r.an, r.bn = len(a), len(b)
@@ -801,8 +804,8 @@ def eval_device(specs, stream=None):
 
 
 def mpn_candidates(an, bn):
-    """the 64 candidate plans (np, bits) of an x bn limbs; (0, 0) = invalid"""
-    out = np.zeros((64, 2), dtype=np.int32)
+    """the 64 candidate plans (np, bits, trunc) of an x bn limbs; (0, 0, 0) = invalid"""
+    out = np.zeros((64, 3), dtype=np.int32)
     _check(lib().fa_mpn_candidates(an, bn, out.ctypes.data))
     return out
 
@@ -859,8 +862,9 @@ class MLP:
         return losses
 
     def plan_mpn(self, shapes):
+        """learned plans: n x (np, bits, trunc)"""
         shapes = np.ascontiguousarray(shapes, dtype=np.int64)
-        out = np.zeros((len(shapes), 2), dtype=np.int32)
+        out = np.zeros((len(shapes), 3), dtype=np.int32)
         _check(lib().fa_mlp_plan_mpn(self.h, shapes.ctypes.data, len(shapes), out.ctypes.data))
         return out
 

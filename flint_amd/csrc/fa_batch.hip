@@ -456,9 +456,15 @@ extern "C" int fa_batch_create(fa_batch_s** out, fa_request* R, int64_t nreq, in
             j.trows = p.depth > FA_ROW_LG ? 1 << (p.depth - FA_ROW_LG) : 0;
             {
                 const uint64_t plen = q.kind == FA_REQ_MPN ? p.zlen : p.zn;
-                if (fa_trunc_enabled && p.depth > FA_ROW_LG && p.depth < 2 * FA_ROW_LG &&
-                    plen && plen <= ((uint64_t)1 << p.depth))
-                    j.trows = (int)((plen + (1u << FA_ROW_LG) - 1) >> FA_ROW_LG);
+                if (fa_trunc_enabled && p.depth > FA_ROW_LG && p.depth < 2 * FA_ROW_LG && plen &&
+                    plen <= ((uint64_t)1 << p.depth)) {
+                    const int d1 = p.depth - FA_ROW_LG;
+                    const int t = (int)((plen + (1u << FA_ROW_LG) - 1) >> FA_ROW_LG);
+                    const int want = q.trunc == FA_TRUNC_ON ||
+                                     (q.trunc == FA_TRUNC_AUTO &&
+                                      (fa_trunc_enabled == 2 || fa_trunc_rule(d1, t)));
+                    if (want) j.trows = t;
+                }
             }
             // normalizers depend on (np, depth, offset) only
             const int nkey = (p.np * 32 + p.depth) * 16 + p.offset;

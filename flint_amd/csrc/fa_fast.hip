@@ -1344,10 +1344,22 @@ int fa_batch_transform_bin(int depth, int kind, const fa_bjob* J, const uint64_t
     }
 }
 
-// truncated transforms (rows past the product's length skipped); switched
-// off by tests comparing them with the full transforms
+// truncated transforms (rows past the product's length skipped): 0 never,
+// 1 automatic, 2 wherever they apply (tests compare them with the full
+// transforms)
 int fa_trunc_enabled = 1;
-extern "C" FA_API void fa_set_truncation(int on) { fa_trunc_enabled = on; }
+extern "C" FA_API void fa_set_truncation(int mode)
+{
+    fa_trunc_enabled = mode < 0 ? 0 : mode > 2 ? 2 : mode;
+}
+// the automatic choice: truncate at t of 2^d1 rows only where the row pass
+// saved outweighs the slower truncated inverse (measured on MI355X: 2^23
+// points with at most 55% of the rows; elsewhere the LDS truncated inverse
+// costs more than the skipped rows, in single products and in batches)
+int fa_trunc_rule(int d1, int t)
+{
+    return d1 == 11 && 20 * t <= 11 << d1;
+}
 
 // column tiles at d1 = 12: 1 = 8 columns, 1024 threads; 2 = 4 columns, 512
 // threads, half-column LDS; 0 = 4 columns, 1024 threads, whole-column LDS
@@ -1634,8 +1646,10 @@ int fa_convolve_fast(int depth, const fa_operand& oa, const fa_operand& ob, int 
     const int nops = sqr ? 1 : 2;
     const int d1 = depth - FA_ROW_LG;
     int trows = d1 >= 1 && d1 <= 12 ? 1 << d1 : 0;
-    if (fa_trunc_enabled && zlen && zlen <= ((uint64_t)1 << depth))
-        trows = (int)((zlen + (1u << FA_ROW_LG) - 1) >> FA_ROW_LG);
+    if (fa_trunc_enabled && d1 >= 1 && d1 < 12 && zlen && zlen <= ((uint64_t)1 << depth)) {
+        const int t = (int)((zlen + (1u << FA_ROW_LG) - 1) >> FA_ROW_LG);
+        if (fa_trunc_enabled == 2 || fa_trunc_rule(d1, t)) trows = t;
+    }
 #define FA_F(D) launch_fast_any<D>(oa, ob, nops, np, A, B, stride, S, s, trows)
     FA_D1_SWITCH(d1, FA_F)
 #undef FA_F

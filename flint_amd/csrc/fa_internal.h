@@ -133,6 +133,8 @@ int fa_tbuf_reserve(fa_thread_state* st, size_t bytes, void** out, hipStream_t s
 void fa_crt_consts_init(fa_crt_consts* C, int np, int offset);
 uint64_t fa_crt_cop(int np, int i);         // (M/p_i) mod p_i
 int fa_mpn_choose_plan(uint64_t an, uint64_t bn, fa_mpn_plan_t* P);
+// automatic truncation of a 2^(12 + d1)-point transform whose product fills t rows
+int fa_trunc_rule(int d1, int t);
 int fa_mpn_fixed_plan(uint64_t an, uint64_t bn, int np, int bits, fa_mpn_plan_t* P);
 int fa_mpn_max_bits(uint64_t an, uint64_t bn, int np);
 void fa_prime_set_init(fa_prime_set* S, const fa_tables* T, int np, int depth, int offset,

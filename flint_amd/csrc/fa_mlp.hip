@@ -367,14 +367,13 @@ extern "C" int fa_mlp_plan_mpn(fa_mlp_s* m, const int64_t* shapes, int n, int32_
     int rc = fa_mlp_forward(m, X.data(), n, Y.data());
     if (rc) return rc;
     for (int k = 0; k < n; k++) {
-        int32_t cand[2 * NO];
+        int32_t cand[3 * NO];
         rc = fa_mpn_candidates(shapes[2 * k], shapes[2 * k + 1], cand);
         if (rc) return rc;
         int best = -1;
         for (int j = 0; j < NO; j++)
-            if (cand[2 * j] && (best < 0 || Y[(size_t)k * NO + j] < Y[(size_t)k * NO + best])) best = j;
-        out[2 * k] = best >= 0 ? cand[2 * best] : 0;
-        out[2 * k + 1] = best >= 0 ? cand[2 * best + 1] : 0;
+            if (cand[3 * j] && (best < 0 || Y[(size_t)k * NO + j] < Y[(size_t)k * NO + best])) best = j;
+        for (int e = 0; e < 3; e++) out[3 * k + e] = best >= 0 ? cand[3 * best + e] : 0;
     }
     return FA_OK;
 }

@@ -1,7 +1,7 @@
 """Learned planner for integer products.
 
 The exhaustive autotuner (fa_autotune_mpn) times all 64 candidate plans
-(np, bits) of a shape on the GPU; the planner network (fa_mlp_*) is trained
+(np, bits, trunc) of a shape on the GPU, as batched products; the planner network (fa_mlp_*) is trained
 on the GPU to predict the cost of every candidate from 16 shape features
 (fa_plan_features) -- the cost as log(ms / ms of the shape's fastest
 candidate), which keeps the targets on one scale across sizes -- and picks
@@ -51,17 +51,18 @@ def load_planner(path=WEIGHTS):
 
 
 def plan_requests(net, reqs):
-    """sets the (np, bits) of every integer request of a ctypes Request array
-    to the learned plan"""
+    """sets the (np, bits, trunc) of every integer request of a ctypes
+    Request array to the learned plan"""
     idx = [k for k in range(len(reqs)) if reqs[k].kind == fa.REQ_MPN and reqs[k].an > 0
            and reqs[k].bn > 0]
     if not idx:
         return
     shapes = np.array([[reqs[k].an, reqs[k].bn] for k in idx], dtype=np.int64)
     plans = net.plan_mpn(shapes)
-    for k, (np_, bits) in zip(idx, plans):
+    for k, (np_, bits, tr) in zip(idx, plans):
         reqs[k].np = int(np_)
         reqs[k].bits = int(bits)
+        reqs[k].trunc = fa.TRUNC_ON if tr else fa.TRUNC_OFF
 
 
 def sample_shapes(n, seed, lo=4.0, hi=22.0):
@@ -103,12 +104,21 @@ def train(X, ms, epochs=1000, batch=128, lr=1e-3, seed=1):
     return net, losses
 
 
+def auto_truncates(p):
+    """the automatic truncation rule (fa_trunc_rule) of a plan dict"""
+    d1 = p["depth"] - 12
+    t = -(-p["zlen"] // 4096)
+    return d1 == 11 and 20 * t <= 11 << d1
+
+
 def heuristic_index(an, bn):
-    """candidate index of the automatic plan (fa_mpn_choose_plan)"""
+    """candidate index of the automatic plan: fa_mpn_choose_plan's (np, bits)
+    and the automatic truncation rule"""
     p = fa.mpn_plan(int(an), int(bn))
     c = fa.mpn_candidates(int(an), int(bn))
-    for j, (np_, bits) in enumerate(c):
-        if np_ == p["np"] and bits == p["bits"]:
+    tr = 1 if auto_truncates(p) else 0
+    for j, (np_, bits, t) in enumerate(c):
+        if np_ == p["np"] and bits == p["bits"] and t == tr:
             return j
     raise ValueError("automatic plan is not a candidate")
 
@@ -116,9 +126,9 @@ def heuristic_index(an, bn):
 def learned_index(net, shapes):
     plans = net.plan_mpn(shapes)
     out = []
-    for (an, bn), (np_, bits) in zip(shapes, plans):
+    for (an, bn), pl in zip(shapes, plans):
         c = fa.mpn_candidates(int(an), int(bn))
-        out.append(next(j for j, (a, b) in enumerate(c) if a == np_ and b == bits))
+        out.append(next(j for j, x in enumerate(c) if tuple(x) == tuple(pl)))
     return np.array(out)
 
 
@@ -137,11 +147,28 @@ def main():
     ap.add_argument("--seed", type=int, default=0xF1A70005)
     ap.add_argument("--epochs", type=int, default=1000)
     ap.add_argument("--out", default="")
-    ap.add_argument("--from-dataset", action="store_true",
-                    help="train on the shipped dataset instead of collecting timings")
+    ap.add_argument("--from-dataset", default="", metavar="NPZ[,NPZ...]",
+                    help="train on stored datasets (comma-separated; 'shipped' = the "
+                         "package's) instead of collecting timings")
+    ap.add_argument("--collect-only", default="", metavar="NPZ",
+                    help="collect timings of --shapes shapes (seed --seed; the first "
+                         "--test-frac of them held out) into NPZ and stop")
     args = ap.parse_args()
+    if args.collect_only:
+        shapes = sample_shapes(args.shapes, args.seed)
+        _, ms = collect(shapes, args.reps, progress=25)
+        ntest = int(len(shapes) * args.test_frac)
+        np.savez(args.collect_only, train_shapes=shapes[ntest:], train_ms=ms[ntest:],
+                 test_shapes=shapes[:ntest], test_ms=ms[:ntest])
+        print(json.dumps({"shapes": int(len(shapes)), "timings": int((ms > 0).sum())}))
+        return
     if args.from_dataset:
-        trs, trm, tes, tem = load_dataset()
+        parts = [load_dataset(DATASET if f == "shipped" else f)
+                 for f in args.from_dataset.split(",")]
+        trs = np.concatenate([p[0] for p in parts])
+        trm = np.concatenate([p[1] for p in parts])
+        tes = np.concatenate([p[2] for p in parts])
+        tem = np.concatenate([p[3] for p in parts])
         Xtr = features(trs)
     else:
         shapes = sample_shapes(args.shapes, args.seed)

@@ -196,9 +196,10 @@ def with_plans(reqs, net):
     shapes = np.array([[len(reqs[k]["a"]), len(reqs[k]["b"])] for k in idx], dtype=np.int64)
     plans = net.plan_mpn(shapes)
     out = list(reqs)
-    for k, (np_, bits) in zip(idx, plans):
+    for k, (np_, bits, tr) in zip(idx, plans):
         q = dict(reqs[k])
         q["np"], q["bits"] = int(np_), int(bits)
+        q["trunc"] = fa.TRUNC_ON if tr else fa.TRUNC_OFF
         out[k] = q
     return out
 
@@ -218,6 +219,7 @@ def device_batch(reqs, torch, dev, ks=True):
             r.an, r.bn = len(q["a"]), len(q["b"])
             zwords = r.an + r.bn
             r.np, r.bits = q.get("np", 0), q.get("bits", 0)
+            r.trunc = q.get("trunc", fa.TRUNC_AUTO)
         elif kind == "nmod":
             r.kind = fa.REQ_NMOD
             r.an, r.bn = len(q["a"]), len(q["b"])
@@ -391,8 +393,9 @@ class StreamChunk:
             mi = [j for j, k in enumerate(idx) if H[k].kind == fa.REQ_MPN]
             if mi:
                 shapes = np.array([[H[idx[j]].an, H[idx[j]].bn] for j in mi], dtype=np.int64)
-                for j, (np_, bits) in zip(mi, net.plan_mpn(shapes)):
+                for j, (np_, bits, tr) in zip(mi, net.plan_mpn(shapes)):
                     R[j].np, R[j].bits = int(np_), int(bits)
+                    R[j].trunc = fa.TRUNC_ON if tr else fa.TRUNC_OFF
         self.R, self.F = R, F
         torch.cuda.synchronize()            # the arena comes from torch's stream
         fa.stream_fill_device(F)

@@ -468,11 +468,14 @@ FA_API int fa_prof_read(char* names, double* ms, int64_t* counts, int max);
 FA_API void fa_set_fast_kernels(int on);
 /* column-pass tiles at 2^24 points: 1 = 8 columns (default), 0 = 4 columns */
 FA_API void fa_set_col_variant(int v);
-/* 1 (default): transforms of 2^13 .. 2^23 points stop at the rows of the
+/* truncation of transforms of 2^13 .. 2^23 points at the rows of the
    product's length (row pass skipped past it, truncated inverse column
-   transform); 0: every transform runs all 2^depth points.  Tuning and test
-   switch; outputs are identical. */
-FA_API void fa_set_truncation(int on);
+   transform): 0 never; 1 (default) automatic: single products and batch
+   requests with FA_TRUNC_AUTO truncate only at 2^23 points with at most 55%
+   of the rows (elsewhere the truncated inverse, which runs in LDS, costs
+   more than the skipped rows save on MI355X); 2 wherever it applies.
+   Tuning and test switch; outputs are identical. */
+FA_API void fa_set_truncation(int mode);
 /* row pass variant of the two-level transforms: 1 = last-stage twiddles
    formed per radix-4 layer, three workgroups per CU (default); 0 = all 15
    held in registers, two workgroups per CU.  Tuning switch, no counterpart
@@ -543,8 +546,11 @@ typedef struct {
     int64_t abits, bbits;  /* fmpz: _fmpz_vec_max_bits of a and b; 0, 0 = measure
                               (host pointers only) */
     int32_t np, bits;      /* mpn plan override (0 = choose); out: plan used */
-    int32_t depth, pad;    /* out: transform depth */
+    int32_t depth;         /* out: transform depth */
+    int32_t trunc;         /* FA_TRUNC_*: transforms of 2^13..2^23 points truncated at the
+                              rows of the product (outputs identical either way) */
 } fa_request;
+enum { FA_TRUNC_AUTO = 0, FA_TRUNC_OFF = 1, FA_TRUNC_ON = 2 };
 
 /* the fmpz product the reference's callers compute when the eight-prime
    path declines, by Kronecker substitution onto the integer product
@@ -580,18 +586,22 @@ FA_API int fa_batch_mul(fa_request* reqs, int64_t n, int flags);
 /* ------------------------------------------------------------------------
    Plan space, autotuner and learned planner for integer products (no
    counterpart in the reference, whose fixed rule is mpn_ctx_best_profile,
-   mpn_mul.c:1248-1340).  Candidate j < 64 of an x bn limbs is
-   (np, bits) = (j / 8 + 1, bmax(np) - 8 (j mod 8)), bmax(np) the largest
-   chunk size np primes carry exactly; invalid candidates are (0, 0).
+   mpn_mul.c:1248-1340).  Candidate j < 56 of an x bn limbs is
+   (np, bits, trunc) = (j / 8 + 2, bmax(np) - 8 ((j / 2) mod 4), j mod 2),
+   bmax(np) the largest chunk size np primes carry exactly, trunc = 1 the
+   transform truncated at the product's rows (valid only where that
+   applies); candidates 56..63 and rejected sizes are (0, 0, 0).
    ------------------------------------------------------------------------ */
-FA_API int fa_mpn_candidates(int64_t an, int64_t bn, int32_t* out /* 64 x {np, bits} */);
+FA_API int fa_mpn_candidates(int64_t an, int64_t bn,
+                             int32_t* out /* 64 x {np, bits, trunc} */);
 /* 16 planner features of a shape (log sizes, the automatic plan, bmax) */
 FA_API int fa_plan_features(int64_t an, int64_t bn, float* f16);
-/* exhaustive autotuner: device time (ms) per product of every candidate:
-   K back-to-back products captured in one hipGraph per candidate, the
-   graphs replayed `reps` times in turn (replay r of every candidate before
-   replay r + 1 of any), the median replay per product (host launch
-   overhead excluded); -1 for invalid candidates */
+/* exhaustive autotuner: device time (ms) per product of every candidate,
+   as batched products: a graph-captured device batch of K copies of the
+   product per candidate (fresh SplitMix64 operands), the batches replayed
+   `reps` times in turn (replay r of every candidate before replay r + 1 of
+   any), the median replay per product (host launch overhead excluded); -1
+   for invalid candidates */
 FA_API int fa_autotune_mpn(int64_t an, int64_t bn, int reps, float* ms64);
 
 /* FP32 MLP 16 -> 64 -> 64 -> 64 predicting log(ms) of the 64 candidates */
@@ -608,7 +618,7 @@ FA_API int fa_mlp_loss_grad(fa_mlp m, const float* X, const float* T, const floa
 /* minibatch Adam on the device; loss_out[e] = mean loss of epoch e */
 FA_API int fa_mlp_train(fa_mlp m, const float* X, const float* T, const float* M, int64_t N,
                         int batch, int epochs, float lr, float* loss_out);
-/* learned plans for n shapes (an, bn): out = n x {np, bits} */
+/* learned plans for n shapes (an, bn): out = n x {np, bits, trunc} */
 FA_API int fa_mlp_plan_mpn(fa_mlp m, const int64_t* shapes, int n, int32_t* out);
 
 #ifdef __cplusplus

@@ -38,12 +38,13 @@ def test_nmod_truncated_products(d1):
         b = rng.integers(0, n, bn, dtype=np.uint64)
         zn = an + bn - 1
         print("d1 %d an %d bn %d rows %d" % (d1, an, bn, -(-zn // 4096)), flush=True)
-        got = fa.nmod_poly_mul_mid(a, b, 0, zn, n)
-        fa.set_truncation(False)
+        fa.set_truncation(2)           # wherever it applies
         try:
+            got = fa.nmod_poly_mul_mid(a, b, 0, zn, n)
+            fa.set_truncation(0)
             full = fa.nmod_poly_mul_mid(a, b, 0, zn, n)
         finally:
-            fa.set_truncation(True)
+            fa.set_truncation(1)
         assert np.array_equal(got, full), (an, bn)
         if zn <= 300000:
             assert np.array_equal(got, O.nmod_poly_mul_mid(a, b, 0, zn, n)), (an, bn)
@@ -57,12 +58,14 @@ def test_mpn_truncated_products(an, bn):
     a, b = rand_limbs(rng, an), rand_limbs(rng, bn)
     p = fa.mpn_plan(an, bn)
     print(an, bn, p, flush=True)
-    got = fa.mpn_mul(a, b)
-    fa.set_truncation(False)
+    fa.set_truncation(2)
     try:
+        got = fa.mpn_mul(a, b)
+        sq = fa.mpn_mul(a, a)
+        fa.set_truncation(0)
         full = fa.mpn_mul(a, b)
     finally:
-        fa.set_truncation(True)
+        fa.set_truncation(1)
     assert np.array_equal(got, full)
     if an + bn <= 300000:
         assert to_int(got) == to_int(a) * to_int(b)
@@ -70,7 +73,6 @@ def test_mpn_truncated_products(an, bn):
         for q in ((1 << 61) - 1, 1152921504606846883):
             assert O.mpn_mod_q(got, q) == O.mpn_mod_q(a, q) * O.mpn_mod_q(b, q) % q
     # a square takes the one-transform route through the same truncation
-    sq = fa.mpn_mul(a, a)
     if an <= 100000:
         assert to_int(sq) == to_int(a) ** 2
 
@@ -90,9 +92,10 @@ def test_batch_truncated_matches_single_calls():
         reqs.append({"kind": "nmod", "a": a, "b": b, "n": n, "zl": 0, "zh": an + bn - 1})
     for an, bn in ((6000, 5000), (90000, 30000)):
         reqs.append({"kind": "mpn", "a": rand_limbs(rng, an), "b": rand_limbs(rng, bn)})
-    res = fa.batch_mul(reqs)
-    fa.set_truncation(False)
+    fa.set_truncation(2)                    # every job truncated where it applies
     try:
+        res = fa.batch_mul(reqs)
+        fa.set_truncation(0)
         for q, (st, z) in zip(reqs, res):
             assert st == 1
             if q["kind"] == "mpn":
@@ -100,4 +103,4 @@ def test_batch_truncated_matches_single_calls():
             else:
                 assert np.array_equal(z, fa.nmod_poly_mul_mid(q["a"], q["b"], 0, q["zh"], q["n"]))
     finally:
-        fa.set_truncation(True)
+        fa.set_truncation(1)

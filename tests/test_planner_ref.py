@@ -32,11 +32,18 @@ def test_candidates_satisfy_plan_constraints():
     for an, bn in [(16, 16), (1000, 900), (65536, 65536), (1 << 22, 1 << 21), (100000, 20)]:
         c = fa.mpn_candidates(an, bn)
         auto = fa.mpn_plan(an, bn)
-        assert (auto["np"], auto["bits"]) in {(int(x), int(y)) for x, y in c}
-        for j, (np_, bits) in enumerate(c):
+        assert (auto["np"], auto["bits"]) in {(int(x), int(y)) for x, y, _ in c}
+        for j, (np_, bits, tr) in enumerate(c):
             if not np_:
                 continue
-            assert np_ == j // 8 + 1 and bits % 8 == 0 and 8 <= bits <= 200
+            assert j < 56 and np_ == j // 8 + 2 and tr == j % 2
+            assert bits % 8 == 0 and 8 <= bits <= 200
+            # the untruncated twin of a truncated candidate is valid too
+            if tr:
+                assert tuple(c[j - 1]) == (np_, bits, 0)
+                zl = int(-(-64 * an // int(bits)) + -(-64 * bn // int(bits)) - 1)
+                depth = max(4, (zl - 1).bit_length())
+                assert 13 <= depth <= 23 and -(-zl // 4096) < 1 << (depth - 12)
             M = 1
             for p in primes[:np_]:
                 M *= int(p)
