@@ -649,84 +649,123 @@ __global__ void __launch_bounds__(64) fa_k_carry_apply16(const uint8_t* __restri
     }
 }
 
-// carry-in of every group from the per-group flags and the workgroup
-// aggregates, in one workgroup of 1024 threads.  Each thread owns a
-// contiguous range of CRT workgroups (a multiple of 8, read as 8-byte
-// words), folds their (generate, propagate) aggregates, the range folds are
-// scanned in LDS, and each thread then walks its range: a workgroup that
-// receives no carry and generates none inside (bit 2 of its aggregate) is
-// skipped, any other is resolved group by group from the group flags.
+// adds 1 to the n limbs at p with carry (stops at the first limb that does
+// not wrap); eight loads in flight at a time
+FA_DEV void fa_add_one(uint64_t* __restrict__ p, int n)
+{
+    for (int l0 = 0; l0 < n; l0 += 8) {
+        const int m = n - l0 < 8 ? n - l0 : 8;
+        uint64_t v[8];
+#pragma unroll
+        for (int l = 0; l < 8; l++)
+            if (l < m) v[l] = p[l0 + l];
+#pragma unroll
+        for (int l = 0; l < 8; l++) {
+            if (l < m) {
+                v[l] += 1;
+                p[l0 + l] = v[l];
+                if (v[l] != 0) return;
+            }
+        }
+    }
+}
+
+FA_DEV uint8_t fa_gp_fold(uint8_t acc, uint8_t a)   // (generate, propagate) of acc then a
+{
+    const uint8_t g = (uint8_t)((a & 1) | (((a >> 1) & 1) & acc));
+    const uint8_t p = (uint8_t)((acc >> 1) & (a >> 1) & 1);
+    return (uint8_t)(g | (p << 1));
+}
+
+// carries between CRT workgroups ("blocks" of FA_CRT16_GROUPS - 1 groups of
+// own limbs), by workgroup wg of nwg (1024 threads each) on its contiguous
+// range of blocks.  A block's aggregate holds bit 0: carry out with no
+// carry in, bit 1: every group propagates, bit 2: some group generates.
+//  1. the carry into the range: each thread folds a slice of the aggregates
+//     before it, the slices are combined in order in LDS;
+//  2. the carry into each block of the range: a scan over the range in
+//     tiles of 1024 blocks;
+//  3. one wave per block: lane k takes group k of the block, its carry-in
+//     composed from the flags of the groups before it, and a group with a
+//     carry-in adds it to its limbs.  A group's addition stays inside its
+//     own limbs except when they are all ones (then they become zeros and
+//     the carry passes on, which the flags already give the next group), so
+//     groups are independent once their carries-in are known.  Blocks that
+//     receive no carry and generate none inside are skipped.
+// No workgroup waits on another: each computes its own prefix.
 FA_DEV void fa_carry_resolve(const uint8_t* __restrict__ flags, const uint8_t* __restrict__ agg,
                              uint64_t* __restrict__ z, uint64_t zn, int own, uint64_t ngroups,
-                             uint64_t nblk)
+                             uint64_t nblk, unsigned wg, unsigned nwg)
 {
-    __shared__ uint8_t sg[1024], sp[1024];
-    const int t = threadIdx.x;
-    const uint64_t chunk = ((nblk + 1023) / 1024 + 7) & ~UINT64_C(7);
-    const uint64_t b0 = (uint64_t)t * chunk;
-    const uint64_t b1 = (b0 + chunk < nblk) ? b0 + chunk : (b0 < nblk ? nblk : b0);
-    uint8_t g = 0, p = 1;
-    for (uint64_t b = b0; b < b1; b += 8) {
-        uint64_t w;
-        if (b + 8 <= b1) {
-            w = *(const uint64_t*)(agg + b);
-        } else {
-            w = 0;
-            for (uint64_t q = b; q < b1; q++) w |= (uint64_t)agg[q] << (8 * (q - b));
-            for (uint64_t q = b1; q < b + 8; q++) w |= (uint64_t)2 << (8 * (q - b));
-        }
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            uint8_t a = (uint8_t)(w >> (8 * q));
-            g = (uint8_t)((a & 1) | (((a >> 1) & 1) & g));
-            p = (uint8_t)(p & ((a >> 1) & 1));
+    __shared__ uint8_t sa[1024];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const uint64_t per = (nblk + nwg - 1) / nwg;
+    const uint64_t r0 = (uint64_t)wg * per < nblk ? (uint64_t)wg * per : nblk;
+    const uint64_t r1 = r0 + per < nblk ? r0 + per : nblk;
+    // 1. aggregate of blocks [0, r0), identity (no generate, propagate) = 2
+    {
+        const uint64_t sl = (r0 + 1023) / 1024;
+        const uint64_t s0 = (uint64_t)t * sl < r0 ? (uint64_t)t * sl : r0;
+        const uint64_t s1 = s0 + sl < r0 ? s0 + sl : r0;
+        uint8_t acc = 2;
+        for (uint64_t b = s0; b < s1; b++) acc = fa_gp_fold(acc, agg[b]);
+        sa[t] = acc;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            uint8_t v = 0;
+            const bool act = (t & (2 * off - 1)) == 0;
+            if (act) v = fa_gp_fold(sa[t], sa[t + off]);
+            __syncthreads();
+            if (act) sa[t] = v;
+            __syncthreads();
         }
     }
-    sg[t] = g;
-    sp[t] = p;
+    uint8_t c = sa[0] & 1;          // carry into block r0
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint8_t g1 = 0, p1 = 1;
-        if (t >= off) { g1 = sg[t - off]; p1 = sp[t - off]; }
-        __syncthreads();
-        uint8_t g2 = sg[t], p2 = sp[t];
-        sg[t] = (uint8_t)(g2 | (p2 & g1));
-        sp[t] = (uint8_t)(p1 & p2);
-        __syncthreads();
-    }
-    uint8_t c = t ? sg[t - 1] : 0;
     const uint64_t gper = FA_CRT16_GROUPS - 1;
-    for (uint64_t b = b0; b < b1; b += 8) {
-        uint64_t w;
-        if (b + 8 <= b1) {
-            w = *(const uint64_t*)(agg + b);
-        } else {
-            w = 0;
-            for (uint64_t q = b; q < b1; q++) w |= (uint64_t)agg[q] << (8 * (q - b));
-            for (uint64_t q = b1; q < b + 8; q++) w |= (uint64_t)2 << (8 * (q - b));
+    for (uint64_t tb = r0; tb < r1; tb += 1024) {
+        // 2. inclusive scan of this tile's aggregates, seeded with c
+        const uint64_t b = tb + (uint64_t)t;
+        const uint8_t a = b < r1 ? agg[b] : 2;
+        sa[t] = a;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            uint8_t v = sa[t];
+            if (t >= off) v = fa_gp_fold(sa[t - off], v);
+            __syncthreads();
+            sa[t] = v;
+            __syncthreads();
         }
-        for (int q = 0; q < 8; q++) {
-            const uint8_t a = (uint8_t)(w >> (8 * q));
-            if (c == 0 && !(a & 4)) {
-                // no carry in and none generated inside: nothing to add,
-                // no carry out
-                continue;
+        // carry into block b: fold of c with the tile's blocks before b
+        const uint8_t cin = t ? (uint8_t)(fa_gp_fold(c, sa[t - 1]) & 1) : c;
+        const uint8_t cout_tile = (uint8_t)(fa_gp_fold(c, sa[1023]) & 1);
+        __syncthreads();
+        sa[t] = (uint8_t)(cin | (a & 4));
+        __syncthreads();
+        // 3. one wave per block of the tile
+        const uint64_t nb = r1 - tb < 1024 ? r1 - tb : 1024;
+        for (uint64_t k = (uint64_t)wave; k < nb; k += 16) {
+            const uint8_t s = sa[k];
+            if (!(s & 1) && !(s & 4)) continue;
+            const uint64_t gs = (tb + k) * gper;
+            const uint64_t ge = gs + gper < ngroups ? gs + gper : ngroups;
+            const uint64_t gi = gs + (uint64_t)lane;
+            const uint8_t f = gi < ge ? flags[gi] : 2;
+            uint8_t cg = s & 1;
+            // every lane takes part in every shuffle (the block test above
+            // is the same for the whole wave)
+            for (int j = 0; j < (int)gper; j++) {
+                const uint8_t fj = (uint8_t)__shfl((int)f, j);
+                if (j < lane) cg = (uint8_t)((fj & 1) | (((fj >> 1) & 1) & cg));
             }
-            const uint64_t gs = (b + q) * gper;
-            const uint64_t ge = (gs + gper < ngroups) ? gs + gper : ngroups;
-            for (uint64_t gi = gs; gi < ge; gi++) {
-                const uint8_t f = flags[gi];
-                if (c) {
-                    const uint64_t zb = gi * own;
-                    for (int l = 0; l < own && zb + l < zn; l++) {
-                        const uint64_t v = z[zb + l] + 1;
-                        z[zb + l] = v;
-                        if (v != 0) break;
-                    }
-                }
-                c = (uint8_t)((f & 1) | (((f >> 1) & 1) & c));
+            if (cg && gi < ge) {
+                const uint64_t zb = gi * (uint64_t)own;
+                const int n = zb + (uint64_t)own <= zn ? own : (int)(zn - zb);
+                fa_add_one(z + zb, n);
             }
         }
+        c = cout_tile;
+        __syncthreads();
     }
 }
 
@@ -735,7 +774,15 @@ __global__ void __launch_bounds__(1024) fa_k_carry_fused(const uint8_t* __restri
                                                          uint64_t* __restrict__ z, uint64_t zn,
                                                          int own, uint64_t ngroups, uint64_t nblk)
 {
-    fa_carry_resolve(flags, agg, z, zn, own, ngroups, nblk);
+    fa_carry_resolve(flags, agg, z, zn, own, ngroups, nblk, blockIdx.x, gridDim.x);
+}
+
+// workgroups of the carry kernel for nblk CRT workgroups: one per 32 blocks,
+// at most 256
+static unsigned carry_grid(uint64_t nblk)
+{
+    const uint64_t g = nblk / 32;
+    return (unsigned)(g < 1 ? 1 : g > 256 ? 256 : g);
 }
 
 // ------------------------------------------------------------------------
@@ -927,7 +974,7 @@ __global__ void __launch_bounds__(1024) fa_kb_carry(const fa_bmpn* __restrict__ 
 {
     if (blockIdx.x >= njobs) return;
     const fa_bmpn& m = M[jobs[blockIdx.x]];
-    fa_carry_resolve(m.flags, m.agg, m.z, m.zn, m.bits / 8, m.ngroups, m.nblk);
+    fa_carry_resolve(m.flags, m.agg, m.z, m.zn, m.bits / 8, m.ngroups, m.nblk, 0, 1);
 }
 
 int fa_batch_crt_mpn(int np, const fa_bmpn* M, const uint64_t* work, unsigned nwork,
@@ -1004,8 +1051,8 @@ static int crt16(int np, const double* Q, uint64_t stride, const fa_crt_consts& 
         case 8: launch_crt_fp<8>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode); break;
         default: return fa_set_error(FA_E_ARG, "crt: np %d", np);
         }
-        FA_LAUNCH("fa_k_carry_fused", fa_k_carry_fused, dim3(1), dim3(1024), 0, s, flags, agg, z,
-                  zn, bits / 8, ngroups, nblk);
+        FA_LAUNCH("fa_k_carry_fused", fa_k_carry_fused, dim3(carry_grid(nblk)), dim3(1024), 0, s,
+                  flags, agg, z, zn, bits / 8, ngroups, nblk);
         return FA_OK;
     }
     switch (np) {
@@ -1019,8 +1066,8 @@ static int crt16(int np, const double* Q, uint64_t stride, const fa_crt_consts& 
     default: return fa_set_error(FA_E_ARG, "crt16: np %d", np);
     }
     (void)cin;
-    FA_LAUNCH("fa_k_carry_fused", fa_k_carry_fused, dim3(1), dim3(1024), 0, s, flags, agg, z, zn,
-              bits / 8, ngroups, nblk);
+    FA_LAUNCH("fa_k_carry_fused", fa_k_carry_fused, dim3(carry_grid(nblk)), dim3(1024), 0, s,
+              flags, agg, z, zn, bits / 8, ngroups, nblk);
     return FA_OK;
 }
 

@@ -13,10 +13,11 @@ timings.
 
 collects timings for 2000 shapes (log-uniform sizes), trains, reports the
 regret of the learned and of the fixed plan rule on held-out shapes and
-writes the weights with the dataset summary.  The package ships such a
-dataset measured on MI355X (data/planner_timings_mi355x.npz: 2000 training
-and 200 held-out shapes) and the weights trained on it
-(data/planner_weights_mi355x.npz, written by `--from-dataset --out`).
+writes the weights with the dataset summary; `--collect-only FILE` stores
+the timings of a run, `--from-dataset A,B,...` trains on stored runs.  The
+package ships a dataset measured on MI355X (data/planner_timings_mi355x.npz:
+19,008 training and 792 held-out shapes, 892,256 timings) and the weights
+trained on it (data/planner_weights_mi355x.npz).
 """
 import argparse
 import json
