@@ -110,9 +110,19 @@ FA_DEV void fa_crt_coeff_poly(const double* __restrict__ Q, uint64_t stride, uin
 // exact.  Bounds: each mulmod sees |a b| < 2 p_k^2 (primes within a factor
 // 2 of each other, the running value reduced to [-p_k/2, p_k/2] first), and
 // v (p_k mod n) + t_k < n 2^64 keeps the high word below n.
+// p_k mod n for the primes of C, computed once per workgroup into pm
+// (threads 0..7; the caller puts a barrier after it)
+FA_DEV void fa_crt_nmod_pm(const fa_crt_consts& C, uint64_t n, uint64_t dinv, int norm,
+                           uint64_t* pm)
+{
+    const int k = threadIdx.x;
+    if (k < FA_NPRIMES) pm[k] = k < C.np ? fa_ll_mod_preinv(0, (uint64_t)C.p[k], n, dinv, norm) : 0;
+}
+
 template <int NP>
 FA_DEV uint64_t fa_crt_nmod_one(const double* __restrict__ Q, uint64_t stride, uint64_t j,
-                                const fa_crt_consts& C, uint64_t n, uint64_t dinv, int norm)
+                                const fa_crt_consts& C, uint64_t n, uint64_t dinv, int norm,
+                                const uint64_t* pm)
 {
     const uint64_t pos = fa_res_pos(j, stride);
     double t[NP];
@@ -136,7 +146,7 @@ FA_DEV uint64_t fa_crt_nmod_one(const double* __restrict__ Q, uint64_t stride, u
     uint64_t v = fa_ll_mod_preinv(0, (uint64_t)t[NP - 1], n, dinv, norm);
 #pragma unroll
     for (int k = NP - 2; k >= 0; k--) {
-        const uint64_t pk = fa_ll_mod_preinv(0, (uint64_t)C.p[k], n, dinv, norm);
+        const uint64_t pk = pm[k];
         const uint64_t tk = (uint64_t)t[k];
         uint64_t lo = v * pk;
         uint64_t hi = __umul64hi(v, pk);
@@ -153,9 +163,12 @@ __global__ void __launch_bounds__(256) fa_k_crt_nmod(const double* __restrict__ 
                                                      uint64_t n, uint64_t dinv, int norm,
                                                      uint64_t* __restrict__ z)
 {
+    __shared__ uint64_t pm[FA_NPRIMES];
+    fa_crt_nmod_pm(C, n, dinv, norm, pm);
+    __syncthreads();
     uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cnt) return;
-    z[t] = fa_crt_nmod_one<NP>(Q, stride, zl + t, C, n, dinv, norm);
+    z[t] = fa_crt_nmod_one<NP>(Q, stride, zl + t, C, n, dinv, norm, pm);
 }
 
 // NP = C.np (1 .. 8) as a template argument
@@ -247,6 +260,11 @@ __global__ void __launch_bounds__(256) fa_kb_crt_poly(const fa_bpoly* __restrict
     if (L >= nwork) return;
     const uint64_t w = work[L];
     const fa_bpoly& j = P[(unsigned)(w & 0xffffffffu)];
+    __shared__ uint64_t pm[FA_NPRIMES];
+    if (!FMPZ && j.Q) {         // the same job for the whole workgroup
+        fa_crt_nmod_pm(*(const fa_crt_consts*)j.C, j.n, j.dinv, j.norm, pm);
+        __syncthreads();
+    }
     const uint64_t t = (w >> 32) * 256 + threadIdx.x;
     if (t >= j.cnt) return;
     const uint64_t idx = j.zl + t;
@@ -267,7 +285,7 @@ __global__ void __launch_bounds__(256) fa_kb_crt_poly(const fa_bpoly* __restrict
         }
         const fa_crt_consts& C = *(const fa_crt_consts*)j.C;
         uint64_t r = 0;
-#define FA_KB_NMOD(NP) r = fa_crt_nmod_one<NP>(j.Q, j.stride, idx, C, j.n, j.dinv, j.norm)
+#define FA_KB_NMOD(NP) r = fa_crt_nmod_one<NP>(j.Q, j.stride, idx, C, j.n, j.dinv, j.norm, pm)
         FA_CRT_NP_SWITCH(C.np, FA_KB_NMOD)
 #undef FA_KB_NMOD
         j.z[t] = r;
