@@ -111,6 +111,9 @@ SIGNATURES = {
     "fa_mpn_mulmod_2expp1": (ctypes.c_int, [_P64, _P64, _P64, _s64]),
     "fa_mpn_mulmod_2expp1_device": (ctypes.c_int, [_VP, _VP, _VP, _s64, _VP]),
     "fa_mpn_mul_range_device": (ctypes.c_int, [_VP, _u64, _u64, _VP, _s64, _VP, _s64, _VP]),
+    "fa_mpn_mul_range_threads": (ctypes.c_int, [_VP, _u64, _u64, _VP, _s64, _VP, _s64,
+                                                ctypes.c_int, _VP]),
+    "fa_mpn_window_plan": (ctypes.c_int, [_u64, _u64, _u64, ctypes.c_int, _P64]),
     "fa_mpn_plan": (ctypes.c_int, [_s64, _s64, _P64]),
     "fa_mpn_mul_plan": (ctypes.c_int, [_P64, _P64, _s64, _P64, _s64, ctypes.c_int, ctypes.c_int]),
     "fa_mpn_mul_device_plan": (ctypes.c_int, [_VP, _VP, _s64, _VP, _s64, ctypes.c_int,
@@ -376,7 +379,8 @@ def mpn_mul(a, b):
 
 
 def mpn_mul_range(a, b, lo, hi):
-    """limbs [lo, hi) of a*b (_mpn_ctx_mpn_mul_range, flint_mpn_mulmid_fft_small)"""
+    """limbs [lo, hi) of a*b as _mpn_ctx_mpn_mul_range returns them
+    (flint_mpn_mulmid_fft_small; FLINT on one thread)"""
     a = _arr(a)
     b = _arr(b)
     z = np.empty(hi - lo, dtype=np.uint64)
@@ -386,6 +390,19 @@ def mpn_mul_range(a, b, lo, hi):
 
 def mpn_mul_range_device(z_ptr, lo, hi, a_ptr, an, b_ptr, bn, stream=None):
     return _check(lib().fa_mpn_mul_range_device(z_ptr, lo, hi, a_ptr, an, b_ptr, bn, stream))
+
+
+def mpn_mul_range_threads(z_ptr, lo, hi, a_ptr, an, b_ptr, bn, nthreads, stream=None):
+    """device window with the words of FLINT on nthreads threads (0: exact)"""
+    return _check(lib().fa_mpn_mul_range_threads(z_ptr, lo, hi, a_ptr, an, b_ptr, bn, nthreads,
+                                                 stream))
+
+
+def mpn_window_plan(an, bn, lo, nthreads=1):
+    """(two_prime, bits, k0, nk): the reference's plan as the window sees it"""
+    out = np.zeros(4, dtype=np.int64)
+    _check(lib().fa_mpn_window_plan(an, bn, lo, nthreads, out.ctypes.data_as(_P64)))
+    return tuple(int(x) for x in out)
 
 
 def mpn_mulmod_2expp1(x, y, n):

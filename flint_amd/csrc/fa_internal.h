@@ -131,6 +131,14 @@ int fa_scratch_reserve(fa_thread_state* st, size_t bytes, void** out, hipStream_
 int fa_hbuf_reserve(fa_thread_state* st, size_t bytes, void** out, hipStream_t s);
 int fa_tbuf_reserve(fa_thread_state* st, size_t bytes, void** out, hipStream_t s);
 void fa_crt_consts_init(fa_crt_consts* C, int np, int offset);
+
+// the reference's integer windows from the exact product (fa_window.hip):
+// work bytes for (an, bn, lo) under FLINT's plan on nthreads threads (0: no
+// correction), and the correction of the window zw = [lo, hi) of P
+size_t fa_window_work_bytes(uint64_t an, uint64_t bn, uint64_t lo, int nthreads);
+int fa_window_fix(uint64_t* zw, const uint64_t* P, uint64_t lo, uint64_t hi, const uint64_t* a,
+                  uint64_t an, const uint64_t* b, uint64_t bn, int nthreads, void* work,
+                  hipStream_t s);
 uint64_t fa_crt_cop(int np, int i);         // (M/p_i) mod p_i
 int fa_mpn_choose_plan(uint64_t an, uint64_t bn, fa_mpn_plan_t* P);
 // automatic truncation of a 2^(12 + d1)-point transform whose product fills t rows

@@ -1815,12 +1815,36 @@ extern "C" int fa_mpn_mulmod_2expp1(uint64_t* z, const uint64_t* x, const uint64
 }
 
 // _mpn_ctx_mpn_mul_range (mpn_mul.c:1617-1978) and flint_mpn_mulmid_fft_small
-// (mpn_extras/mulmid_fft_small.c:26-33): limbs [lo, hi) of a*b.  The
-// reference returns a lower approximation (partial products landing wholly
-// below lo dropped, deficit below 2^64 min(an, bn, lo) in units of limb lo,
-// doc/source/mpn_extras.rst:221-231); this window is exact, the tightest
-// approximation that contract admits.  The product is formed on the device
-// and only the window crosses PCIe.
+// (mpn_extras/mulmid_fft_small.c:26-33): limbs [lo, hi) of a*b as the
+// reference returns them, a lower approximation whose dropped carry
+// depends on FLINT's plan for (an, bn) (doc/source/mpn_extras.rst:221-231).
+// The exact product is formed on the device and the reference's words are
+// recovered from it by an exact correction (fa_window.hip); nthreads is the
+// FLINT thread count whose plan the words follow, 0 for the exact window.
+static int mpn_window_device(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a,
+                             uint64_t an, const uint64_t* b, uint64_t bn, int nthreads,
+                             fa_thread_state* st, hipStream_t s)
+{
+    const uint64_t zn = an + bn;
+    const size_t wb = fa_window_work_bytes(an, bn, lo, nthreads);
+    // the window's own buffer (block products, mpn_mul_blocked, use tbuf):
+    // the full product, then the correction's work space
+    void* hb;
+    int rc = fa_hbuf_reserve(st, zn * sizeof(uint64_t) + wb, &hb, s);
+    if (rc) return rc;
+    uint64_t* dz = (uint64_t*)hb;
+    rc = mpn_mul_device(dz, a, an, b, bn, s);
+    if (rc) return rc;
+    if (wb) {
+        rc = fa_window_fix(dz + lo, dz, lo, hi, a, an, b, bn, nthreads, dz + zn, s);
+        if (rc) return rc;
+    }
+    if (hipMemcpyAsync(z, dz + lo, (hi - lo) * sizeof(uint64_t), hipMemcpyDeviceToDevice, s) !=
+        hipSuccess)
+        return fa_set_error(FA_E_HIP, "window copy failed");
+    return fa_buffers_leave(st, s);
+}
+
 extern "C" int fa_mpn_mul_range(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a,
                                 int64_t an, const uint64_t* b, int64_t bn)
 {
@@ -1829,8 +1853,9 @@ extern "C" int fa_mpn_mul_range(uint64_t* z, uint64_t lo, uint64_t hi, const uin
     fa_thread_state* st = fa_state();
     if (!st) return FA_E_NODEV;
     const uint64_t zn = (uint64_t)(an + bn);
+    const size_t wb = fa_window_work_bytes((uint64_t)an, (uint64_t)bn, lo, 1);
     void* hb;
-    int rc = fa_hbuf_reserve(st, ((size_t)an + (size_t)bn + zn) * sizeof(uint64_t), &hb,
+    int rc = fa_hbuf_reserve(st, ((size_t)an + (size_t)bn + zn) * sizeof(uint64_t) + wb, &hb,
                              st->stream);
     if (rc) return rc;
     uint64_t* da = (uint64_t*)hb;
@@ -1843,6 +1868,9 @@ extern "C" int fa_mpn_mul_range(uint64_t* z, uint64_t lo, uint64_t hi, const uin
                                 st->stream) != hipSuccess))
         return fa_set_error(FA_E_HIP, "operand upload failed");
     rc = mpn_mul_device(dz, da, (uint64_t)an, db, (uint64_t)bn, st->stream);
+    if (rc == FA_OK && wb)
+        rc = fa_window_fix(dz + lo, dz, lo, hi, da, (uint64_t)an, db, (uint64_t)bn, 1, dz + zn,
+                           st->stream);
     if (rc) {
         (void)hipStreamSynchronize(st->stream);
         return rc;
@@ -1856,30 +1884,26 @@ extern "C" int fa_mpn_mul_range(uint64_t* z, uint64_t lo, uint64_t hi, const uin
     return FA_OK;
 }
 
-// the same on device operands: the product goes to the thread's block
-// buffer and the window is copied to z on `stream`
+// the same on device operands, enqueued on `stream`
 extern "C" int fa_mpn_mul_range_device(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a,
                                        int64_t an, const uint64_t* b, int64_t bn, void* stream)
 {
-    if (an <= 0 || bn <= 0 || !a || !b || lo >= hi || hi > (uint64_t)(an + bn) || !z)
-        return fa_set_error(FA_E_ARG, "fa_mpn_mul_range_device: bad arguments");
+    return fa_mpn_mul_range_threads(z, lo, hi, a, an, b, bn, 1, stream);
+}
+
+extern "C" int fa_mpn_mul_range_threads(uint64_t* z, uint64_t lo, uint64_t hi,
+                                        const uint64_t* a, int64_t an, const uint64_t* b,
+                                        int64_t bn, int nthreads, void* stream)
+{
+    if (an <= 0 || bn <= 0 || !a || !b || lo >= hi || hi > (uint64_t)(an + bn) || !z ||
+        nthreads < 0)
+        return fa_set_error(FA_E_ARG, "fa_mpn_mul_range_threads: bad arguments");
     fa_thread_state* st = fa_state();
     if (!st) return FA_E_NODEV;
     hipStream_t s = stream ? (hipStream_t)stream : st->stream;
     const uint64_t zn = (uint64_t)(an + bn);
     if (lo == 0 && hi == zn) return mpn_mul_device_call(z, a, an, b, bn, s, 0, 0);
-    // the window's own buffer: block products (mpn_mul_blocked) use tbuf,
-    // so the full product goes to hbuf here
-    void* hb;
-    int rc = fa_hbuf_reserve(st, zn * sizeof(uint64_t), &hb, s);
-    if (rc) return rc;
-    uint64_t* dz = (uint64_t*)hb;
-    rc = mpn_mul_device(dz, a, (uint64_t)an, b, (uint64_t)bn, s);
-    if (rc) return rc;
-    if (hipMemcpyAsync(z, dz + lo, (hi - lo) * sizeof(uint64_t), hipMemcpyDeviceToDevice, s) !=
-        hipSuccess)
-        return fa_set_error(FA_E_HIP, "window copy failed");
-    return fa_buffers_leave(st, s);
+    return mpn_window_device(z, lo, hi, a, (uint64_t)an, b, (uint64_t)bn, nthreads, st, s);
 }
 
 extern "C" int fa_mpn_mul_plan(uint64_t* z, const uint64_t* a, int64_t an, const uint64_t* b,

@@ -129,11 +129,25 @@ FA_API int fa_mpn_mul(uint64_t* z, const uint64_t* a, int64_t an,
 
 /* _mpn_ctx_mpn_mul_range, mpn_mul.c:1617-1978, as flint_mpn_mulmid_fft_small
    (mpn_extras/mulmid_fft_small.c:26-33) calls it: z[0, hi-lo) = limbs
-   [lo, hi) of a*b, 0 <= lo < hi <= an+bn.  Exact: the reference's contract
-   is a lower approximation whose deficit is at most one carry from below
-   lo (doc/source/mpn_extras.rst:221-231); the exact window meets it. */
+   [lo, hi) of a*b, 0 <= lo < hi <= an+bn, word for word as the reference
+   returns them with FLINT on one thread (its default): a lower
+   approximation that drops carries from below lo as the reference's plan
+   for (an, bn) does (mpn_mul_np2.c:380-402 per chunk for max(an, bn) <=
+   40000, mpn_mul.c:1669-1978 below coefficient c_lo otherwise; deficit at
+   most one carry, doc/source/mpn_extras.rst:221-231). */
 FA_API int fa_mpn_mul_range(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a,
                             int64_t an, const uint64_t* b, int64_t bn);
+/* the same on device operands and output, enqueued on `stream`, with the
+   words of FLINT running nthreads threads (its plan, mpn_mul.c:1248-1340,
+   depends on the thread count); nthreads = 0 gives the exact window */
+FA_API int fa_mpn_mul_range_threads(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a,
+                                    int64_t an, const uint64_t* b, int64_t bn, int nthreads,
+                                    void* stream);
+/* the reference's plan as the window sees it: out4 = {two-prime plan,
+   chunk bits, first boundary coefficient, boundary coefficients (0: the
+   window is exact)} */
+FA_API int fa_mpn_window_plan(uint64_t an, uint64_t bn, uint64_t lo, int nthreads,
+                              int64_t* out4);
 /* sd_fft_mpn_mulmod_2expp1 (negacyclic.c:491-501), the pointwise stage of
    FLINT's Schoenhage-Strassen convolutions (fmpz_poly/mul_SS.c):
    z[0, n] = x y mod 2^(64 n) + 1 for x, y of n + 1 limbs with values at
@@ -143,7 +157,8 @@ FA_API int fa_mpn_mul_range(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_
 FA_API int fa_mpn_mulmod_2expp1(uint64_t* z, const uint64_t* x, const uint64_t* y, int64_t n);
 FA_API int fa_mpn_mulmod_2expp1_device(uint64_t* z, const uint64_t* x, const uint64_t* y,
                                        int64_t n, void* stream);
-/* the same with device operands and output, enqueued on `stream` */
+/* fa_mpn_mul_range with device operands and output, enqueued on `stream`
+   (FLINT's one-thread words) */
 FA_API int fa_mpn_mul_range_device(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a,
                                    int64_t an, const uint64_t* b, int64_t bn, void* stream);
 

@@ -727,9 +727,18 @@ static void crt_combine(const crt_t* C, u64* const* res, u64 j, u64* s)
     while (ml_cmp(s, C->M) >= 0) ml_sub(s, s, C->M);
 }
 
-/* _mpn_ctx_mpn_mul_range with lo = 0, hi = an + bn (mpn_mul.c:1617-1978)
-   and the two-prime window (mpn_mul_np2.c:247-310) */
-int or_mpn_mul(u64* z, const u64* a, u64 an, const u64* b, u64 bn, u64 nthreads)
+/* the coefficient stage of _mpn_ctx_mpn_mul_range (mpn_mul.c:1657-1858)
+   and of the two-prime window (mpn_mul_np2.c:247-310): residues of the
+   chunk convolution for each prime of the reference's plan, ready for
+   crt_combine.  Returns the number of primes (0: no plan). */
+typedef struct {
+    int np;
+    u64 bits, zlen;
+    crt_t C;
+    u64* res[NCRTS];
+} coeffs_t;
+
+static int mpn_coeffs(coeffs_t* K, const u64* a, u64 an, const u64* b, u64 bn, u64 nthreads)
 {
     u64 plan[6];
     if (an == 0 || bn == 0) return 0;
@@ -737,32 +746,128 @@ int or_mpn_mul(u64* z, const u64* a, u64 an, const u64* b, u64 bn, u64 nthreads)
     int np = (int)plan[0];
     u64 bits = plan[1], depth = plan[2], ztrunc = plan[3];
     u64 alen = cdiv(64 * an, bits), blen = cdiv(64 * bn, bits);
-    u64 zlen = alen + blen - 1;
     u64 N = UINT64_C(1) << depth;
     const u64* P = default_primes();
-    crt_t C;
-    crt_init(&C, P, np);
-    u64* res[NCRTS];
+    K->np = np;
+    K->bits = bits;
+    K->zlen = alen + blen - 1;
+    crt_init(&K->C, P, np);
     u64* B = (u64*)malloc(N * sizeof(u64));
     for (int l = 0; l < np; l++) {
         u64 p = P[l];
-        res[l] = (u64*)malloc(N * sizeof(u64));
-        or_mpn_to_residues(p, res[l], N, a, an, bits);
+        K->res[l] = (u64*)malloc(N * sizeof(u64));
+        or_mpn_to_residues(p, K->res[l], N, a, an, bits);
         or_mpn_to_residues(p, B, N, b, bn, bits);
         /* normalizer m = (cop * 2^depth)^-1 (mpn_mul.c:1517-1518) */
-        u64 m = invm(mulm(C.cop[l], powm(2, depth, p), p), p);
-        conv_mod_p(p, res[l], round_up(alen, BLK_SZ), B, round_up(blen, BLK_SZ),
+        u64 m = invm(mulm(K->C.cop[l], powm(2, depth, p), p), p);
+        conv_mod_p(p, K->res[l], round_up(alen, BLK_SZ), B, round_up(blen, BLK_SZ),
                    depth, ztrunc, m);
     }
     free(B);
+    return np;
+}
+
+static void coeffs_clear(coeffs_t* K)
+{
+    for (int l = 0; l < K->np; l++) free(K->res[l]);
+}
+
+/* _mpn_ctx_mpn_mul_range with lo = 0, hi = an + bn (mpn_mul.c:1617-1978)
+   and the two-prime window (mpn_mul_np2.c:247-310) */
+int or_mpn_mul(u64* z, const u64* a, u64 an, const u64* b, u64 bn, u64 nthreads)
+{
+    coeffs_t K;
+    if (!mpn_coeffs(&K, a, an, b, bn, nthreads)) return 0;
     u64 zn = an + bn;
     memset(z, 0, zn * sizeof(u64));
     u64 s[ML];
-    for (u64 j = 0; j < zlen; j++) {
-        crt_combine(&C, res, j, s);
-        add_shifted(z, zn, s, ML, j * bits);
+    for (u64 j = 0; j < K.zlen; j++) {
+        crt_combine(&K.C, K.res, j, s);
+        add_shifted(z, zn, s, ML, j * K.bits);
     }
-    for (int l = 0; l < np; l++) free(res[l]);
+    coeffs_clear(&K);
+    return 1;
+}
+
+/* limbs of M * np, the reference's coeff_len (mpn_mul.c:1105-1117) */
+static u64 coeff_len(const crt_t* C)
+{
+    u64 t[ML];
+    ml_mul_1(t, C->M, (u64)C->np);
+    u64 n = ML;
+    while (n > 1 && t[n - 1] == 0) n--;
+    return n;
+}
+
+/* s >> sh for an ML-limb s */
+static void ml_rshift(u64* r, const u64* s, u64 sh)
+{
+    u64 q = sh / 64, b = sh % 64;
+    for (u64 i = 0; i < ML; i++) {
+        u64 lo = (i + q < ML) ? s[i + q] : 0;
+        u64 hi = (i + q + 1 < ML) ? s[i + q + 1] : 0;
+        r[i] = b ? ((lo >> b) | (hi << (64 - b))) : lo;
+    }
+}
+
+/* _mpn_ctx_mpn_mul_range(R, z, lo, hi, a, an, b, bn) with FLINT running
+   nthreads threads: z[0, hi - lo) = the reference's limbs [lo, hi).
+   - two-prime plans (max(an, bn) <= 40000, mpn_mul_np2.c:247-433): every
+     chunk coefficient c_j at bit j*bits contributes floor(c_j 2^(j bits) /
+     2^(64 lo)); the low bits of the chunks straddling bit 64 lo are
+     masked off one chunk at a time (mpn_mul_np2.c:380-402);
+   - wider plans (mpn_mul.c:1669-1978): the coefficients j >= c_lo,
+     c_lo = ceil((lo - n) 64 / bits) for lo > n (n = coeff_len, else 0),
+     are summed exactly (the bottom band into a buffer anchored at limb
+     floor(c_lo bits / 64), the rest in place) and limbs [lo, hi) of that
+     sum are returned: carries of the coefficients below c_lo are lost;
+   - limbs at or above an + bn are zero (mpn_mul.c:1636-1645). */
+int or_mpn_mul_range(u64* z, u64 lo, u64 hi, const u64* a, u64 an, const u64* b, u64 bn,
+                     u64 nthreads)
+{
+    if (an == 0 || bn == 0) return 0;
+    if (lo >= hi) return 1;
+    u64 zn = an + bn;
+    memset(z, 0, (hi - lo) * sizeof(u64));
+    if (lo >= zn) return 1;
+    if (hi > zn) hi = zn;
+    coeffs_t K;
+    if (!mpn_coeffs(&K, a, an, b, bn, nthreads)) return 0;
+    const u64 w = hi - lo, bits = K.bits;
+    u64 s[ML], v[ML];
+    if (K.np == 2) {
+        for (u64 j = 0; j < K.zlen; j++) {
+            u64 pos = j * bits;
+            if (pos >= 64 * hi) break;
+            crt_combine(&K.C, K.res, j, s);
+            if (pos < 64 * lo) {
+                ml_rshift(v, s, 64 * lo - pos);
+                add_shifted(z, w, v, ML, 0);
+            } else {
+                add_shifted(z, w, s, ML, pos - 64 * lo);
+            }
+        }
+    } else {
+        u64 nn = coeff_len(&K.C);
+        u64 c_lo = 0;
+        if (lo > nn) {
+            c_lo = cdiv((lo - nn) * 64, bits);
+            if (c_lo > K.zlen) c_lo = K.zlen;
+        }
+        u64 c_hi = cdiv(64 * hi, bits);
+        if (c_hi > K.zlen) c_hi = K.zlen;
+        if (c_lo < c_hi) {
+            u64 L0 = c_lo * bits / 64, tn = hi - L0;
+            u64* t = (u64*)calloc(tn, sizeof(u64));
+            for (u64 j = c_lo; j < c_hi; j++) {
+                crt_combine(&K.C, K.res, j, s);
+                add_shifted(t, tn, s, ML, j * bits - 64 * L0);
+            }
+            memcpy(z, t + (lo - L0), w * sizeof(u64));
+            free(t);
+        }
+    }
+    coeffs_clear(&K);
     return 1;
 }
 

@@ -64,6 +64,9 @@ int or_mpn_mul(uint64_t* z, const uint64_t* a, uint64_t an,
                const uint64_t* b, uint64_t bn, uint64_t nthreads);
 void or_mpn_mul_basecase(uint64_t* z, const uint64_t* a, uint64_t an,
                          const uint64_t* b, uint64_t bn);
+/* _mpn_ctx_mpn_mul_range's limbs [lo, hi) with FLINT on nthreads threads */
+int or_mpn_mul_range(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a, uint64_t an,
+                     const uint64_t* b, uint64_t bn, uint64_t nthreads);
 int or_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh,
                          const uint64_t* a, int64_t an,
                          const uint64_t* b, int64_t bn, or_nmod_t mod);

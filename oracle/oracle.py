@@ -73,6 +73,8 @@ def lib():
         L.or_mpn_mul.argtypes = [P64, P64, u64, P64, u64, u64]
         L.or_mpn_mul.restype = ctypes.c_int
         L.or_mpn_mul_basecase.argtypes = [P64, P64, u64, P64, u64]
+        L.or_mpn_mul_range.argtypes = [P64, u64, u64, P64, u64, P64, u64, u64]
+        L.or_mpn_mul_range.restype = ctypes.c_int
         L.or_nmod_poly_mul_mid.argtypes = [P64, s64, s64, P64, s64, P64, s64, NmodT]
         L.or_nmod_poly_mul_mid.restype = ctypes.c_int
         L.or_fmpz_poly_mul_mid.argtypes = [P64, u64, s64, s64, P64, u64, s64, P64, u64, s64]
@@ -160,6 +162,16 @@ def mpn_mul(a, b, nthreads=1):
     z = np.zeros(len(a) + len(b), dtype=np.uint64)
     assert lib().or_mpn_mul(_ptr(z), _ptr(a), len(a), _ptr(b), len(b), nthreads)
     return z
+
+
+def mpn_mul_range(a, b, lo, hi, nthreads=1):
+    """limbs [lo, hi) as _mpn_ctx_mpn_mul_range returns them (FLINT on
+    nthreads threads)"""
+    a = _u64(a)
+    b = _u64(b)
+    z = np.zeros(max(hi - lo, 1), dtype=np.uint64)
+    assert lib().or_mpn_mul_range(_ptr(z), lo, hi, _ptr(a), len(a), _ptr(b), len(b), nthreads)
+    return z[:hi - lo]
 
 
 def mpn_mul_basecase(a, b):
