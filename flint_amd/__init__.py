@@ -175,6 +175,7 @@ SIGNATURES = {
     "fa_set_crt_variant": (None, [ctypes.c_int]),
     "fa_set_square_route": (None, [ctypes.c_int]),
     "fa_plan_init_mpn": (ctypes.c_int, [_VP, _u64, _u64, _u64, ctypes.c_int]),
+    "fa_plan_init_mpn_np": (ctypes.c_int, [_VP, _u64, _u64, _u64, ctypes.c_int, _u64]),
     "fa_plan_init_nmod": (ctypes.c_int, [_VP, _u64, _u64, _u64, _u64, _u64, _u64, NmodT]),
     "fa_plan_clear": (None, [_VP]),
     "fa_op_init": (ctypes.c_int, [_VP, _VP]),
@@ -193,6 +194,7 @@ SIGNATURES = {
     "fa_ifft": (ctypes.c_int, [_VP, _VP, _VP]),
     "fa_export_mpn": (ctypes.c_int, [_VP, _s64, _VP, _VP, _VP]),
     "fa_export_mpn_signed": (ctypes.c_int, [_VP, _s64, _VP, _VP, _u64, _VP, _VP]),
+    "fa_export_mpn_signed_trunc": (ctypes.c_int, [_VP, _s64, _VP, _VP, _u64, _u64, _VP, _VP]),
     "fa_export_nmod": (ctypes.c_int, [_VP, _VP, NmodT, _VP, _VP]),
     "fa_export_nmod_range": (ctypes.c_int, [_VP, _VP, _u64, _u64, NmodT, _VP, _VP]),
     "fa_set_device_memory_limit": (None, [_u64]),
@@ -512,9 +514,15 @@ class Plan:
         self.ptr = ctypes.addressof(self.s)
 
     @classmethod
-    def mpn(cls, an_max, bn_max, len_bound=1, signed=False):
+    def mpn(cls, an_max, bn_max, len_bound=1, signed=False, np=None):
+        """np: None or 0 the automatic choice, 2 two primes, else three or more
+        (fft_small_plan_init_mpn_np)"""
         P = cls()
-        _check(lib().fa_plan_init_mpn(P.ptr, an_max, bn_max, len_bound, 1 if signed else 0))
+        if np is None:
+            _check(lib().fa_plan_init_mpn(P.ptr, an_max, bn_max, len_bound, 1 if signed else 0))
+        else:
+            _check(lib().fa_plan_init_mpn_np(P.ptr, an_max, bn_max, len_bound,
+                                             1 if signed else 0, np))
         return P
 
     @classmethod
@@ -584,6 +592,11 @@ class Op:
     def export_mpn_signed(self, z_ptr, zn, sign_ptr, nslots, stream=None):
         return _check(lib().fa_export_mpn_signed(z_ptr, zn, sign_ptr, self.ptr, nslots,
                                                  self.plan.ptr, stream))
+
+    def export_mpn_signed_trunc(self, z_ptr, zn, sign_ptr, nslots, lo, stream=None):
+        """limbs [lo, lo + zn) of |value| with the reference's truncation"""
+        return _check(lib().fa_export_mpn_signed_trunc(z_ptr, zn, sign_ptr, self.ptr, nslots,
+                                                       lo, self.plan.ptr, stream))
 
     def export_nmod(self, z_ptr, n, zl=None, zh=None, stream=None):
         if zl is None:

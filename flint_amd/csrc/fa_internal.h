@@ -136,6 +136,8 @@ void fa_crt_consts_init(fa_crt_consts* C, int np, int offset);
 // work bytes for (an, bn, lo) under FLINT's plan on nthreads threads (0: no
 // correction), and the correction of the window zw = [lo, hi) of P
 size_t fa_window_work_bytes(uint64_t an, uint64_t bn, uint64_t lo, int nthreads);
+// limbs of M np for the first np default primes (the reference's coeff_len)
+int fa_ref_coeff_len(int np);
 int fa_window_fix(uint64_t* zw, const uint64_t* P, uint64_t lo, uint64_t hi, const uint64_t* a,
                   uint64_t an, const uint64_t* b, uint64_t bn, int nthreads, void* work,
                   hipStream_t s);

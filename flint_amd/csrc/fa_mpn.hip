@@ -814,7 +814,8 @@ template <int NP, int MODE = 0>
 FA_DEV void fa_crt_fp_block(const double* __restrict__ Q, uint64_t stride,
                             const fa_crt_fp_consts& F, uint64_t zlen, int bits,
                             uint64_t* __restrict__ z, uint64_t zn, uint64_t ngroups,
-                            uint8_t* __restrict__ flags, uint8_t* __restrict__ agg, uint64_t blk)
+                            uint8_t* __restrict__ flags, uint8_t* __restrict__ agg, uint64_t blk,
+                            uint64_t jlo = 0)
 {
     constexpr int NL = fa_crt_fp_dims<NP>::NL;
     constexpr int NLS = NL + 1;
@@ -828,9 +829,10 @@ FA_DEV void fa_crt_fp_block(const double* __restrict__ Q, uint64_t stride,
     const int64_t g0 = (int64_t)blk * (FA_CRT16_GROUPS - 1) - 1;
 
     {   // 1. the coefficient of thread t, shifted to its bit offset in the group
+        //    (coefficients below jlo read as zero)
         const int64_t j = g0 * 8 + t;
         uint64_t L[NL];
-        if (j >= 0 && (uint64_t)j < zlen) {
+        if (j >= (int64_t)jlo && (uint64_t)j < zlen) {
             fa_crt_fp_value<NP, MODE>(Q, stride, (uint64_t)j, F, L);
         } else {
 #pragma unroll
@@ -942,9 +944,10 @@ __global__ void __launch_bounds__(256) fa_k_crt_fp(const double* __restrict__ Q,
                                                    fa_crt_fp_consts F, uint64_t zlen, int bits,
                                                    uint64_t* __restrict__ z, uint64_t zn,
                                                    uint64_t ngroups, uint8_t* __restrict__ flags,
-                                                   uint8_t* __restrict__ agg)
+                                                   uint8_t* __restrict__ agg, uint64_t jlo)
 {
-    fa_crt_fp_block<NP, MODE>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, blockIdx.x);
+    fa_crt_fp_block<NP, MODE>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, blockIdx.x,
+                              jlo);
 }
 
 // batched CRT of integer products: work item = job | block << 32
@@ -1010,17 +1013,17 @@ template <int NP>
 static void launch_crt_fp(const double* Q, uint64_t stride, const fa_crt_fp_consts& F,
                           uint64_t zlen, int bits, uint64_t* z, uint64_t zn, uint64_t ngroups,
                           uint8_t* flags, uint8_t* agg, uint64_t nblk, hipStream_t s,
-                          int mode = 0)
+                          int mode = 0, uint64_t jlo = 0)
 {
     if (mode == 1)
         FA_LAUNCH("fa_k_crt_fp", (fa_k_crt_fp<NP, 1>), dim3((unsigned)nblk), dim3(256), 0, s, Q,
-                  stride, F, zlen, bits, z, zn, ngroups, flags, agg);
+                  stride, F, zlen, bits, z, zn, ngroups, flags, agg, jlo);
     else if (mode == 2)
         FA_LAUNCH("fa_k_crt_fp", (fa_k_crt_fp<NP, 2>), dim3((unsigned)nblk), dim3(256), 0, s, Q,
-                  stride, F, zlen, bits, z, zn, ngroups, flags, agg);
+                  stride, F, zlen, bits, z, zn, ngroups, flags, agg, jlo);
     else
         FA_LAUNCH("fa_k_crt_fp", fa_k_crt_fp<NP>, dim3((unsigned)nblk), dim3(256), 0, s, Q,
-                  stride, F, zlen, bits, z, zn, ngroups, flags, agg);
+                  stride, F, zlen, bits, z, zn, ngroups, flags, agg, jlo);
 }
 
 template <int NP>
@@ -1036,19 +1039,19 @@ static void launch_crt16(const double* Q, uint64_t stride, const fa_crt_consts& 
 // negative parts of the centred value (FP64 route only)
 static int crt16(int np, const double* Q, uint64_t stride, const fa_crt_consts& C, uint64_t zlen,
                  int bits, uint64_t* z, uint64_t zn, uint64_t ngroups, uint8_t* flags, uint8_t* agg,
-                 uint8_t* cin, uint64_t nblk, hipStream_t s, int mode = 0)
+                 uint8_t* cin, uint64_t nblk, hipStream_t s, int mode = 0, uint64_t jlo = 0)
 {
-    if (fa_crt_fp_enabled || mode) {
+    if (fa_crt_fp_enabled || mode || jlo) {
         fa_crt_fp_consts F;
         fa_crt_fp_consts_init(&F, C);
         switch (np) {
-        case 2: launch_crt_fp<2>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode); break;
-        case 3: launch_crt_fp<3>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode); break;
-        case 4: launch_crt_fp<4>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode); break;
-        case 5: launch_crt_fp<5>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode); break;
-        case 6: launch_crt_fp<6>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode); break;
-        case 7: launch_crt_fp<7>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode); break;
-        case 8: launch_crt_fp<8>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode); break;
+        case 2: launch_crt_fp<2>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode, jlo); break;
+        case 3: launch_crt_fp<3>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode, jlo); break;
+        case 4: launch_crt_fp<4>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode, jlo); break;
+        case 5: launch_crt_fp<5>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode, jlo); break;
+        case 6: launch_crt_fp<6>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode, jlo); break;
+        case 7: launch_crt_fp<7>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode, jlo); break;
+        case 8: launch_crt_fp<8>(Q, stride, F, zlen, bits, z, zn, ngroups, flags, agg, nblk, s, mode, jlo); break;
         default: return fa_set_error(FA_E_ARG, "crt: np %d", np);
         }
         FA_LAUNCH("fa_k_carry_fused", fa_k_carry_fused, dim3(carry_grid(nblk)), dim3(1024), 0, s,
@@ -1441,7 +1444,7 @@ static int mpn_mul_device(uint64_t* z, const uint64_t* a, uint64_t an, const uin
 // CRT of zlen coefficients of `bits`-bit chunks into z[0, zn) (the low zn
 // limbs of the sum; zeros above it), carry flags in the thread's scratch
 int fa_crt_mpn_export(int np, const double* A, uint64_t stride, uint64_t zlen, int bits,
-                      uint64_t* z, uint64_t zn, hipStream_t stream, int mode)
+                      uint64_t* z, uint64_t zn, hipStream_t stream, int mode, uint64_t jlo)
 {
     fa_thread_state* st = fa_state();
     if (!st) return FA_E_NODEV;
@@ -1459,7 +1462,7 @@ int fa_crt_mpn_export(int np, const double* A, uint64_t stride, uint64_t zlen, i
     fa_crt_consts C;
     fa_crt_consts_init(&C, np, 0);
     rc = crt16(np, A, stride, C, zlen, bits, z, zn, ngroups, flags, agg, cin, nblk16, stream,
-               mode);
+               mode, jlo);
     if (rc) return rc;
     rc = fa_buffers_leave(st, stream);
     if (rc) return rc;
@@ -1750,6 +1753,49 @@ int fa_signed_diff_launch(uint64_t* z, const uint64_t* w, uint64_t zn, int32_t* 
 {
     FA_LAUNCH("fa_k_signed_diff", fa_k_signed_diff, dim3(1), dim3(1024), 0, s, z, w, zn, sign);
     return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "signed diff launch");
+}
+
+// z[0, zn) = limbs [lo, lo + zn) of the magnitude t (one workgroup), plus
+// one when `up` is set, the value is negative (*sign) and a limb of t below
+// lo is nonzero: the wide-plan truncated signed export negates only the
+// window's limbs of the two's-complement sum, ~W + 1, which is the ceiling
+// of |value| / 2^(64 lo) (export_signed.c:592-604)
+__global__ void __launch_bounds__(1024) fa_k_signed_window(uint64_t* __restrict__ z, uint64_t zn,
+                                                           const uint64_t* __restrict__ t,
+                                                           uint64_t lo, int up,
+                                                           const int32_t* __restrict__ sign)
+{
+    __shared__ uint8_t sg[1024], sp[1024];
+    __shared__ int snz;
+    const int k = threadIdx.x;
+    if (k == 0) snz = 0;
+    __syncthreads();
+    int nz = 0;
+    if (up && *sign)
+        for (uint64_t i = k; i < lo; i += 1024) nz |= t[i] != 0;
+    if (nz) snz = 1;
+    __syncthreads();
+    const uint8_t c0 = (uint8_t)snz;
+    const uint64_t L = (zn + 1023) / 1024;
+    const uint64_t i0 = (uint64_t)k * L < zn ? (uint64_t)k * L : zn;
+    const uint64_t i1 = i0 + L < zn ? i0 + L : zn;
+    uint8_t P = 1;
+    for (uint64_t i = i0; i < i1; i++) P = (uint8_t)(P & (t[lo + i] == ~UINT64_C(0)));
+    uint8_t c = fa_wg_chain_in(0, P, c0, sg, sp);
+    for (uint64_t i = i0; i < i1; i++) {
+        const uint64_t v = t[lo + i] + c;
+        c = (uint8_t)(c && v == 0);
+        z[i] = v;
+    }
+}
+
+int fa_signed_window_launch(uint64_t* z, uint64_t zn, const uint64_t* t, uint64_t lo, int up,
+                            const int32_t* sign, hipStream_t s)
+{
+    FA_LAUNCH("fa_k_signed_window", fa_k_signed_window, dim3(1), dim3(1024), 0, s, z, zn, t, lo,
+              up, sign);
+    return hipGetLastError() == hipSuccess ? FA_OK
+                                           : fa_set_error(FA_E_HIP, "signed window launch");
 }
 
 // z[0, n] = x y mod 2^(64 n) + 1 on device operands of n + 1 limbs with

@@ -27,9 +27,11 @@ int fa_forward_fast(int depth, const fa_operand& oa, int np, double* A, uint64_t
 int fa_inverse_fast(int depth, int np, double* A, uint64_t stride, const fa_prime_set& S,
                     hipStream_t s);
 int fa_crt_mpn_export(int np, const double* A, uint64_t stride, uint64_t zlen, int bits,
-                      uint64_t* z, uint64_t zn, hipStream_t stream, int mode);
+                      uint64_t* z, uint64_t zn, hipStream_t stream, int mode, uint64_t jlo = 0);
 int fa_signed_diff_launch(uint64_t* z, const uint64_t* w, uint64_t zn, int32_t* sign,
                           hipStream_t s);
+int fa_signed_window_launch(uint64_t* z, uint64_t zn, const uint64_t* t, uint64_t lo, int up,
+                            const int32_t* sign, hipStream_t s);
 int fa_crt_nmod_launch(const double* A, uint64_t stride, const fa_crt_consts& C, uint64_t zl,
                        uint64_t cnt, uint64_t n, uint64_t* dz, hipStream_t s);
 int fa_nmod_primes_bound(const fa_tables* T, uint64_t len_bound, uint64_t prod_bits, uint64_t n,
@@ -165,13 +167,12 @@ __global__ void __launch_bounds__(256) fa_k_op_point(int op, double* __restrict_
 // plans
 // ------------------------------------------------------------------------
 
-// fft_small_plan_init_mpn (plan.c:443-459): products of naturals of at most
-// an_max x bn_max limbs, with up to len_bound such products accumulated in
-// transform space.  Among the exact (np, bits) -- len_bound * min(alen, blen)
-// * 2^(2 bits) <= prod p_i, chunks a multiple of 8 bits -- the plan with the
-// fewest transform points np * 2^depth, then the fewest primes.
-extern "C" int fa_plan_init_mpn(fa_plan_struct* P, uint64_t an_max, uint64_t bn_max,
-                                uint64_t len_bound, int is_signed)
+// the mpn plan over prime counts np_lo .. np_hi: among the exact (np, bits)
+// -- len_bound * min(alen, blen) * 2^(2 bits) <= prod p_i, chunks a
+// multiple of 8 bits -- the one with the fewest transform points np *
+// 2^depth, then the fewest primes
+static int plan_mpn(fa_plan_struct* P, uint64_t an_max, uint64_t bn_max, uint64_t len_bound,
+                    int is_signed, int np_lo, int np_hi)
 {
     if (!P || !an_max || !bn_max || !len_bound)
         return fa_set_error(FA_E_ARG, "fa_plan_init_mpn: bad arguments");
@@ -184,7 +185,7 @@ extern "C" int fa_plan_init_mpn(fa_plan_struct* P, uint64_t an_max, uint64_t bn_
     }
     memset(P, 0, sizeof(*P));
     uint64_t best_cost = UINT64_MAX;
-    for (int np = 2; np <= FA_NPRIMES; np++) {
+    for (int np = np_lo; np <= np_hi; np++) {
         fa_crt_consts C;
         fa_crt_consts_init(&C, np, 0);
         for (int bits = 200; bits >= 8; bits -= 8) {
@@ -208,9 +209,10 @@ extern "C" int fa_plan_init_mpn(fa_plan_struct* P, uint64_t an_max, uint64_t bn_
         }
     }
     if (!P->np)
-        return fa_set_error(FA_E_RANGE, "no plan for %llu x %llu limbs, %llu accumulated",
-                            (unsigned long long)an_max, (unsigned long long)bn_max,
-                            (unsigned long long)len_bound);
+        return fa_set_error(FA_E_RANGE, "no plan for %llu x %llu limbs, %llu accumulated, "
+                            "primes %d..%d", (unsigned long long)an_max,
+                            (unsigned long long)bn_max, (unsigned long long)len_bound, np_lo,
+                            np_hi);
     P->kind = 0;
     P->offset = 0;
     P->stride = fa_res_stride(P->depth);
@@ -221,6 +223,26 @@ extern "C" int fa_plan_init_mpn(fa_plan_struct* P, uint64_t an_max, uint64_t bn_
     P->len_bound = len_bound;
     P->pad = is_signed ? 1 : 0;         // signed: exports centre the coefficients
     return FA_OK;
+}
+
+// fft_small_plan_init_mpn (plan.c:443-459): products of naturals of at most
+// an_max x bn_max limbs, with up to len_bound such products accumulated in
+// transform space
+extern "C" int fa_plan_init_mpn(fa_plan_struct* P, uint64_t an_max, uint64_t bn_max,
+                                uint64_t len_bound, int is_signed)
+{
+    return plan_mpn(P, an_max, bn_max, len_bound, is_signed, 2, FA_NPRIMES);
+}
+
+// fft_small_plan_init_mpn_np (plan.c:482-495): np = 0 the automatic choice,
+// np = 2 the two-prime plan (failing when it is not exact), any other value
+// a plan of three or more primes
+extern "C" int fa_plan_init_mpn_np(fa_plan_struct* P, uint64_t an_max, uint64_t bn_max,
+                                   uint64_t len_bound, int is_signed, uint64_t np)
+{
+    if (np == 0) return plan_mpn(P, an_max, bn_max, len_bound, is_signed, 2, FA_NPRIMES);
+    if (np == 2) return plan_mpn(P, an_max, bn_max, len_bound, is_signed, 2, 2);
+    return plan_mpn(P, an_max, bn_max, len_bound, is_signed, 3, FA_NPRIMES);
 }
 
 // fft_small_plan_init_nmod (plan.c:400-417): the window [zl, zh) of a
@@ -463,37 +485,66 @@ extern "C" int fa_export_mpn(uint64_t* z, int64_t zn, const fa_op_struct* X,
                              stream_of(stream), 0);
 }
 
-// fft_small_export_mpn_signed (export_signed.c:367-606): the first nslots
-// coefficients, each centred to (-M/2, M/2], summed at their bit offsets:
-// z[0, zn) = |value| (zn must leave a spare top bit, 64 zn >= nslots bits
-// + 64 (limbs of M) + 2 as in the reference), *sign (device int32) = 1 when
-// the value is negative.  The positive and the negated negative parts are
-// two CRT passes of the product path; one workgroup subtracts them.
-extern "C" int fa_export_mpn_signed(uint64_t* z, int64_t zn, int32_t* sign,
-                                    const fa_op_struct* X, uint64_t nslots,
-                                    const fa_plan_struct* P, void* stream)
+// fft_small_export_mpn_signed_trunc (export_signed.c:367-606): the first
+// nslots coefficients, each centred to (-M/2, M/2], summed at their bit
+// offsets; z[0, zn) = limbs [lo, lo + zn) of |value|, *sign (device int32)
+// = 1 when the value is negative; 64 (lo + zn) >= nslots bits + 64 n + 2
+// with n the limbs of M np, as the reference asserts.  lo > 0 keeps the
+// reference's truncation: only slots from j0 = ceil((64 lo - 64 n - 1) /
+// bits) on are summed (rounded down to a multiple of 256 on plans of more
+// than two primes), and on those plans a negative value's window is the
+// ceiling of |value| / 2^(64 lo) (the window of ~W + 1), on two-prime plans
+// its floor.  The positive and the negated negative parts are two CRT
+// passes of the product path; one workgroup subtracts them, another cuts
+// the window.
+extern "C" int fa_export_mpn_signed_trunc(uint64_t* z, int64_t zn, int32_t* sign,
+                                          const fa_op_struct* X, uint64_t nslots, uint64_t lo,
+                                          const fa_plan_struct* P, void* stream)
 {
     if (!compatible(X, P) || P->kind != 0 || X->domain != FA_OP_COEFF || !z || zn <= 0 ||
         !sign || nslots > P->zn)
         return fa_set_error(FA_E_ARG, "fa_export_mpn_signed: bad arguments");
     if (!P->pad)
         return fa_set_error(FA_E_ARG, "fa_export_mpn_signed: the plan is not signed");
+    const uint64_t n = (uint64_t)fa_ref_coeff_len(P->np);
+    if (64 * (lo + (uint64_t)zn) < nslots * (uint64_t)P->bits + 64 * n + 2)
+        return fa_set_error(FA_E_ARG, "fa_export_mpn_signed: %lld limbs from limb %llu cannot "
+                            "hold %llu slots of %d bits", (long long)zn, (unsigned long long)lo,
+                            (unsigned long long)nslots, P->bits);
+    uint64_t j0 = 0;
+    if (64 * lo > 64 * n + 1) j0 = cdiv(64 * lo - 64 * n - 1, (uint64_t)P->bits);
+    if (j0 > nslots) j0 = nslots;
+    if (P->np > 2) j0 &= ~UINT64_C(255);
     fa_thread_state* st = fa_state();
     if (!st) return FA_E_NODEV;
     hipStream_t s = stream_of(stream);
-    // the negative part goes to the thread's block buffer, kept apart from
-    // the CRT's scratch
+    const uint64_t L = lo + (uint64_t)zn;
+    // the negative part (and, truncated, the whole magnitude) goes to the
+    // thread's block buffer, kept apart from the CRT's scratch
     void* tb;
-    int rc = fa_tbuf_reserve(st, (size_t)zn * 8, &tb, s);
+    int rc = fa_tbuf_reserve(st, (size_t)(lo ? 2 * L : L) * 8, &tb, s);
     if (rc) return rc;
     uint64_t* w = (uint64_t*)tb;
-    rc = fa_crt_mpn_export(P->np, X->data, X->stride, nslots, P->bits, z, (uint64_t)zn, s, 1);
+    uint64_t* t = lo ? w + L : z;
+    rc = fa_crt_mpn_export(P->np, X->data, X->stride, nslots, P->bits, t, L, s, 1, j0);
     if (rc) return rc;
-    rc = fa_crt_mpn_export(P->np, X->data, X->stride, nslots, P->bits, w, (uint64_t)zn, s, 2);
+    rc = fa_crt_mpn_export(P->np, X->data, X->stride, nslots, P->bits, w, L, s, 2, j0);
     if (rc) return rc;
-    rc = fa_signed_diff_launch(z, w, (uint64_t)zn, sign, s);
+    rc = fa_signed_diff_launch(t, w, L, sign, s);
     if (rc) return rc;
+    if (lo) {
+        rc = fa_signed_window_launch(z, (uint64_t)zn, t, lo, P->np > 2, sign, s);
+        if (rc) return rc;
+    }
     return fa_buffers_leave(st, s);
+}
+
+// fft_small_export_mpn_signed (export_signed.c:608-612): the same with lo = 0
+extern "C" int fa_export_mpn_signed(uint64_t* z, int64_t zn, int32_t* sign,
+                                    const fa_op_struct* X, uint64_t nslots,
+                                    const fa_plan_struct* P, void* stream)
+{
+    return fa_export_mpn_signed_trunc(z, zn, sign, X, nslots, 0, P, stream);
 }
 
 // fft_small_export_nmod_range (nmod_poly_mul.c:860-901): coefficients

@@ -67,6 +67,12 @@ static int coeff_len(int np, uint64_t* M)
     return n;
 }
 
+int fa_ref_coeff_len(int np)
+{
+    uint64_t M[FA_CRT_MAXL + 2];
+    return coeff_len(np, M);
+}
+
 // crt_data_find_bn_bound (mpn_mul.c:152-184): the largest bn with
 // 64 bn <= bits M / 2^(2 bits) - (bits - 1), all in coeff_len limbs
 static uint64_t ref_bn_bound(int np, uint64_t bits)

@@ -298,6 +298,11 @@ typedef struct {
    records it) */
 FA_API int fa_plan_init_mpn(fa_plan_struct* P, uint64_t an_max, uint64_t bn_max,
                             uint64_t len_bound, int is_signed);
+/* fft_small_plan_init_mpn_np, plan.c:482-495: np = 0 the automatic choice,
+   np = 2 forces two primes (fails when no two-prime plan is exact), any
+   other value a plan of three or more primes */
+FA_API int fa_plan_init_mpn_np(fa_plan_struct* P, uint64_t an_max, uint64_t bn_max,
+                               uint64_t len_bound, int is_signed, uint64_t np);
 /* fft_small_plan_init_nmod, plan.c:400: len_bound products of prod_bits-bit
    values per coefficient (2 * bits of n for a plain product) */
 FA_API int fa_plan_init_nmod(fa_plan_struct* P, uint64_t zl, uint64_t zh, uint64_t zn,
@@ -339,12 +344,22 @@ FA_API int fa_ifft(fa_op_struct* X, const fa_plan_struct* P, void* stream);
    integer into device z */
 FA_API int fa_export_mpn(uint64_t* z, int64_t zn, const fa_op_struct* X,
                          const fa_plan_struct* P, void* stream);
-/* fft_small_export_mpn_signed (export_signed.c:367-606), signed plans:
+/* fft_small_export_mpn_signed (export_signed.c:608-612), signed plans:
    the first nslots coefficients centred to (-M/2, M/2] and summed at their
-   bit offsets; z = zn limbs of |value| (64 zn >= nslots bits + 64 (limbs of
-   M) + 2), *sign (a device int32) = 1 when the value is negative */
+   bit offsets; z = zn limbs of |value| (64 zn >= nslots bits + 64 n + 2, n
+   the limbs of M np), *sign (a device int32) = 1 when the value is
+   negative */
 FA_API int fa_export_mpn_signed(uint64_t* z, int64_t zn, int32_t* sign, const fa_op_struct* X,
                                 uint64_t nslots, const fa_plan_struct* P, void* stream);
+/* fft_small_export_mpn_signed_trunc (export_signed.c:367-606): limbs
+   [lo, lo + zn) of |value| with the reference's truncation (64 (lo + zn) >=
+   nslots bits + 64 n + 2): slots below the first that can reach limb lo
+   are dropped (that slot index rounded down to a multiple of 256 on plans
+   of more than two primes), and on those plans a negative value's window
+   is rounded up, on two-prime plans down, as the reference's words are */
+FA_API int fa_export_mpn_signed_trunc(uint64_t* z, int64_t zn, int32_t* sign,
+                                      const fa_op_struct* X, uint64_t nslots, uint64_t lo,
+                                      const fa_plan_struct* P, void* stream);
 /* fft_small_export_nmod / _range (nmod_poly_mul.c:800-901): coefficients
    of the plan's window, or of [zl, zh), mod n into device z */
 FA_API int fa_export_nmod(uint64_t* z, const fa_op_struct* X, fa_nmod_t mod,

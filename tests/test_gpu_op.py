@@ -216,3 +216,116 @@ def test_signed_export(an, bn):
     U.ifft()
     with pytest.raises(fa.FlintAmdError):
         U.export_mpn_signed(dz.data_ptr(), zn, ds.data_ptr(), Pu.s.zn)
+
+
+def _slot_values(a, b, bits):
+    """the chunk convolution c_j = sum_{i+k=j} A_i B_k (A_i = bits
+    [i bits, (i+1) bits) of a, bits a multiple of 8) by Kronecker
+    substitution on Python integers"""
+    cb = bits // 8
+    alen, blen = -(-64 * len(a) // bits), -(-64 * len(b) // bits)
+    S = (2 * bits + min(alen, blen).bit_length() + 8) // 8      # bytes per slot
+
+    def spread(x, n):
+        raw = np.frombuffer(np.asarray(x, dtype=np.uint64).tobytes(), dtype=np.uint8)
+        raw = np.concatenate([raw, np.zeros(n * cb - len(raw), dtype=np.uint8)])
+        out = np.zeros((n, S), dtype=np.uint8)
+        out[:, :cb] = raw.reshape(n, cb)
+        return int.from_bytes(out.tobytes(), "little")
+
+    prod = spread(a, alen) * spread(b, blen)
+    zlen = alen + blen - 1
+    raw = prod.to_bytes(zlen * S + S, "little")
+    return [int.from_bytes(raw[j * S:(j + 1) * S], "little") for j in range(zlen)]
+
+
+def _slot_sum(c, bits, j0):
+    """sum_{j < j0} c_j 2^(j bits) for 0 <= c_j < 2^(3 bits), as three
+    concatenations of bits-wide fields"""
+    cb, mask = bits // 8, (1 << bits) - 1
+    total = 0
+    for t in range(3):
+        field = b"".join(((v >> (t * bits)) & mask).to_bytes(cb, "little") for v in c[:j0])
+        total += int.from_bytes(field, "little") << (t * bits)
+    return total
+
+
+def _signed_trunc_expected(A, B, bits, np_, nslots, lo, zn, slots=None):
+    """export_signed.c:367-606 on Python integers: V' = the sum of the slots
+    j >= j0 of a0 b0 - a1 b1 (j0 = the first slot that can reach limb lo,
+    rounded down to a multiple of 256 on plans of more than two primes);
+    z = limbs [lo, lo + zn) of |V'|, rounded up for negative V' on those
+    plans (the window of ~W + 1), down on two-prime plans"""
+    from tests.test_oracle_window import _coeff_len
+    n = _coeff_len(np_)
+    j0 = 0
+    if 64 * lo > 64 * n + 1:
+        j0 = -(-(64 * lo - 64 * n - 1) // bits)
+    j0 = min(j0, nslots)
+    if np_ > 2:
+        j0 = j0 // 256 * 256
+    if slots is None:
+        slots = [_slot_values(A[k], B[k], bits) for k in range(2)]
+    V = to_int(A[0]) * to_int(B[0]) - to_int(A[1]) * to_int(B[1])
+    V -= _slot_sum(slots[0], bits, j0) - _slot_sum(slots[1], bits, j0)
+    T = 1 << (64 * lo)
+    mag = abs(V)
+    if V < 0 and np_ > 2:
+        z = -(-mag // T)
+    else:
+        z = mag // T
+    return z % (1 << (64 * zn)), int(V < 0), V
+
+
+@pytest.mark.parametrize("np_", [2, 0])
+@pytest.mark.parametrize("an,bn", [(30, 20), (3000, 2500), (9000, 7000)])
+def test_signed_export_trunc(an, bn, np_):
+    """fft_small_export_mpn_signed_trunc: limbs [lo, lo + zn) of the signed
+    sum a0 b0 - a1 b1 with the reference's slot cut and rounding, on a
+    two-prime plan and an automatic (wider) one, both signs, against the
+    rule restated on Python integers; lo = 0 is the exact export"""
+    import torch
+    import flint_amd as fa
+    from tests.test_oracle_window import _coeff_len
+    rng = np.random.default_rng(an + np_)
+    P = fa.Plan.mpn(an, bn, len_bound=2, signed=True, np=np_)
+    assert (P.s.np == 2) == (np_ == 2)
+    A = [rand_limbs(rng, an) for _ in range(2)]
+    B = [rand_limbs(rng, bn) for _ in range(2)]
+    dA = [_dev(torch, A[k]) for k in range(2)]
+    dB = [_dev(torch, B[k]) for k in range(2)]
+    ops = []
+    for k in range(2):
+        oa, ob = fa.Op(P), fa.Op(P)
+        oa.fft_mpn(dA[k].data_ptr(), an)
+        ob.fft_mpn(dB[k].data_ptr(), bn)
+        ops.append((oa, ob))
+    nslots, bits = P.s.zn, P.s.bits
+    need = -(-(nslots * bits + 64 * _coeff_len(P.s.np) + 2) // 64)
+    ds = torch.full((1,), 7, dtype=torch.int32, device="cuda:0")
+    for first, second in ((0, 1), (1, 0)):
+        Z = fa.Op(P)
+        Z.mul(ops[first][0], ops[first][1])
+        Z.submul(ops[second][0], ops[second][1])
+        Z.ifft()
+        Ao, Bo = [A[first], A[second]], [B[first], B[second]]
+        slots = [_slot_values(Ao[k], Bo[k], bits) for k in range(2)]
+        for lo in sorted({0, 1, 3, need // 3, need // 2, need - 6, need - 2}):
+            zn = need - lo + 1
+            print(an, bn, np_, first, lo, zn, flush=True)
+            dz = torch.zeros(zn, dtype=torch.int64, device="cuda:0")
+            torch.cuda.synchronize()
+            Z.export_mpn_signed_trunc(dz.data_ptr(), zn, ds.data_ptr(), nslots, lo)
+            torch.cuda.synchronize()
+            got = to_int(dz.cpu().numpy().view(np.uint64))
+            sign = int(ds.cpu()[0])
+            want, wsign, V = _signed_trunc_expected(Ao, Bo, bits, P.s.np, nslots, lo, zn, slots)
+            assert (got, sign) == (want, wsign), (first, lo)
+            # within two units of limb lo of the exact signed value
+            exact = abs(V) >> (64 * lo)
+            assert abs(got - exact) <= 2
+            if lo == 0:
+                assert (-got if sign else got) == V
+    # too few limbs for the value is refused
+    with pytest.raises(fa.FlintAmdError):
+        Z.export_mpn_signed_trunc(dz.data_ptr(), 1, ds.data_ptr(), nslots, 0)
