@@ -176,6 +176,8 @@ SIGNATURES = {
     "fa_set_square_route": (None, [ctypes.c_int]),
     "fa_plan_init_mpn": (ctypes.c_int, [_VP, _u64, _u64, _u64, ctypes.c_int]),
     "fa_plan_init_mpn_np": (ctypes.c_int, [_VP, _u64, _u64, _u64, ctypes.c_int, _u64]),
+    "fa_plan_init_nmod_cyclic": (ctypes.c_int, [_VP, _u64, _u64, _u64, _u64, NmodT]),
+    "fa_plan_init_mpn_cyclic": (ctypes.c_int, [_VP, _P64, _u64]),
     "fa_plan_init_nmod": (ctypes.c_int, [_VP, _u64, _u64, _u64, _u64, _u64, _u64, NmodT]),
     "fa_plan_clear": (None, [_VP]),
     "fa_op_init": (ctypes.c_int, [_VP, _VP]),
@@ -523,6 +525,22 @@ class Plan:
         else:
             _check(lib().fa_plan_init_mpn_np(P.ptr, an_max, bn_max, len_bound,
                                              1 if signed else 0, np))
+        return P
+
+    @classmethod
+    def mpn_cyclic(cls, nn, len_bound=1):
+        """products mod 2^(64 nn) - 1; returns (plan, the rounded-up nn)"""
+        P = cls()
+        v = ctypes.c_uint64(nn)
+        _check(lib().fa_plan_init_mpn_cyclic(P.ptr, ctypes.byref(v), len_bound))
+        return P, int(v.value)
+
+    @classmethod
+    def nmod_cyclic(cls, depth, zh, len_bound, prod_bits, n):
+        """convolutions mod x^(2^depth) - 1 over Z/nZ, coefficients [0, zh)"""
+        P = cls()
+        _check(lib().fa_plan_init_nmod_cyclic(P.ptr, depth, zh, len_bound, prod_bits,
+                                              nmod_init(n)))
         return P
 
     @classmethod

@@ -279,6 +279,73 @@ extern "C" int fa_plan_init_nmod(fa_plan_struct* P, uint64_t zl, uint64_t zh, ui
     return FA_OK;
 }
 
+// fft_small_plan_init_nmod_cyclic (plan.c:425-441): convolutions modulo
+// x^N - 1, N = 2^depth, output coefficients [0, zh); operands of up to N
+// coefficients (a full cyclic convolution accumulates N products, so a
+// plain product passes len_bound = N)
+extern "C" int fa_plan_init_nmod_cyclic(fa_plan_struct* P, uint64_t depth, uint64_t zh,
+                                        uint64_t len_bound, uint64_t prod_bits, fa_nmod_t mod)
+{
+    if (!P || depth < 4 || depth > FA_MAX_DEPTH || zh > (UINT64_C(1) << depth))
+        return fa_set_error(FA_E_ARG, "fa_plan_init_nmod_cyclic: bad arguments");
+    const uint64_t N = UINT64_C(1) << depth;
+    return fa_plan_init_nmod(P, 0, zh, N, N, len_bound, prod_bits, mod);
+}
+
+// fft_small_plan_init_mpn_cyclic (plan.c:497-647): products of naturals of
+// at most *nn limbs modulo 2^(64 *nn) - 1.  The chunks tile the operand
+// exactly, bits 2^depth = 64 nn, so the coefficient convolution's wrap mod
+// x^(2^depth) - 1 is the integer wrap mod B^nn - 1; *nn is rounded up to
+// the smallest such length whose plan is exact for len_bound 2^depth chunk
+// products (bits a multiple of 8 here), ties to fewer primes, and written
+// back.  fa_export_mpn then gives a representative below len_bound
+// 2^(64 nn + 2 bits + depth), which the caller folds.
+extern "C" int fa_plan_init_mpn_cyclic(fa_plan_struct* P, uint64_t* nn, uint64_t len_bound)
+{
+    if (!P || !nn || !*nn || !len_bound)
+        return fa_set_error(FA_E_ARG, "fa_plan_init_mpn_cyclic: bad arguments");
+    uint64_t best_nn = 0;
+    int best_np = 0, best_bits = 0, best_depth = 0;
+    for (int np = 2; np <= FA_NPRIMES; np++) {
+        fa_crt_consts C;
+        fa_crt_consts_init(&C, np, 0);
+        for (int bits = 8; bits <= 200; bits += 8) {
+            if (9 * bits < 64 * C.nl + 64) continue;   // the CRT's limb layout
+            int depth = 4;
+            while (depth <= FA_MAX_DEPTH && ((uint64_t)bits << depth) < 64 * *nn) depth++;
+            if (depth > FA_MAX_DEPTH) continue;
+            const uint64_t N = UINT64_C(1) << depth;
+            if (len_bound > UINT64_MAX / N || !capacity(np, len_bound * N, 2 * (uint64_t)bits))
+                continue;
+            const uint64_t nn1 = ((uint64_t)bits << depth) / 64;
+            if (!best_np || nn1 < best_nn) {
+                best_nn = nn1;
+                best_np = np;
+                best_bits = bits;
+                best_depth = depth;
+            }
+        }
+    }
+    if (!best_np)
+        return fa_set_error(FA_E_RANGE, "no cyclic plan for %llu limbs, %llu accumulated",
+                            (unsigned long long)*nn, (unsigned long long)len_bound);
+    memset(P, 0, sizeof(*P));
+    P->kind = 0;
+    P->np = best_np;
+    P->offset = 0;
+    P->bits = best_bits;
+    P->depth = best_depth;
+    P->stride = fa_res_stride(best_depth);
+    P->zn = UINT64_C(1) << best_depth;
+    P->zl = 0;
+    P->zh = P->zn;
+    P->an_max = best_nn;
+    P->bn_max = best_nn;
+    P->len_bound = len_bound;
+    *nn = best_nn;
+    return FA_OK;
+}
+
 extern "C" void fa_plan_clear(fa_plan_struct* P)
 {
     if (P) memset(P, 0, sizeof(*P));

@@ -308,6 +308,16 @@ FA_API int fa_plan_init_mpn_np(fa_plan_struct* P, uint64_t an_max, uint64_t bn_m
 FA_API int fa_plan_init_nmod(fa_plan_struct* P, uint64_t zl, uint64_t zh, uint64_t zn,
                              uint64_t xtrunc_max, uint64_t len_bound, uint64_t prod_bits,
                              fa_nmod_t mod);
+/* fft_small_plan_init_nmod_cyclic, plan.c:425-441: convolutions mod
+   x^N - 1, N = 2^depth (depth >= 4), output coefficients [0, zh) */
+FA_API int fa_plan_init_nmod_cyclic(fa_plan_struct* P, uint64_t depth, uint64_t zh,
+                                    uint64_t len_bound, uint64_t prod_bits, fa_nmod_t mod);
+/* fft_small_plan_init_mpn_cyclic, plan.c:497-647: products of naturals of
+   at most *nn limbs mod 2^(64 *nn) - 1, *nn rounded up to a length with
+   bits 2^depth = 64 nn and written back; fa_export_mpn gives a
+   representative the caller folds (zn >= nn + (2 bits + depth +
+   bits(len_bound)) / 64 + 1 limbs hold it) */
+FA_API int fa_plan_init_mpn_cyclic(fa_plan_struct* P, uint64_t* nn, uint64_t len_bound);
 FA_API void fa_plan_clear(fa_plan_struct* P);
 /* fft_small_op_init / _init_borrowed / _sizeof_data / _clear, op.c:18-80
    (init allocates device memory; borrowed uses the caller's) */

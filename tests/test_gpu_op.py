@@ -329,3 +329,92 @@ def test_signed_export_trunc(an, bn, np_):
     # too few limbs for the value is refused
     with pytest.raises(fa.FlintAmdError):
         Z.export_mpn_signed_trunc(dz.data_ptr(), 1, ds.data_ptr(), nslots, 0)
+
+
+@pytest.mark.parametrize("nn_req,K", [(100, 1), (1500, 3), (123, 2), (7999, 1), (50000, 2)])
+def test_mpn_cyclic_plan(nn_req, K):
+    """fft_small_plan_init_mpn_cyclic as t-mpn_cyclic.c checks it: the
+    rounded-up length tiles exactly (bits 2^depth = 64 nn), and a sum of K
+    products of operands of up to nn limbs, exported and folded, is the sum
+    mod B^nn - 1; squares, an operand of all ones (B^nn - 1 = 0) and short
+    operands included"""
+    import torch
+    import flint_amd as fa
+    rng = np.random.default_rng(nn_req)
+    P, nn = fa.Plan.mpn_cyclic(nn_req, K)
+    bits, depth = P.s.bits, P.s.depth
+    assert nn >= nn_req and bits << depth == 64 * nn
+    zn = nn + (2 * bits + depth + K.bit_length()) // 64 + 1
+    mod = (1 << (64 * nn)) - 1
+    for case in ("full", "short", "square", "ones"):
+        A, B, want = [], [], 0
+        for k in range(K):
+            an = nn if case != "short" else int(rng.integers(1, nn + 1))
+            bn = nn if case != "short" else int(rng.integers(1, nn + 1))
+            a = rand_limbs(rng, an)
+            if case == "ones" and k == 0:
+                a[:] = np.uint64(2**64 - 1)
+            b = a.copy() if case == "square" else rand_limbs(rng, bn)
+            A.append(a)
+            B.append(b)
+            want += to_int(a) * to_int(b)
+        print(nn_req, K, nn, case, flush=True)
+        dA = [_dev(torch, x) for x in A]
+        dB = [_dev(torch, x) for x in B]
+        Z, T, oa, ob = fa.Op(P), fa.Op(P), fa.Op(P), fa.Op(P)
+        for k in range(K):
+            oa.fft_mpn(dA[k].data_ptr(), len(A[k]))
+            if case == "square":
+                (Z if k == 0 else T).sqr(oa)
+                if k:
+                    Z.add(Z, T)
+            else:
+                ob.fft_mpn(dB[k].data_ptr(), len(B[k]))
+                if k == 0:
+                    Z.mul(oa, ob)
+                else:
+                    Z.addmul(oa, ob)
+            torch.cuda.synchronize()
+        Z.ifft()
+        dz = torch.zeros(zn, dtype=torch.int64, device="cuda:0")
+        torch.cuda.synchronize()
+        Z.export_mpn(dz.data_ptr(), zn)
+        torch.cuda.synchronize()
+        got = to_int(dz.cpu().numpy().view(np.uint64))
+        assert got % mod == want % mod, case
+    with pytest.raises(fa.FlintAmdError):
+        fa.Plan.mpn_cyclic(nn_req, 2**64 - 1)       # no plan: refused
+
+
+@pytest.mark.parametrize("depth,n", [(4, 65537), (9, (1 << 61) - 1), (13, 2**64 - 59),
+                                     (16, 1000003)])
+def test_nmod_cyclic_plan(depth, n):
+    """fft_small_plan_init_nmod_cyclic: the product of two length-N
+    polynomials mod x^N - 1 over Z/nZ, coefficients [0, zh), against the
+    oracle's linear product folded"""
+    import torch
+    import flint_amd as fa
+    rng = np.random.default_rng(depth)
+    N = 1 << depth
+
+    def draw(m):
+        return np.array([int.from_bytes(rng.bytes(8), "little") % n for _ in range(m)],
+                        dtype=np.uint64)
+    a, b = draw(N), draw(N - 3)
+    for zh in (N, N // 2 + 1):
+        P = fa.Plan.nmod_cyclic(depth, zh, N, 2 * int(n).bit_length(), n)
+        print(depth, n, zh, P.s.np, flush=True)
+        da, db = _dev(torch, a), _dev(torch, b)
+        ta, tb, Z = fa.Op(P), fa.Op(P), fa.Op(P)
+        ta.fft_nmod(da.data_ptr(), N, n)
+        tb.fft_nmod(db.data_ptr(), N - 3, n)
+        Z.mul(ta, tb)
+        Z.ifft()
+        dz = torch.zeros(zh, dtype=torch.int64, device="cuda:0")
+        torch.cuda.synchronize()
+        Z.export_nmod(dz.data_ptr(), n)
+        torch.cuda.synchronize()
+        full = O.nmod_poly_mul_mid(a, b, 0, 2 * N - 4, n)
+        cyc = [(int(full[k]) + (int(full[k + N]) if k + N < len(full) else 0)) % n
+               for k in range(N)]
+        assert [int(x) for x in dz.cpu().numpy().view(np.uint64)] == cyc[:zh]
