@@ -178,6 +178,12 @@ SIGNATURES = {
     "fa_plan_init_mpn_np": (ctypes.c_int, [_VP, _u64, _u64, _u64, ctypes.c_int, _u64]),
     "fa_plan_init_nmod_cyclic": (ctypes.c_int, [_VP, _u64, _u64, _u64, _u64, NmodT]),
     "fa_plan_init_mpn_cyclic": (ctypes.c_int, [_VP, _P64, _u64]),
+    "fa_mulmod_2expp1_ctx_init": (ctypes.c_int, [_VP, _s64, _s64]),
+    "fa_mulmod_2expp1_ctx_clear": (None, [_VP]),
+    "fa_mulmod_2expp1": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "fa_mulmod_2expp1_transformed_size": (_s64, [_VP]),
+    "fa_mulmod_2expp1_transform": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+    "fa_mulmod_2expp1_mul_cached": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
     "fa_plan_init_nmod": (ctypes.c_int, [_VP, _u64, _u64, _u64, _u64, _u64, _u64, NmodT]),
     "fa_plan_clear": (None, [_VP]),
     "fa_op_init": (ctypes.c_int, [_VP, _VP]),
@@ -549,6 +555,33 @@ class Plan:
         _check(lib().fa_plan_init_nmod(P.ptr, zl, zh, zn, xtrunc_max, len_bound, prod_bits,
                                        nmod_init(n)))
         return P
+
+
+class Mulmod2expp1Ctx(ctypes.Structure):
+    """fa_mulmod_2expp1_ctx_struct: products mod 2^N + 1 with cached
+    transforms (sd_fft_mpn_mulmod_2expp1_ctx)."""
+    _fields_ = [("N", ctypes.c_int64), ("n", ctypes.c_int64), ("m", ctypes.c_int64),
+                ("P", PlanStruct)]
+
+    def __init__(self, N, m=0):
+        super().__init__()
+        _check(lib().fa_mulmod_2expp1_ctx_init(ctypes.addressof(self), N, m))
+
+    @property
+    def ptr(self):
+        return ctypes.addressof(self)
+
+    def transformed_size(self):
+        return lib().fa_mulmod_2expp1_transformed_size(self.ptr)
+
+    def transform(self, F_ptr, y_ptr, stream=None):
+        return _check(lib().fa_mulmod_2expp1_transform(self.ptr, F_ptr, y_ptr, stream))
+
+    def mul_cached(self, z_ptr, x_ptr, F_ptr, stream=None):
+        return _check(lib().fa_mulmod_2expp1_mul_cached(self.ptr, z_ptr, x_ptr, F_ptr, stream))
+
+    def mul(self, z_ptr, x_ptr, y_ptr, stream=None):
+        return _check(lib().fa_mulmod_2expp1(self.ptr, z_ptr, x_ptr, y_ptr, stream))
 
 
 class Op:

@@ -1798,6 +1798,13 @@ int fa_signed_window_launch(uint64_t* z, uint64_t zn, const uint64_t* t, uint64_
                                            : fa_set_error(FA_E_HIP, "signed window launch");
 }
 
+// z[0, n] = P mod 2^(64 n) + 1 for the 2n + 2 limbs of P (one workgroup)
+int fa_fold_2expp1_launch(uint64_t* z, const uint64_t* P, uint64_t n, hipStream_t s)
+{
+    FA_LAUNCH("fa_k_fold_2expp1", fa_k_fold_2expp1, dim3(1), dim3(1024), 0, s, z, P, n);
+    return hipGetLastError() == hipSuccess ? FA_OK : fa_set_error(FA_E_HIP, "fold launch failed");
+}
+
 // z[0, n] = x y mod 2^(64 n) + 1 on device operands of n + 1 limbs with
 // values <= 2^(64 n), enqueued on `stream`
 extern "C" int fa_mpn_mulmod_2expp1_device(uint64_t* z, const uint64_t* x, const uint64_t* y,

@@ -377,6 +377,30 @@ FA_API int fa_export_nmod(uint64_t* z, const fa_op_struct* X, fa_nmod_t mod,
 FA_API int fa_export_nmod_range(uint64_t* z, const fa_op_struct* X, uint64_t zl, uint64_t zh,
                                 fa_nmod_t mod, const fa_plan_struct* P, void* stream);
 
+/* sd_fft_mpn_mulmod_2expp1_ctx (negacyclic.c:43-501): products in
+   Z/(2^N + 1), N a multiple of 64, operands and results of n + 1 = N/64 + 1
+   limbs with values in [0, 2^N], device memory.  m = 0 or a power of two
+   >= 16 dividing N, checked as the reference checks it and kept; the
+   device plan (the full product folded) does not depend on it. */
+typedef struct {
+    int64_t N, n, m;
+    fa_plan_struct P;        /* transforms of one (n + 1)-limb operand */
+} fa_mulmod_2expp1_ctx_struct;
+/* sd_fft_mpn_mulmod_2expp1_ctx_init / _clear (negacyclic.c:65-230) */
+FA_API int fa_mulmod_2expp1_ctx_init(fa_mulmod_2expp1_ctx_struct* C, int64_t N, int64_t m);
+FA_API void fa_mulmod_2expp1_ctx_clear(fa_mulmod_2expp1_ctx_struct* C);
+/* sd_fft_mpn_mulmod_2expp1 with a context (negacyclic.c:491-501) */
+FA_API int fa_mulmod_2expp1(const fa_mulmod_2expp1_ctx_struct* C, uint64_t* z,
+                            const uint64_t* x, const uint64_t* y, void* stream);
+/* _transformed_size / _transform / _mul_cached (negacyclic.c:349-489): F
+   (device, transformed_size doubles) caches y's transform for products
+   z = x y mod 2^N + 1 */
+FA_API int64_t fa_mulmod_2expp1_transformed_size(const fa_mulmod_2expp1_ctx_struct* C);
+FA_API int fa_mulmod_2expp1_transform(const fa_mulmod_2expp1_ctx_struct* C, double* F,
+                                      const uint64_t* y, void* stream);
+FA_API int fa_mulmod_2expp1_mul_cached(const fa_mulmod_2expp1_ctx_struct* C, uint64_t* z,
+                                       const uint64_t* x, const double* F, void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* sd_fft_ctx plugin surface, fft_small.h:206-281                         */
 /* ---------------------------------------------------------------------- */

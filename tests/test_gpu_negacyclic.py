@@ -65,3 +65,43 @@ def test_mulmod_2expp1_device_square_and_rejects():
     bad[n] = 2
     with pytest.raises(fa.FlintAmdError):
         fa.mpn_mulmod_2expp1(bad, x, n)
+
+
+@pytest.mark.parametrize("n,m", [(1, 0), (5, 16), (1000, 0), (4096, 64), (33000, 0)])
+def test_mulmod_2expp1_cached_transform(n, m):
+    """the context form (sd_fft_mpn_mulmod_2expp1_ctx_init / _transform /
+    _mul_cached, the precached Schoenhage-Strassen shape): y's transform
+    cached once and reused for several x, edge values included, and the
+    one-shot product with the context, against Python integers"""
+    import torch
+    import flint_amd as fa
+    rng = np.random.default_rng(n)
+    N = 64 * n
+    C = fa.Mulmod2expp1Ctx(N, m)
+    assert (C.N, C.n, C.m) == (N, n, m)
+    size = C.transformed_size()
+    assert size > 0
+    dev = torch.device("cuda:0")
+    s = torch.cuda.Stream(device=dev)
+    for v in (_norm_rand(rng, n), 1 << N, (1 << N) - 1):
+        F = torch.empty(size, dtype=torch.float64, device=dev)
+        dy = torch.from_numpy(from_int(v, n + 1).view(np.int64)).to(dev)
+        torch.cuda.synchronize()
+        C.transform(F.data_ptr(), dy.data_ptr(), s.cuda_stream)
+        xs = [_norm_rand(rng, n) for _ in range(3)] + [0, 1, 1 << N]
+        for u in xs:
+            print(n, m, u.bit_length(), v.bit_length(), flush=True)
+            dx = torch.from_numpy(from_int(u, n + 1).view(np.int64)).to(dev)
+            dz = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+            torch.cuda.synchronize()
+            C.mul_cached(dz.data_ptr(), dx.data_ptr(), F.data_ptr(), s.cuda_stream)
+            s.synchronize()
+            assert to_int(dz.cpu().numpy().view(np.uint64)) == _mod_fermat(u * v, N)
+        dz = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+        torch.cuda.synchronize()
+        C.mul(dz.data_ptr(), dx.data_ptr(), dy.data_ptr(), s.cuda_stream)
+        s.synchronize()
+        assert to_int(dz.cpu().numpy().view(np.uint64)) == _mod_fermat(xs[-1] * v, N)
+    for badN, badm in ((100, 0), (64 * 5, 8), (64 * 5, 48), (64, 128)):
+        with pytest.raises(fa.FlintAmdError):
+            fa.Mulmod2expp1Ctx(badN, badm)
