@@ -178,6 +178,10 @@ SIGNATURES = {
     "fa_plan_init_mpn_np": (ctypes.c_int, [_VP, _u64, _u64, _u64, ctypes.c_int, _u64]),
     "fa_plan_init_nmod_cyclic": (ctypes.c_int, [_VP, _u64, _u64, _u64, _u64, NmodT]),
     "fa_plan_init_mpn_cyclic": (ctypes.c_int, [_VP, _P64, _u64]),
+    "fa_nmod_poly_mul_precomp_init": (ctypes.c_int, [ctypes.POINTER(_VP), _P64, _s64, _s64,
+                                                     ctypes.c_int, NmodT]),
+    "fa_nmod_poly_mul_mid_precomp": (ctypes.c_int, [_P64, _u64, _u64, _P64, _s64, _VP, NmodT]),
+    "fa_nmod_poly_mul_precomp_clear": (None, [_VP]),
     "fa_mulmod_2expp1_ctx_init": (ctypes.c_int, [_VP, _s64, _s64]),
     "fa_mulmod_2expp1_ctx_clear": (None, [_VP]),
     "fa_mulmod_2expp1": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
@@ -736,6 +740,37 @@ class DivremPrecomp:
     def close(self):
         if self.h:
             lib().fa_nmod_poly_divrem_precomp_clear(self.h)
+            self.h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MulPrecomp:
+    """mul_precomp_struct: b transformed once on a cyclic plan of 2^depth
+    points, for middle products a b on windows the transform holds"""
+
+    def __init__(self, b, depth, n, btrunc=0):
+        b = _arr(b)
+        self.h = _VP()
+        self.n = n
+        _check(lib().fa_nmod_poly_mul_precomp_init(ctypes.byref(self.h), _p(b), len(b), btrunc,
+                                                   depth, nmod_init(n)))
+
+    def mul_mid(self, a, zl, zh):
+        """coefficients [zl, zh) of a b, or None when the window does not fit"""
+        a = _arr(a)
+        z = np.empty(max(zh - zl, 1), dtype=np.uint64)
+        rc = _check(lib().fa_nmod_poly_mul_mid_precomp(_p(z), zl, zh, _p(a), len(a), self.h,
+                                                       nmod_init(self.n)))
+        return z[: zh - zl] if rc == 1 else None
+
+    def close(self):
+        if self.h:
+            lib().fa_nmod_poly_mul_precomp_clear(self.h)
             self.h = _VP()
 
     def __del__(self):

@@ -248,6 +248,19 @@ FA_API int fa_nmod_poly_divrem_precomp_init(fa_divrem_precomp* M, const uint64_t
 FA_API int fa_nmod_poly_divrem_precomp(uint64_t* q, uint64_t* r, const uint64_t* a, int64_t an,
                                        fa_divrem_precomp M, fa_nmod_t mod);
 FA_API void fa_nmod_poly_divrem_precomp_clear(fa_divrem_precomp M);
+/* mul_precomp_struct (fft_small.h:796-822, nmod_poly_mul.c:1052-1140): b
+   (host) transformed once on a cyclic plan of 2^depth points, depth >= 4;
+   fa_nmod_poly_mul_mid_precomp writes coefficients [zl, zh) of a b to z
+   (host) and returns 1, or returns 0 without computing when the window
+   does not fit the transform (zn = an + bn - 1 > 2^depth and not zh <=
+   2^depth <= zn <= zl + 2^depth); a longer than 2^depth is folded mod
+   x^(2^depth) - 1.  Calls on one precomp from several threads are safe. */
+typedef struct fa_mul_precomp_s* fa_mul_precomp;
+FA_API int fa_nmod_poly_mul_precomp_init(fa_mul_precomp* M, const uint64_t* b, int64_t bn,
+                                         int64_t btrunc, int depth, fa_nmod_t mod);
+FA_API int fa_nmod_poly_mul_mid_precomp(uint64_t* z, uint64_t zl, uint64_t zh, const uint64_t* a,
+                                        int64_t an, fa_mul_precomp M, fa_nmod_t mod);
+FA_API void fa_nmod_poly_mul_precomp_clear(fa_mul_precomp M);
 
 /* _fmpz_poly_mul_mid_mpn_ctx, fmpz_poly_mul.c:319-401.  Coefficients are
    passed as fixed-width two's-complement limb vectors (aw, bw, zw limbs per

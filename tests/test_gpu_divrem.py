@@ -88,3 +88,44 @@ def test_divrem_rejects_non_invertible_leading_coefficient():
     b = np.array([1, 2, 6], dtype=np.uint64)          # 6 is not invertible mod 12
     with pytest.raises(fa.FlintAmdError):
         fa.nmod_poly_divrem(a, b, 12)
+
+
+@pytest.mark.parametrize("n", [65537, (1 << 61) - 1, 2**64 - 59])
+def test_mul_mid_precomp_windows(n):
+    """_mul_precomp_init / _nmod_poly_mul_mid_precomp: b transformed once on
+    2^depth points; windows of a b that fit the transform (zn <= N, or the
+    wraparound zh <= N <= zn <= zl + N, with a longer than N folded) against
+    the oracle's linear product, zero fill past zn, and 0 (nothing computed)
+    for a window the transform cannot hold"""
+    import flint_amd as fa
+    rng = np.random.default_rng(n % 1000)
+
+    def draw(m):
+        return np.array([int.from_bytes(rng.bytes(8), "little") % n for _ in range(m)],
+                        dtype=np.uint64)
+    for depth, bn in ((4, 5), (9, 300), (13, 5000), (15, 20000)):
+        N = 1 << depth
+        b = draw(bn)
+        M = fa.MulPrecomp(b, depth, n)
+        an = N - bn + 1                       # zn = N: linear
+        cases = [(an, 0, N, True), (an, N // 3, N // 3 + 7, True),
+                 (an, 1, N + 5, True)]        # zh past zn: zero filled
+        an = N - bn // 2 + 1                  # wraparound, zl = zn - N, zh = N
+        cases.append((an, an + bn - 1 - N, N, True))
+        an = N + (N - bn) // 2 + 1            # wraparound with a folded mod x^N - 1
+        cases.append((an, an + bn - 1 - N, N, True))
+        cases.append((2 * N + bn, 0, 10, False))     # does not fit
+        for an, zl, zh, fits in cases:
+            print(depth, bn, an, zl, zh, flush=True)
+            a = draw(an)
+            got = M.mul_mid(a, zl, zh)
+            if not fits:
+                assert got is None
+                continue
+            zn = an + bn - 1
+            want = np.zeros(zh - zl, dtype=np.uint64)
+            hi = min(zh, zn)
+            if zl < hi:
+                want[: hi - zl] = O.nmod_poly_mul_mid(a, b, zl, hi, n)
+            assert got is not None and np.array_equal(got, want), (depth, an, zl, zh)
+        M.close()
