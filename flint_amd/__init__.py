@@ -183,6 +183,9 @@ SIGNATURES = {
     "fa_nmod_poly_mul_mid_precomp": (ctypes.c_int, [_P64, _u64, _u64, _P64, _s64, _VP, NmodT]),
     "fa_nmod_poly_mul_precomp_clear": (None, [_VP]),
     "fa_mulmod_2expp1_ctx_init": (ctypes.c_int, [_VP, _s64, _s64]),
+    "fa_mulmod_satisfies_bounds": (ctypes.c_int, [_u64]),
+    "fa_sd_fft_ctx_point_mul": (ctypes.c_int, [_VP, _VP, _VP, _u64, _u64]),
+    "fa_sd_fft_ctx_point_sqr": (ctypes.c_int, [_VP, _VP, _u64, _u64]),
     "fa_mulmod_2expp1_ctx_clear": (None, [_VP]),
     "fa_mulmod_2expp1": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
     "fa_mulmod_2expp1_transformed_size": (_s64, [_VP]),
@@ -1022,3 +1025,18 @@ class SdFft:
         d = np.ascontiguousarray(d, dtype=np.float64).copy()
         _check(lib().fa_sd_ifft_trunc(ctypes.byref(self.ctx), d.ctypes.data_as(_PD), L, trunc))
         return d
+
+    def point_mul(self, a, b, m, depth):
+        """sd_fft_ctx_point_mul: a m b pointwise mod p, in [0, p)"""
+        a = np.ascontiguousarray(a, dtype=np.float64).copy()
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        _check(lib().fa_sd_fft_ctx_point_mul(ctypes.byref(self.ctx), a.ctypes.data_as(_PD),
+                                             b.ctypes.data_as(_PD), m, depth))
+        return a
+
+    def point_sqr(self, a, m, depth):
+        """sd_fft_ctx_point_sqr: a m a pointwise mod p, in [0, p)"""
+        a = np.ascontiguousarray(a, dtype=np.float64).copy()
+        _check(lib().fa_sd_fft_ctx_point_sqr(ctypes.byref(self.ctx), a.ctypes.data_as(_PD), m,
+                                             depth))
+        return a

@@ -113,7 +113,6 @@ int fa_mpn_mul_prealloc(uint64_t* z, const uint64_t* a, uint64_t an, const uint6
                         uint64_t bn, hipStream_t stream, void* buf, size_t cap);
 int fa_prime_info(uint64_t n, fa_prime_host* pr);
 int fa_build_prime_tables(const fa_prime_host* pr, int depth, double** w2, double** w2inv);
-int fa_mulmod_satisfies_bounds(uint64_t n);
 // keeps a superseded device table allocated until the process ends
 void fa_retire_device_table(void* p);
 fa_thread_state* fa_state();

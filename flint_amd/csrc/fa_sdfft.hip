@@ -35,8 +35,8 @@ std::mutex slot_mutex;
 slot_t slots[FA_SD_MAX_SLOTS];
 }  // namespace
 
-// mulmod_satisfies_bounds.c:120-150
-int fa_mulmod_satisfies_bounds(uint64_t nn)
+// mulmod_satisfies_bounds.c:120-150 (fft_small_mulmod_satisfies_bounds)
+extern "C" int fa_mulmod_satisfies_bounds(uint64_t nn)
 {
     double n = (double)nn;
     double ninv = 1.0 / n;
@@ -391,6 +391,58 @@ extern "C" int fa_sd_fft_trunc(fa_sd_fft_ctx_struct* Q, double* d, uint64_t L, u
     (void)it;
     permute16(d, L, N);
     return FA_OK;
+}
+
+// a[i] = a[i] m b[i] mod n, or a[i]^2 m when b is null, canonical in [0, n)
+__global__ void fa_k_sd_point(double* __restrict__ a, const double* __restrict__ b,
+                              uint64_t npts, double m, double n, double ninv)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    const double x = fa_mulmod(a[i], m, n, ninv);
+    double v = fa_reduce_pm1n(fa_mulmod(x, b ? b[i] : a[i], n, ninv), n, ninv);
+    a[i] = v < 0.0 ? v + n : v;
+}
+
+namespace {
+// sd_fft_ctx_point_mul / _sqr (mpn_mul.c:1392-1452) on host arrays of
+// 2^depth doubles: a = a m b (b null: a m a) pointwise, entries canonical
+int sd_point(fa_sd_fft_ctx_struct* Q, double* a, const double* b, uint64_t m, uint64_t depth)
+{
+    if (!a || depth < 4 || depth > FA_MAX_DEPTH)
+        return fa_set_error(FA_E_ARG, "sd_fft_ctx_point_mul: bad arguments");
+    sd_run R;
+    double* x;
+    int rc = prepare(Q, depth, &R, &x);
+    if (rc) return rc;
+    const uint64_t N = UINT64_C(1) << depth;
+    double* y = b ? R.save_base : nullptr;
+    if (hipMemcpyAsync(x, a, N * sizeof(double), hipMemcpyHostToDevice, R.s) != hipSuccess ||
+        (b && hipMemcpyAsync(y, b, N * sizeof(double), hipMemcpyHostToDevice, R.s) != hipSuccess))
+        return fa_set_error(FA_E_HIP, "upload failed");
+    const double n = R.S->pr.p;
+    const double md = fa_signed_rep(m % R.S->pr.n, R.S->pr.n);
+    FA_LAUNCH("fa_k_sd_point", fa_k_sd_point, dim3((unsigned)((N + 255) / 256)), dim3(256), 0,
+              R.s, x, y, N, md, n, R.S->pr.pinv);
+    if (hipGetLastError() != hipSuccess) return fa_set_error(FA_E_HIP, "point launch failed");
+    if (hipMemcpyAsync(a, x, N * sizeof(double), hipMemcpyDeviceToHost, R.s) != hipSuccess ||
+        hipStreamSynchronize(R.s) != hipSuccess)
+        return fa_set_error(FA_E_HIP, "download failed");
+    return FA_OK;
+}
+}  // namespace
+
+extern "C" int fa_sd_fft_ctx_point_mul(fa_sd_fft_ctx_struct* Q, double* a, const double* b,
+                                       uint64_t m, uint64_t depth)
+{
+    if (!b) return fa_set_error(FA_E_ARG, "sd_fft_ctx_point_mul: bad arguments");
+    return sd_point(Q, a, b, m, depth);
+}
+
+extern "C" int fa_sd_fft_ctx_point_sqr(fa_sd_fft_ctx_struct* Q, double* a, uint64_t m,
+                                       uint64_t depth)
+{
+    return sd_point(Q, a, nullptr, m, depth);
 }
 
 extern "C" int fa_sd_ifft_trunc(fa_sd_fft_ctx_struct* Q, double* d, uint64_t L, uint64_t trunc)

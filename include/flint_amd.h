@@ -427,6 +427,9 @@ typedef struct {
     int32_t slot;                      /* library-internal table handle */
 } fa_sd_fft_ctx_struct;
 
+/* fft_small_mulmod_satisfies_bounds (mulmod_satisfies_bounds.c:120-150):
+   1 when the FP64 mulmod's range claims hold for the odd modulus n */
+FA_API int fa_mulmod_satisfies_bounds(uint64_t n);
 /* sd_fft_ctx_init_prime, sd_fft_ctx.c:56-154 */
 FA_API int fa_sd_fft_ctx_init_prime(fa_sd_fft_ctx_struct* Q, uint64_t pp);
 /* sd_fft_ctx_clear, sd_fft_ctx.c:16-26 */
@@ -436,6 +439,13 @@ FA_API int fa_sd_fft_trunc(fa_sd_fft_ctx_struct* Q, double* d, uint64_t L,
                     uint64_t itrunc, uint64_t otrunc);
 /* sd_ifft_trunc, sd_ifft.c:1503-1533 (host data, in place) */
 FA_API int fa_sd_ifft_trunc(fa_sd_fft_ctx_struct* Q, double* d, uint64_t L, uint64_t trunc);
+/* sd_fft_ctx_point_mul / _point_sqr (mpn_mul.c:1392-1452) on host arrays
+   of 2^depth doubles (depth >= 4): a[i] = a[i] m b[i] (or a[i] m a[i]) mod
+   p, entries returned in [0, p) */
+FA_API int fa_sd_fft_ctx_point_mul(fa_sd_fft_ctx_struct* Q, double* a, const double* b,
+                                   uint64_t m, uint64_t depth);
+FA_API int fa_sd_fft_ctx_point_sqr(fa_sd_fft_ctx_struct* Q, double* a, uint64_t m,
+                                   uint64_t depth);
 /* sd_fft_ctx_w, fft_small.h:308-312 */
 FA_API double fa_sd_fft_ctx_w(const fa_sd_fft_ctx_struct* Q, uint64_t j);
 

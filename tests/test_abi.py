@@ -75,3 +75,13 @@ def test_errors_are_raised_not_swallowed():
         assert False, "no exception"
     except fa.FlintAmdError:
         pass
+
+
+def test_mulmod_bounds_check_matches_oracle():
+    """fft_small_mulmod_satisfies_bounds (host side) against the oracle's
+    restatement, on the default primes and moduli around 2^50"""
+    rng = np.random.default_rng(9)
+    cands = [int(p) for p in O.default_primes()] + [0x0003f00000000001, 257, 65537]
+    cands += [int(x) | 1 for x in rng.integers(1 << 40, 1 << 52, 200)]
+    for n in cands:
+        assert fa.lib().fa_mulmod_satisfies_bounds(n) == O.lib().or_mulmod_satisfies_bounds(n), n

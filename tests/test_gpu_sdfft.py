@@ -67,3 +67,33 @@ def test_sd_fft_matches_oracle(L):
         refb = O.ifft_trunc(p, ref, L, ot)
         # the inverse reads block-rounded truncations; compare the valid part
         assert np.array_equal(back[:ot_r].astype(np.uint64), refb[:ot_r])
+
+
+@pytest.mark.parametrize("p", [0x0003f00000000001, 257])
+def test_sd_point_mul_and_convolution(p):
+    """sd_fft_ctx_point_mul / _point_sqr: a m b and a m a mod p pointwise;
+    fft -> point_mul with m = 2^-L -> ifft is the product mod x^N - 1 of
+    the inputs, against schoolbook arithmetic"""
+    import flint_amd as fa
+    Q = fa.SdFft(p)
+    rnd = random.Random(p % 997)
+    for L in (4, 8, 11):
+        if p == 257 and L > 8:
+            continue
+        N = 1 << L
+        a = [rnd.randrange(p) for _ in range(N)]
+        b = [rnd.randrange(p) for _ in range(N)]
+        m = rnd.randrange(1, p)
+        got = Q.point_mul(np.array(a, dtype=np.float64), np.array(b, dtype=np.float64), m, L)
+        assert [int(x) for x in got] == [x * m * y % p for x, y in zip(a, b)]
+        got = Q.point_sqr(np.array(a, dtype=np.float64), m, L)
+        assert [int(x) for x in got] == [x * m * x % p for x in a]
+        A = Q.fft_trunc(np.array(a, dtype=np.float64), L, N, N)
+        B = Q.fft_trunc(np.array(b, dtype=np.float64), L, N, N)
+        C = Q.point_mul(A, B, pow(N, -1, p), L)
+        c = Q.ifft_trunc(C, L, N)
+        want = [0] * N
+        for i in range(N):
+            for j in range(N):
+                want[(i + j) % N] = (want[(i + j) % N] + a[i] * b[j]) % p
+        assert [int(x) % p for x in c] == want
