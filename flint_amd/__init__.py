@@ -184,6 +184,8 @@ SIGNATURES = {
     "fa_nmod_poly_mul_precomp_clear": (None, [_VP]),
     "fa_mulmod_2expp1_ctx_init": (ctypes.c_int, [_VP, _s64, _s64]),
     "fa_mulmod_satisfies_bounds": (ctypes.c_int, [_u64]),
+    "fa_nmod_direct_count": (_u64, []),
+    "fa_set_direct_fft": (None, [ctypes.c_int]),
     "fa_sd_fft_ctx_point_mul": (ctypes.c_int, [_VP, _VP, _VP, _u64, _u64]),
     "fa_sd_fft_ctx_point_sqr": (ctypes.c_int, [_VP, _VP, _u64, _u64]),
     "fa_mulmod_2expp1_ctx_clear": (None, [_VP]),
@@ -679,6 +681,12 @@ def nmod_poly_mul_mid(a, b, zl, zh, n):
     z = np.zeros(max(zh - zl, 1), dtype=np.uint64)
     _check(lib().fa_nmod_poly_mul_mid(_p(z), zl, zh, _p(a), len(a), _p(b), len(b), nmod_init(n)))
     return z[: max(zh - zl, 0)]
+
+
+def nmod_poly_mul_mid_device(z_ptr, zl, zh, a_ptr, an, b_ptr, bn, n, stream=None):
+    """_nmod_poly_mul_mid on device arrays, enqueued on `stream`"""
+    return _check(lib().fa_nmod_poly_mul_mid_device(z_ptr, zl, zh, a_ptr, an, b_ptr, bn,
+                                                    nmod_init(n), stream))
 
 
 def nmod_poly_mulmid_repack(a, b, znlo, zn, n):

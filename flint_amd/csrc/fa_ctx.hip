@@ -136,6 +136,8 @@ static void find_default_primes(uint64_t* out)
     }
 }
 
+int fa_is_prime_u64(uint64_t n) { return is_prime_u64(n); }
+
 // the primes are found once per process (the search runs Miller-Rabin on
 // each candidate; plans and CRT constants ask for them on every product)
 static void default_primes(uint64_t* out)

@@ -149,6 +149,7 @@ int fa_mpn_max_bits(uint64_t an, uint64_t bn, int np);
 void fa_prime_set_init(fa_prime_set* S, const fa_tables* T, int np, int depth, int offset,
                        int with_norm);
 uint64_t fa_mulmod_u64(uint64_t a, uint64_t b, uint64_t n);
+int fa_is_prime_u64(uint64_t n);                 // deterministic Miller-Rabin
 uint64_t fa_powmod_u64(uint64_t a, uint64_t e, uint64_t n);
 double fa_signed_rep(uint64_t x, uint64_t n);
 

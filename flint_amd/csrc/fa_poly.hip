@@ -14,6 +14,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
+#include <mutex>
+
 #include "fa_internal.h"
 #include "fa_ntt.h"
 
@@ -350,6 +353,107 @@ static int poly_buffers(fa_thread_state* st, uint64_t an_words, uint64_t bn_word
     return FA_OK;
 }
 
+// ------------------------------------------------------------------------
+// direct transforms mod the modulus (plan.c:319-349): when n is itself an
+// FFT prime whose 2-valuation covers the transform, the product is
+// transformed mod n alone and needs no CRT (one transform set where the CRT
+// needs two or more).  n's twiddle tables are built on first use and kept
+// in a small per-device cache; a lease holds the cache lock from lookup to
+// the last launch that reads the tables, and an eviction synchronizes the
+// device before freeing them.  Not taken while the stream is captured (the
+// tables could be evicted under a recorded graph).
+// ------------------------------------------------------------------------
+namespace {
+struct direct_entry {
+    int device = -1;
+    uint64_t n = 0, used = 0;
+    fa_tables T;
+};
+std::mutex direct_mu;
+direct_entry direct_cache[4];
+uint64_t direct_clock = 0;
+std::atomic<uint64_t> direct_count{0};
+
+struct direct_lease {
+    std::unique_lock<std::mutex> lock;
+    const fa_tables* T = nullptr;
+    fa_crt_consts C;
+};
+
+int direct_enabled = 1;
+
+bool direct_applies(uint64_t n, int depth, int crt_np)
+{
+    if (!direct_enabled || crt_np < 2 || n < (UINT64_C(1) << 20) || n > (UINT64_C(1) << 50) || !(n & 1)) return false;
+    const int v = __builtin_ctzll(n - 1);
+    const int tdepth = depth < 20 ? 20 : depth;
+    if (v < tdepth) return false;
+    return fa_mulmod_satisfies_bounds(n) && fa_is_prime_u64(n);
+}
+
+bool direct_acquire(uint64_t n, int depth, int crt_np, hipStream_t s, direct_lease* L)
+{
+    if (!direct_applies(n, depth, crt_np)) return false;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+        return false;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    const int tdepth = depth < 20 ? 20 : depth;
+    L->lock = std::unique_lock<std::mutex>(direct_mu);
+    direct_entry* e = nullptr;
+    for (auto& c : direct_cache)
+        if (c.device == dev && c.n == n && c.T.depth >= tdepth) e = &c;
+    if (!e) {
+        // the least recently used slot (an empty one first)
+        e = &direct_cache[0];
+        for (auto& c : direct_cache)
+            if (c.device < 0 || (e->device >= 0 && c.used < e->used)) e = &c;
+        if (e->device >= 0) {
+            int cur = dev;
+            (void)hipSetDevice(e->device);
+            (void)hipDeviceSynchronize();
+            (void)hipFree(e->T.w2[0]);
+            (void)hipFree(e->T.w2inv[0]);
+            (void)hipSetDevice(cur);
+            e->device = -1;
+        }
+        fa_tables T;
+        memset(&T, 0, sizeof(T));
+        if (fa_prime_info(n, &T.pr[0]) != FA_OK ||
+            fa_build_prime_tables(&T.pr[0], tdepth, &T.w2[0], &T.w2inv[0]) != FA_OK) {
+            (void)hipGetLastError();
+            L->lock.unlock();
+            return false;
+        }
+        T.device = dev;
+        T.depth = tdepth;
+        e->T = T;
+        e->device = dev;
+        e->n = n;
+    }
+    e->used = ++direct_clock;
+    L->T = &e->T;
+    memset(&L->C, 0, sizeof(L->C));
+    L->C.np = 1;
+    L->C.nl = 1;
+    L->C.p[0] = (double)n;
+    L->C.pinv[0] = 1.0 / (double)n;
+    L->C.rp[0] = 1.0 / (double)n;
+    L->C.M[0] = n;
+    L->C.cof[0][0] = 1;
+    L->C.gcm[0] = 1.0;
+    L->C.ginv[0] = 1.0;
+    direct_count++;
+    return true;
+}
+}  // namespace
+
+// products computed with direct transforms so far (a counter for tests)
+extern "C" uint64_t fa_nmod_direct_count(void) { return direct_count.load(); }
+// 0: always the CRT primes (comparisons of the two plans), 1: direct where it applies
+extern "C" void fa_set_direct_fft(int on) { direct_enabled = on ? 1 : 0; }
+
 #define HIPTRY(x, msg)                                                                  \
     do {                                                                                \
         if ((x) != hipSuccess) return fa_set_error(FA_E_HIP, "%s: %s", msg,             \
@@ -552,6 +656,13 @@ extern "C" int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh, const u
                                    (uint64_t)(an > bn ? an : bn));
     if (const uint64_t lim = poly_block_limit((uint64_t)zn))
         return poly_mul_blocked(0, z, 1, zl, zh, a, 1, an, b, 1, bn, n, J.np, J.offset, J.C, lim);
+    direct_lease D;
+    if (direct_acquire(n, J.depth, J.np, st->stream, &D)) {
+        T = const_cast<fa_tables*>(D.T);
+        J.np = 1;
+        J.offset = 0;
+        J.C = D.C;
+    }
 
     uint64_t cnt = (uint64_t)(zh - zl);
     uint64_t *da, *db, *dz;
@@ -578,6 +689,7 @@ extern "C" int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh, const u
     FA_CRT_NP_SWITCH(J.C.np, FA_L)
 #undef FA_L
     HIPTRY(hipGetLastError(), "crt launch");
+    if (D.lock.owns_lock()) D.lock.unlock();     // the tables' last reader is enqueued
     HIPTRY(hipMemcpyAsync(z, dz, cnt * sizeof(uint64_t), hipMemcpyDeviceToHost, s), "download");
     HIPTRY(hipStreamSynchronize(s), "synchronize");
     return FA_OK;
@@ -629,6 +741,13 @@ extern "C" int fa_nmod_poly_mul_mid_device(uint64_t* z, int64_t zl, int64_t zh,
     if (rc) return rc;
     fa_crt_consts C;
     fa_crt_consts_init(&C, np, offset);
+    direct_lease D;
+    if (direct_acquire(n, depth, np, s, &D)) {
+        T = const_cast<fa_tables*>(D.T);
+        np = 1;
+        offset = 0;
+        C = D.C;
+    }
     const size_t tr = (size_t)np * fa_res_stride(depth) * sizeof(double);
     void* buf;
     rc = fa_scratch_reserve(st, 2 * tr + 256, &buf, s);

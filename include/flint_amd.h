@@ -191,6 +191,13 @@ FA_API int fa_mpn_mul_device_plan(uint64_t* z, const uint64_t* a, int64_t an,
 /* plan the device call would use: out = {np, bits, depth, d1, d2, zlen} */
 FA_API int fa_mpn_plan(int64_t an, int64_t bn, int64_t* out);
 
+/* nmod products computed with direct transforms mod the modulus so far
+   (plan.c:319-349: n an FFT prime of 20..50 bits whose 2-valuation covers
+   the transform, where the CRT would need two or more primes; the outputs
+   are the same either way) */
+FA_API uint64_t fa_nmod_direct_count(void);
+/* 1 (default): direct transforms where they apply; 0: always the CRT primes */
+FA_API void fa_set_direct_fft(int on);
 /* _nmod_poly_mul_mid_mpn_ctx, nmod_poly_mul.c:904-972 */
 FA_API int fa_nmod_poly_mul_mid(uint64_t* z, int64_t zl, int64_t zh,
                          const uint64_t* a, int64_t an,

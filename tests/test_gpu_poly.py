@@ -189,3 +189,65 @@ def test_nmod_beyond_one_transform():
     s = want[h:] + hi
     want[h:] = np.where(s >= n, s - np.uint64(n), s)
     assert np.array_equal(z, want)
+
+
+def _fft_prime(bits, v, rng):
+    """a prime k 2^v + 1 of the given bit length"""
+    while True:
+        k = int(rng.integers(1 << (bits - v - 1), 1 << (bits - v)))
+        n = k * (1 << v) + 1
+        if n.bit_length() == bits and O.lib().or_is_prime(n):
+            return n
+
+
+def test_nmod_direct_fft_moduli():
+    """plan.c:319-349: an FFT prime modulus whose 2-valuation covers the
+    transform is transformed directly (one prime, no CRT).  Products and
+    windows (wraparound included) for such moduli of 23 to 50 bits against
+    the oracle, a 2^21-coefficient product by evaluation at random points,
+    the device entry, and the counter that shows which path ran; moduli
+    that are not FFT primes keep the CRT path"""
+    import torch
+    import flint_amd as fa
+    rng = np.random.default_rng(41)
+    moduli = [998244353, 469762049, 7340033, _fft_prime(40, 24, rng), _fft_prime(50, 22, rng)]
+
+    def draw(n, k):
+        return np.array([int.from_bytes(rng.bytes(8), "little") % n for _ in range(k)],
+                        dtype=np.uint64)
+    for n in moduli:
+        for an, bn, zl, zh in ((3000, 2500, 0, 5499), (40000, 30000, 17, 69999),
+                               (70000, 3000, 20000, 72999), (300000, 200000, 123456, 123999)):
+            a, b = draw(n, an), draw(n, bn)
+            before = fa.lib().fa_nmod_direct_count()
+            print(n, an, bn, zl, zh, flush=True)
+            z = fa.nmod_poly_mul_mid(a, b, zl, zh, n)
+            assert fa.lib().fa_nmod_direct_count() == before + 1, n
+            assert np.array_equal(z, O.nmod_poly_mul_mid(a, b, zl, zh, n)), (n, an, bn)
+    # a long product (depth 21) checked by evaluation
+    n = 998244353
+    an, bn = 1 << 20, (1 << 20) - 5
+    a, b = draw(n, an), draw(n, bn)
+    z = fa.nmod_poly_mul_mid(a, b, 0, an + bn - 1, n)
+    for x in (3, 123456789, int(rng.integers(2, n))):
+        assert O.nmod_poly_eval(z, x, n) == O.nmod_poly_eval(a, x, n) * O.nmod_poly_eval(b, x, n) % n
+    # the device entry on the caller's stream
+    a, b = draw(n, 5000), draw(n, 4000)
+    da = torch.from_numpy(a.view(np.int64)).to("cuda:0")
+    db = torch.from_numpy(b.view(np.int64)).to("cuda:0")
+    dz = torch.zeros(8999, dtype=torch.int64, device="cuda:0")
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    before = fa.lib().fa_nmod_direct_count()
+    fa.nmod_poly_mul_mid_device(dz.data_ptr(), 0, 8999, da.data_ptr(), 5000, db.data_ptr(), 4000,
+                                n, s.cuda_stream)
+    s.synchronize()
+    assert fa.lib().fa_nmod_direct_count() == before + 1
+    assert np.array_equal(dz.cpu().numpy().view(np.uint64), O.nmod_poly_mul_mid(a, b, 0, 8999, n))
+    # not FFT primes (or too small a 2-valuation): the CRT path
+    for n in ((1 << 61) - 1, 1000003, 3 * (1 << 10) + 1):
+        a, b = draw(n, 3000), draw(n, 2000)
+        before = fa.lib().fa_nmod_direct_count()
+        z = fa.nmod_poly_mul_mid(a, b, 0, 4999, n)
+        assert fa.lib().fa_nmod_direct_count() == before
+        assert np.array_equal(z, O.nmod_poly_mul_mid(a, b, 0, 4999, n))
