@@ -60,6 +60,23 @@ def test_mpn_windows_match_reference_words():
     assert lowered > 5      # two-prime windows drop chunk tails' carries often
 
 
+def test_mpn_window_edges():
+    """window edges against the oracle: one limb at either end, everything
+    above limb 1, and windows starting just below, at and above the limb
+    count n of the wide plans' coefficients (c_lo = 0 up to lo = n)"""
+    import flint_amd as fa
+    rng = np.random.default_rng(99)
+    for an, bn in ((7, 3), (3, 7), (1000, 999), (39999, 40000), (50000, 7), (7, 50000),
+                   (60000, 41000)):
+        a, b = rand_limbs(rng, an), rand_limbs(rng, bn)
+        zn = an + bn
+        wins = [(0, 1), (zn - 1, zn), (1, zn), (zn // 2, zn // 2 + 1)]
+        wins += [(lo, min(lo + 3, zn)) for lo in range(2, 10) if lo < zn]
+        for lo, hi in wins:
+            z = fa.mpn_mul_range(a, b, lo, hi)
+            assert np.array_equal(z, O.mpn_mul_range(a, b, lo, hi)), (an, bn, lo, hi)
+
+
 def test_mpn_window_planted_dropped_carry():
     """wide plan (an > 40000), b = 2^64 - 1: a's bits in [c_lo bits, 64 lo)
     make the coefficients from c_lo up sum to T - 2^(c_lo bits) below T, so
@@ -118,7 +135,7 @@ def test_mpn_window_thread_counts_and_exact():
         torch.cuda.synchronize()
         s = torch.cuda.Stream(device=dev)
         for lo, hi in ((an // 3, an // 3 + 50), (an - 2, an + bn)):
-            for nt in (0, 1, 2, 8):
+            for nt in (0, 1, 2, 3, 5, 6, 7, 8):
                 print(an, bn, lo, hi, nt, flush=True)
                 dz = torch.zeros(hi - lo, dtype=torch.int64, device=dev)
                 torch.cuda.synchronize()
