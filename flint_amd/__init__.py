@@ -113,6 +113,8 @@ SIGNATURES = {
     "fa_mpn_mul_range_device": (ctypes.c_int, [_VP, _u64, _u64, _VP, _s64, _VP, _s64, _VP]),
     "fa_mpn_mul_range_threads": (ctypes.c_int, [_VP, _u64, _u64, _VP, _s64, _VP, _s64,
                                                 ctypes.c_int, _VP]),
+    "fa_mpn_mul_range_host": (ctypes.c_int, [_P64, _u64, _u64, _P64, _s64, _P64, _s64,
+                                             ctypes.c_int]),
     "fa_mpn_window_plan": (ctypes.c_int, [_u64, _u64, _u64, ctypes.c_int, _P64]),
     "fa_mpn_plan": (ctypes.c_int, [_s64, _s64, _P64]),
     "fa_mpn_mul_plan": (ctypes.c_int, [_P64, _P64, _s64, _P64, _s64, ctypes.c_int, ctypes.c_int]),
@@ -397,13 +399,18 @@ def mpn_mul(a, b):
     return z
 
 
-def mpn_mul_range(a, b, lo, hi):
+def mpn_mul_range(a, b, lo, hi, nthreads=None):
     """limbs [lo, hi) of a*b as _mpn_ctx_mpn_mul_range returns them
-    (flint_mpn_mulmid_fft_small; FLINT on one thread)"""
+    (flint_mpn_mulmid_fft_small) with FLINT on one thread, or on nthreads
+    threads (0: the exact window)"""
     a = _arr(a)
     b = _arr(b)
     z = np.empty(hi - lo, dtype=np.uint64)
-    _check(lib().fa_mpn_mul_range(_p(z), lo, hi, _p(a), len(a), _p(b), len(b)))
+    if nthreads is None:
+        _check(lib().fa_mpn_mul_range(_p(z), lo, hi, _p(a), len(a), _p(b), len(b)))
+    else:
+        _check(lib().fa_mpn_mul_range_host(_p(z), lo, hi, _p(a), len(a), _p(b), len(b),
+                                           nthreads))
     return z
 
 

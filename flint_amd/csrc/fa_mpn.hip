@@ -1898,15 +1898,18 @@ static int mpn_window_device(uint64_t* z, uint64_t lo, uint64_t hi, const uint64
     return fa_buffers_leave(st, s);
 }
 
-extern "C" int fa_mpn_mul_range(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a,
-                                int64_t an, const uint64_t* b, int64_t bn)
+// host operands and output; the words of FLINT on nthreads threads (0: the
+// exact window)
+extern "C" int fa_mpn_mul_range_host(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a,
+                                     int64_t an, const uint64_t* b, int64_t bn, int nthreads)
 {
-    if (an <= 0 || bn <= 0 || !a || !b || lo >= hi || hi > (uint64_t)(an + bn) || !z)
+    if (an <= 0 || bn <= 0 || !a || !b || lo >= hi || hi > (uint64_t)(an + bn) || !z ||
+        nthreads < 0)
         return fa_set_error(FA_E_ARG, "fa_mpn_mul_range: bad arguments");
     fa_thread_state* st = fa_state();
     if (!st) return FA_E_NODEV;
     const uint64_t zn = (uint64_t)(an + bn);
-    const size_t wb = fa_window_work_bytes((uint64_t)an, (uint64_t)bn, lo, 1);
+    const size_t wb = fa_window_work_bytes((uint64_t)an, (uint64_t)bn, lo, nthreads);
     void* hb;
     int rc = fa_hbuf_reserve(st, ((size_t)an + (size_t)bn + zn) * sizeof(uint64_t) + wb, &hb,
                              st->stream);
@@ -1922,8 +1925,8 @@ extern "C" int fa_mpn_mul_range(uint64_t* z, uint64_t lo, uint64_t hi, const uin
         return fa_set_error(FA_E_HIP, "operand upload failed");
     rc = mpn_mul_device(dz, da, (uint64_t)an, db, (uint64_t)bn, st->stream);
     if (rc == FA_OK && wb)
-        rc = fa_window_fix(dz + lo, dz, lo, hi, da, (uint64_t)an, db, (uint64_t)bn, 1, dz + zn,
-                           st->stream);
+        rc = fa_window_fix(dz + lo, dz, lo, hi, da, (uint64_t)an, db, (uint64_t)bn, nthreads,
+                           dz + zn, st->stream);
     if (rc) {
         (void)hipStreamSynchronize(st->stream);
         return rc;
@@ -1935,6 +1938,12 @@ extern "C" int fa_mpn_mul_range(uint64_t* z, uint64_t lo, uint64_t hi, const uin
         return fa_set_error(FA_E_HIP, "stream synchronize failed: %s",
                             hipGetErrorString(hipGetLastError()));
     return FA_OK;
+}
+
+extern "C" int fa_mpn_mul_range(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a,
+                                int64_t an, const uint64_t* b, int64_t bn)
+{
+    return fa_mpn_mul_range_host(z, lo, hi, a, an, b, bn, 1);
 }
 
 // the same on device operands, enqueued on `stream`

@@ -137,9 +137,12 @@ FA_API int fa_mpn_mul(uint64_t* z, const uint64_t* a, int64_t an,
    most one carry, doc/source/mpn_extras.rst:221-231). */
 FA_API int fa_mpn_mul_range(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a,
                             int64_t an, const uint64_t* b, int64_t bn);
-/* the same on device operands and output, enqueued on `stream`, with the
-   words of FLINT running nthreads threads (its plan, mpn_mul.c:1248-1340,
-   depends on the thread count); nthreads = 0 gives the exact window */
+/* fa_mpn_mul_range with the words of FLINT running nthreads threads (its
+   plan, mpn_mul.c:1248-1340, depends on the thread count: pass
+   flint_get_num_threads()); nthreads = 0 gives the exact window.  _host:
+   host operands and output; _threads: device ones, enqueued on `stream` */
+FA_API int fa_mpn_mul_range_host(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a,
+                                 int64_t an, const uint64_t* b, int64_t bn, int nthreads);
 FA_API int fa_mpn_mul_range_threads(uint64_t* z, uint64_t lo, uint64_t hi, const uint64_t* a,
                                     int64_t an, const uint64_t* b, int64_t bn, int nthreads,
                                     void* stream);

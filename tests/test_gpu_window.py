@@ -145,6 +145,8 @@ def test_mpn_window_thread_counts_and_exact():
                 got = dz.cpu().numpy().view(np.uint64)
                 want = full[lo:hi] if nt == 0 else O.mpn_mul_range(a, b, lo, hi, nt)
                 assert np.array_equal(got, want), (an, bn, lo, hi, nt)
+                # the host-pointer entry with the same thread count
+                assert np.array_equal(fa.mpn_mul_range(a, b, lo, hi, nt), want)
 
 
 def test_mpn_middle_product_device():
