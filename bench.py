@@ -19,8 +19,8 @@ The CPU baseline describes the host (CPU model, core count), times FLINT's
 own mpn_mul_default_mpn_ctx at 1 and nproc threads when a system libflint is
 present, and otherwise GMP's mpn_mul and the oracle (oracle/fsmall_oracle.c,
 a scalar C restatement of the reference) on the same inputs, each checked
-word for word against the GPU product; FLINT's own times from the survey
-host are carried beside them, labelled as another host.
+word for word against the GPU product; FLINT's own times measured on
+another host are carried beside them, labelled as such.
 """
 import argparse
 import json
@@ -101,10 +101,10 @@ KERNEL_FP64_OPS = {
     "fa_k_col_inv16": 12 * 4.5,
 }
 
-# FLINT's own CPU path on the same configurations, measured in the survey
-# container (Intel Xeon, Sapphire Rapids flags, 8 vCPU; FLINT 3.7.0-dev,
+# FLINT's own CPU path on the same configurations, measured on another host
+# (Intel Xeon, Sapphire Rapids flags, 8 vCPU; FLINT 3.7.0-dev,
 # GMP 6.2.1): not this host
-FLINT_SURVEY_HOST = {
+FLINT_OTHER_HOST = {
     "mpn_2p24": {"1_thread_s": 3.13, "8_threads_s": 0.694},
     "mpn_65536": {"1_thread_s": 0.01010, "8_threads_s": 0.00210},
 }
@@ -158,10 +158,10 @@ def _median_time(fn, reps):
 
 
 def cpu_baseline(W, a_h, b_h, z_gpu, budget_s=30.0):
-    """host description; FLINT's CPU path if a system
+    """The CPU legs: host description; FLINT's CPU path if a system
     libflint is present (1 and nproc threads); otherwise GMP's mpn_mul and the
     oracle (the scalar C restatement), labelled, on the same inputs; and
-    FLINT's own times from the survey host, labelled as another host"""
+    FLINT's own times from another host, labelled as such"""
     import ctypes
     info = host_info()
     an, bn = W["an"], W["bn"]
@@ -224,12 +224,12 @@ def cpu_baseline(W, a_h, b_h, z_gpu, budget_s=30.0):
         legs["oracle_1_thread"] = {"s": t_or, "value": lp / t_or, "threads": 1,
                                    "sample": "2^16 x 2^16 took %.3f s, scaled by n log n (x%.1f)"
                                              % (t_small, scale)}
-    ref = FLINT_SURVEY_HOST.get(W.get("key"))
+    ref = FLINT_OTHER_HOST.get(W.get("key"))
     if ref:
         res["flint_same_code_different_host"] = {
             k: {"s": v, "value": lp / v} for k, v in ref.items()}
         res["flint_same_code_different_host"]["host"] = (
-            "survey container: Intel Xeon (Sapphire Rapids flags), 8 vCPU, FLINT 3.7.0-dev, "
+            "Intel Xeon (Sapphire Rapids flags), 8 vCPU, FLINT 3.7.0-dev, "
             "GMP 6.2.1")
     res["legs"] = legs
     if flint:

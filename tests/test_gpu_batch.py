@@ -115,7 +115,7 @@ def test_batch_mixed_matches_single_calls():
             assert to_int(got[k][1]) == to_int(q["a"]) * to_int(q["b"])
 
 
-def test_batch_nmod_stream_like_config3():
+def test_batch_nmod_shaped_like_the_batched_workload():
     """a draw shaped like the batched nmod workload (lengths log-uniform in
     [2^6, 2^12], prime moduli of 20..63 bits) against the oracle"""
     import flint_amd as fa

@@ -42,7 +42,7 @@ def test_bn_bounds_match_reference_table():
 
 
 def test_prime_products_log2():
-    # survey table: log2 prod p_0..p_(k-1)
+    # log2 of p_0 ... p_(k-1) for the default primes, to two decimals
     import math
     expect = [49.98, 99.21, 149.15, 198.85, 248.16, 298.02, 347.87, 397.50]
     acc = 1
@@ -195,8 +195,9 @@ def test_mpn_to_residues_closed_form():
                 assert int(d[i]) == ((A >> (i * bits)) & ((1 << bits) - 1)) % p
 
 
-def test_plans_match_survey_table():
-    """mpn_ctx_best_profile / _fft_small_np2_choose at the configs"""
+def test_plans_at_the_benchmark_shapes():
+    """mpn_ctx_best_profile / _fft_small_np2_choose at the benchmark shapes
+    (2^16 and 2^24 limbs on 1 and 8 threads) and at the two-prime limit"""
     assert O.mpn_plan(65536, 65536, 1) == dict(np=6, bits=140, depth=16, ztrunc=60160,
                                                alen=29960, blen=29960)
     assert O.mpn_plan(65536, 65536, 8)["np"] == 8
