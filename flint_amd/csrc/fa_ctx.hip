@@ -189,7 +189,7 @@ static int check_device(int dev)
     hipDeviceProp_t prop;
     HIPCHECK(hipGetDeviceProperties(&prop, dev));
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-        return fa_set_error(FA_E_NODEV, "device %d is %s, this build targets gfx950", dev,
+        return fa_set_error(FA_E_NODEV, "device %d is %s, this library targets gfx950", dev,
                             prop.gcnArchName);
     return FA_OK;
 }

@@ -11,7 +11,7 @@
 // mod x^(2^depth) - 1 first (_mod_red, nmod_poly_mul.c:33-168).  Otherwise
 // the call returns 0 and computes nothing, as the reference does.  (Below
 // 256 points the reference's fit also accepts a linear window longer than
-// the transform, whose coefficients alias; this build returns 0 there.)
+// the transform, whose coefficients alias; this library returns 0 there.)
 #include <string.h>
 
 #include "fa_internal.h"

@@ -9,7 +9,7 @@
 // exported by the CRT of the product path, so transforms of an operand that
 // enters many products are computed once, and sums of products are
 // accumulated in transform space before one inverse transform and one CRT.
-// The layout is this build's own; ops are not interchangeable with the
+// The layout is this library's own; ops are not interchangeable with the
 // reference's host buffers.
 //
 // Every function here enqueues on the given stream (NULL: the calling

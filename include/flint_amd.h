@@ -307,7 +307,7 @@ typedef struct {
 #define FA_OP_COEFF 2        /* after fa_ifft: ready for export */
 
 /* A transformed operand in device memory: np arrays of `stride` doubles in
-   this build's own layout (fft_small_op_struct, fft_small.h:620-640). */
+   this library's own layout (fft_small_op_struct, fft_small.h:620-640). */
 typedef struct {
     double* data;
     uint64_t stride, np, offset, depth;
