@@ -248,6 +248,26 @@ def test_crt_variants_agree(an, bn, np_, bits):
     assert (to_int(a) % q) * (to_int(b) % q) % q == to_int(z1) % q
 
 
+def test_every_plan_of_the_2p24_product_agrees():
+    """the four valid plans of 2^24 x 2^24 limbs (6 primes at 136 and 128
+    bits, 7 at 160, 8 at 184; every one a full 2^24-point transform) give
+    the same product, which matches the residue check mod 2^61 - 1"""
+    import flint_amd as fa
+    n = 1 << 24
+    a, b = fa.synth_mpn_pair(0x2424, n, n)
+    q = (1 << 61) - 1
+    want = (to_int(a) % q) * (to_int(b) % q) % q
+    z0 = None
+    for np_, bits in ((6, 136), (6, 128), (7, 160), (8, 184)):
+        print("plan-agree", np_, bits, flush=True)
+        z = fa.mpn_mul_plan(a, b, np_, bits)
+        if z0 is None:
+            z0 = z
+            assert to_int(z) % q == want
+        else:
+            assert np.array_equal(z, z0), (np_, bits)
+
+
 @pytest.mark.parametrize("an,bn,cap", [(2000, 1500, 3000), (5000, 300, 3000), (4096, 4096, 3000),
                                        (7000, 2000, 2500)])
 def test_blocked_product_exact(an, bn, cap):
