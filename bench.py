@@ -5,7 +5,8 @@ Workload (N=1): plain mpn_mul of two naturals of 2^24 64-bit limbs each,
 uniform random limbs from the seeded generator (seed 0xF1A70002, top limb
 nonzero), one product per call, operands and product resident in HBM.
 A "step" is one full product.  The metric is limb-products/s = an*bn per
-second.  After the timed region the product is checked against
+second.  By default 30 untimed products come first: back-to-back products
+of this size take about 25 ms to reach their steady rate.  After the timed region the product is checked against
 (a mod q)(b mod q) for three 61-bit primes q.
 
 The roofline entry is for the dominant kernel: algorithmic bytes per launch
@@ -553,8 +554,12 @@ def full_stream_main(args, torch, fa):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 30 for the plain products, 10 for batches)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps first (default 30 for the plain products, 2 for "
+                         "batches): the first 20-30 ms of back-to-back 2^24-limb products "
+                         "run about 5%% slower than the steady state")
     ap.add_argument("--workload", default="mpn_2p24",
                     choices=sorted(WORKLOADS) + ["nmod4096", "fmpz256", "stream",
                                                  "stream-full"])
@@ -575,6 +580,11 @@ def main():
                     help="only run the library's plain copy kernel over MIB MiB, 5 times (the "
                          "known-size reference for scaling HBM counter readings), and exit")
     args = ap.parse_args()
+    plain = args.workload in WORKLOADS
+    if args.steps is None:
+        args.steps = 30 if plain else 10
+    if args.warmup is None:
+        args.warmup = 30 if plain else 2
 
     if args.gpus != 1:
         print("bench.py: runs on one GPU; --gpus %d is not supported (sharding the primes "
