@@ -557,9 +557,9 @@ def main():
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default 30 for the plain products, 10 for batches)")
     ap.add_argument("--warmup", type=int, default=None,
-                    help="untimed steps first (default 30 for the plain products, 2 for "
-                         "batches): the first 20-30 ms of back-to-back 2^24-limb products "
-                         "run about 5%% slower than the steady state")
+                    help="untimed steps first (default 30 for the plain products, 10 for "
+                         "batches): the first 20-30 ms of back-to-back work run about 5%% "
+                         "slower than the steady state")
     ap.add_argument("--workload", default="mpn_2p24",
                     choices=sorted(WORKLOADS) + ["nmod4096", "fmpz256", "stream",
                                                  "stream-full"])
@@ -584,7 +584,7 @@ def main():
     if args.steps is None:
         args.steps = 30 if plain else 10
     if args.warmup is None:
-        args.warmup = 30 if plain else 2
+        args.warmup = 30 if plain else 10
 
     if args.gpus != 1:
         print("bench.py: runs on one GPU; --gpus %d is not supported (sharding the primes "
